@@ -1,0 +1,157 @@
+/*
+ * prism_amd.h -- C ABI of the MI355X (gfx950) PRISM compress / progressive
+ * retrieve library (libprism_amd.so).
+ *
+ * One entry point per stage function of the reference's pipeline; each
+ * comment names the reference interface it replaces.  All pointers named
+ * d_* are device pointers; `stream` is a hipStream_t passed as void* (NULL =
+ * the null stream).  Every call is stream-ordered and returns an int status
+ * (PA_OK or a negative PA_ERR_*); no call synchronises the host unless its
+ * comment says so.  Buffers are owned by the caller: scratch memory comes in
+ * through a workspace argument sized by the matching *_workspace() query.
+ *
+ * Data are T[nz][ny][nx] with x fastest; dtype is PA_F32 or PA_F64 (the
+ * reference's prism_dtype values F4 / F8, include/utils/traits.hpp:81-95).
+ * flags8 = {use_md[0..3], reverse[0..3]} indexed by level_id (0 = unit 1).
+ * begin/end are int32[4] indexed by lr = 3 - level_id, like the reference.
+ */
+#ifndef PRISM_AMD_H
+#define PRISM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PA_OK 0
+#define PA_ERR_ARG (-1)       /* invalid argument or unsupported shape */
+#define PA_ERR_HIP (-2)       /* a HIP runtime call failed */
+#define PA_ERR_WORKSPACE (-3) /* workspace too small */
+#define PA_ERR_STREAM (-4)    /* malformed compressed stream */
+#define PA_ERR_CAPACITY (-5)  /* output buffer too small */
+
+#define PA_F32 4
+#define PA_F64 8
+#define PA_MODE_NB 0 /* negabinary bit-planes: progressive streams ("-prog") */
+#define PA_MODE_SM 1 /* sign-magnitude bit-planes: plain streams (default) */
+
+/* Geometry and stream layout of one field.  Same content as the sizes the
+ * reference derives in Bitplane::h_comput (io.hpp:104-132), d_comput
+ * (bitplane.cu:10-53), pre_compute (InterpolationPredictor.cu:969-995) and
+ * lossless_encode (zbpc.cu:478-481). */
+typedef struct pa_layout {
+    uint32_t nx, ny, nz;
+    int64_t n, n4;
+    int64_t lx[5], lxy[5];
+    int64_t cnt[4];
+    int64_t prefix[5];
+    int64_t dev_align, real_end, code_len;
+    int64_t dev_len[4], dev_stride[4], dev_base[4], dev_size;
+    int64_t host_stride[4], host_base[4], host_size;
+    int64_t chunks_per_plane[4], nchunks;
+    int64_t trailer_fixed;
+} pa_layout;
+
+/* Bitplane::Bitplane + calculate_aligned_buffer_size (io.hpp:86-92,
+ * io.cu:283-302) and Compressor::init sizes (compressor.cu:77-117). Host only. */
+int pa_layout_init(pa_layout* L, uint32_t nx, uint32_t ny, uint32_t nz);
+/* upper bound of a compressed stream (payload + pad + trailer) in bytes */
+int64_t pa_compress_bound(const pa_layout* L, int dtype);
+
+/* StatBuffer::findMinMax -> extrema_scan (compressor.hpp:23-31,
+ * compressor.cu:228-302).  d_minmax: double[4]; [0] = min and [1] = max on
+ * return, [2..3] are scratch. */
+int pa_extrema(const void* d_data, int dtype, int64_t n, double* d_minmax, void* stream);
+
+/* Auto-tuning part of spline_construct (InterpolationPredictor.cu:2273-2355,
+ * kernel auto_tuning_interpolation 880-967).  Scores the 15 candidates with a
+ * fixed-order sum and picks flags on the device.  d_scores15 (double[15]) may
+ * be NULL. */
+size_t pa_autotune_workspace(const pa_layout* L);
+int pa_autotune(const void* d_data, int dtype, const pa_layout* L, double* d_scores15,
+                uint8_t* d_flags8, void* d_ws, size_t ws_bytes, void* stream);
+
+/* Prediction part of spline_construct (kernel interpolation,
+ * InterpolationPredictor.cu:2137-2173): quantisation codes in level-major
+ * order (d_qc: L->code_len int32, fully written) and the anchors (n4 T). */
+int pa_spline_construct(const void* d_data, int dtype, const pa_layout* L, double eb,
+                        double rel_eb, const uint8_t* d_flags8, int32_t* d_qc, void* d_anchors,
+                        void* stream);
+
+/* spline_reconstruct (InterpolationPredictor.cu:2372-2421).  d_maxlevel is a
+ * device int (levels with level_id > *d_maxlevel are skipped) or NULL for 4. */
+int pa_spline_reconstruct(const int32_t* d_qc, const void* d_anchors, int dtype,
+                          const pa_layout* L, double eb, double rel_eb, const uint8_t* d_flags8,
+                          const int32_t* d_maxlevel, void* d_out, void* stream);
+
+/* spline_progressive_reconstruct (InterpolationPredictor.cu:2423-2484):
+ * out = old + reconstruction of delta codes with zero anchors. */
+int pa_spline_delta_reconstruct(const int32_t* d_qc, int dtype, const pa_layout* L, double eb,
+                                double rel_eb, const uint8_t* d_flags8,
+                                const int32_t* d_maxlevel, const void* d_old, void* d_out,
+                                void* stream);
+
+/* convert_to_bitplane<E, NB|SM> (bitplane.cu:604-622).  d_bp: L->host_size
+ * bytes, fully written (bytes between the device and host layouts are 0). */
+int pa_bitshuffle(const int32_t* d_qc, const pa_layout* L, int mode, uint8_t* d_bp,
+                  void* stream);
+/* inverse_convert_to_bitplane (bitplane.cu:624-643) */
+int pa_inverse_bitshuffle(const uint8_t* d_bp, const pa_layout* L, int mode, int32_t* d_qc,
+                          void* stream);
+/* inverse_convert_to_bitplane_progressive (bitplane.cu:645-667): only the
+ * planes [begin[lr], end[lr]) of each level contribute (plus the sign plane
+ * for PA_MODE_SM).  Only those planes are read. */
+int pa_inverse_bitshuffle_window(const uint8_t* d_bp, const pa_layout* L, int mode,
+                                 const int32_t* d_begin, const int32_t* d_end, int32_t* d_qc,
+                                 void* stream);
+
+/* lossless_encode (zbpc.cu:476-520, kernel zbpc_encode 155-263): writes
+ * header, size table and payloads to d_out (capacity >= pa_compress_bound),
+ * the 4x32 cumulative size table to d_cum and the unpadded stream length to
+ * d_outsize.  Single pass with a decoupled look-back for chunk offsets. */
+size_t pa_encode_workspace(const pa_layout* L);
+int pa_lossless_encode(const uint8_t* d_bp, const pa_layout* L, uint8_t* d_out, uint64_t* d_cum,
+                       uint64_t* d_outsize, void* d_ws, size_t ws_bytes, void* stream);
+
+/* compress_merge (compressor.cu:139-151): pads the stream to 8 bytes and
+ * appends flags, range, size table and anchors; d_total receives the final
+ * stream length.  d_range is a device double (NULL: 1.0, absolute mode). */
+int pa_compress_merge(uint8_t* d_out, const pa_layout* L, int dtype, const uint64_t* d_outsize,
+                      const uint8_t* d_flags8, const double* d_range, const uint64_t* d_cum,
+                      const void* d_anchors, uint64_t* d_total, void* stream);
+
+/* lossless_decode (zbpc.cu:523-558, kernel zbpc_decode 265-365).  in_len is
+ * the stream length in bytes; chunks whose table entries point outside it
+ * are not read.  The first int32 of the workspace collects error bits
+ * (non-zero: malformed stream; the affected chunks decode to zeros). */
+size_t pa_decode_workspace(const pa_layout* L);
+int pa_lossless_decode(const uint8_t* d_in, int64_t in_len, const pa_layout* L, uint8_t* d_bp,
+                       void* d_ws, size_t ws_bytes, void* stream);
+/* lossless_decode_progressive (zbpc.cu:560-599, kernel 368-473).  Decodes
+ * every chunk that holds a byte of a requested device bit-plane (the covering
+ * set; the reference's own selection loses plane heads, see the design
+ * notes).  d_bytes_read (may be NULL) accumulates the payload bytes read.
+ * The workspace must stay untouched between calls on the same stream when
+ * offsets_ready != 0 (offsets are then reused). */
+int pa_lossless_decode_window(const uint8_t* d_in, int64_t in_len, const pa_layout* L,
+                              const int32_t* d_begin,
+                              const int32_t* d_end, uint8_t* d_bp, uint64_t* d_bytes_read,
+                              int offsets_ready, void* d_ws, size_t ws_bytes, void* stream);
+
+/* findStrategy_h (dataloader.cu:179-208): updates d_begin/d_end in place and
+ * writes d_maxlevel.  eb and target_error are absolute. */
+size_t pa_strategy_workspace(void);
+int pa_find_strategy(const uint64_t* d_cum, int32_t* d_begin, int32_t* d_end,
+                     int32_t* d_maxlevel, double eb, double target_error, void* d_ws,
+                     size_t ws_bytes, void* stream);
+
+/* Library / device information (no reference counterpart). */
+const char* pa_version(void);
+int pa_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,205 @@
+// bitplane.hip -- bit-plane shuffle and its inverses on gfx950.
+//
+// Layout (bitplane.cu:55-253, stated as a closed form): level l, plane k
+// (0 = most significant bit) and 8-code group g of that level's segment map
+// to byte A_l + k*s_l + g, whose bit 7-j is bit 31-k of code 8g+j (after the
+// NB or SM transform).
+//
+// MI355X mapping: a lane owns 4 consecutive groups (32 codes) and turns them
+// into one 32-bit word per plane with a single in-register 32x32 bit
+// transpose, so each of the 32 plane rows is written with coalesced 4-byte
+// stores (a wave covers 256 contiguous bytes of a row).  Codes are staged
+// through LDS so the global reads are coalesced as well.  A workgroup of 256
+// lanes covers 1024 groups (8192 codes) of one level.
+#include "common.hpp"
+
+namespace pa {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kGroupsPerBlock = 4 * kThreads;         // 1024 groups
+constexpr int kCodesPerBlock = 8 * kGroupsPerBlock;   // 8192 codes
+constexpr int kRow = 33;                              // padded LDS row (32 codes)
+
+// In-register transpose of a 32x32 bit matrix: afterwards bit c of a[k]
+// equals bit k of the original a[c] (bit 0 = least significant).
+__device__ __forceinline__ void transpose32(uint32_t (&a)[32]) {
+    uint32_t m = 0x0000FFFFu;
+#pragma unroll
+    for (int j = 16; j != 0; j >>= 1, m ^= (m << j)) {
+#pragma unroll
+        for (int k = 0; k < 32; k = ((k | j) + 1) & ~j) {
+            const uint32_t t = ((a[k] >> j) ^ a[k + j]) & m;
+            a[k] ^= t << j;
+            a[k + j] ^= t;
+        }
+    }
+}
+
+// position of code q of a 4-group lane slice inside the transpose input:
+// reversing codes within each group puts code j of a group at bit 7-j.
+__device__ __forceinline__ int slot(int q) { return (q & ~7) | (7 - (q & 7)); }
+
+struct Levels {
+    int64_t prefix[4];     // absolute start of each level's codes
+    int64_t real_cnt[4];   // real codes per level
+    int64_t seg_len[4];    // codes per level segment (pads and phantom tail included)
+    int64_t stride[4];     // s_l
+    int64_t base[4];       // A_l
+    int blk_start[5];      // first workgroup of each level (levels 3,2,1,0 in order)
+    int64_t dev_size, host_size;
+};
+
+Levels make_levels(const pa_layout* L) {
+    Levels v;
+    const int64_t cnt[4] = {L->n - L->cnt[0], L->cnt[0] - L->cnt[1], L->cnt[1] - L->cnt[2], L->cnt[2] - L->n4};
+    int b = 0;
+    for (int lv = 3, i = 0; lv >= 0; --lv, ++i) {
+        v.prefix[lv] = L->prefix[lv];
+        v.real_cnt[lv] = cnt[lv];
+        v.seg_len[lv] = L->dev_len[lv];
+        v.stride[lv] = L->dev_stride[lv];
+        v.base[lv] = L->dev_base[lv];
+        v.blk_start[i] = b;
+        b += (int)((L->dev_stride[lv] + kGroupsPerBlock - 1) / kGroupsPerBlock);
+    }
+    v.blk_start[4] = b;
+    v.dev_size = L->dev_size;
+    v.host_size = L->host_size;
+    return v;
+}
+
+__device__ __forceinline__ int block_level(const Levels& v, int blk, int* first) {
+    int lv = 3, i = 0;
+    while (i < 3 && blk >= v.blk_start[i + 1]) { ++i; --lv; }
+    *first = v.blk_start[i];
+    return lv;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void shuffle_kernel(const int32_t* __restrict__ qc, Levels v,
+                                                           uint8_t* __restrict__ bp) {
+    __shared__ int32_t stage[kThreads * kRow];
+    int first;
+    const int lv = block_level(v, blockIdx.x, &first);
+    const int64_t g0 = (int64_t)(blockIdx.x - first) * kGroupsPerBlock;
+    const int64_t c0 = 8 * g0;  // first code of the block within the level
+    const int64_t real = v.real_cnt[lv];
+    const int32_t* src = qc + v.prefix[lv];
+    for (int i = threadIdx.x; i < kCodesPerBlock; i += kThreads) {
+        const int64_t r = c0 + i;
+        stage[(i >> 5) * kRow + (i & 31)] = r < real ? src[r] : 0;
+    }
+    __syncthreads();
+    const int64_t g = g0 + 4 * threadIdx.x;
+    if (g < v.stride[lv]) {
+        uint32_t a[32];
+#pragma unroll
+        for (int q = 0; q < 32; ++q) {
+            const int32_t c = stage[threadIdx.x * kRow + q];
+            a[slot(q)] = MODE == PA_MODE_NB ? nb_fwd(c) : sm_fwd(c);
+        }
+        transpose32(a);
+        uint8_t* row = bp + v.base[lv] + g;
+#pragma unroll
+        for (int k = 0; k < 32; ++k) *reinterpret_cast<uint32_t*>(row + k * v.stride[lv]) = a[31 - k];
+    }
+    if (blockIdx.x == gridDim.x - 1) {  // bytes between the device and host layouts
+        for (int64_t i = v.dev_size + 4 * threadIdx.x; i < v.host_size; i += 4 * kThreads)
+            *reinterpret_cast<uint32_t*>(bp + i) = 0u;
+    }
+}
+
+// Inverse; WINDOW restricts each level to planes [begin[lr], end[lr]) (plus
+// the sign plane in SM mode, bitplane.cu:467-471) and reads only those.
+template <int MODE, bool WINDOW>
+__global__ __launch_bounds__(kThreads) void unshuffle_kernel(const uint8_t* __restrict__ bp, Levels v,
+                                                             const int32_t* __restrict__ begin,
+                                                             const int32_t* __restrict__ end,
+                                                             int32_t* __restrict__ qc) {
+    __shared__ int32_t stage[kThreads * kRow];
+    int first;
+    const int lv = block_level(v, blockIdx.x, &first);
+    const int64_t g0 = (int64_t)(blockIdx.x - first) * kGroupsPerBlock;
+    uint32_t planes = 0xFFFFFFFFu;
+    if (WINDOW) {
+        const int lr = 3 - lv;
+        const int b = begin[lr], e = end[lr];
+        planes = 0;
+        if (b < e) {
+            for (int k = b; k < e && k < 32; ++k) planes |= 1u << k;
+            if (MODE == PA_MODE_SM) planes |= 1u;
+        }
+    }
+    const int64_t g = g0 + 4 * threadIdx.x;
+    if (g < v.stride[lv]) {
+        uint32_t a[32];
+        const uint8_t* row = bp + v.base[lv] + g;
+#pragma unroll
+        for (int k = 0; k < 32; ++k)
+            a[31 - k] = (planes >> k) & 1u ? *reinterpret_cast<const uint32_t*>(row + k * v.stride[lv]) : 0u;
+        transpose32(a);
+#pragma unroll
+        for (int q = 0; q < 32; ++q) {
+            const uint32_t w = a[slot(q)];
+            stage[threadIdx.x * kRow + q] = MODE == PA_MODE_NB ? nb_inv(w) : sm_inv(w);
+        }
+    }
+    __syncthreads();
+    const int64_t c0 = 8 * g0;
+    const int64_t seg = v.seg_len[lv];
+    int32_t* dst = qc + v.prefix[lv];
+    for (int i = threadIdx.x; i < kCodesPerBlock; i += kThreads) {
+        const int64_t r = c0 + i;
+        if (r < seg) dst[r] = stage[(i >> 5) * kRow + (i & 31)];
+    }
+}
+
+}  // namespace
+}  // namespace pa
+
+using namespace pa;
+
+extern "C" int pa_bitshuffle(const int32_t* d_qc, const pa_layout* L, int mode, uint8_t* d_bp, void* stream) {
+    if (!L || !d_qc || !d_bp || (mode != PA_MODE_NB && mode != PA_MODE_SM)) return PA_ERR_ARG;
+    Levels v = make_levels(L);
+    hipStream_t st = as_stream(stream);
+    if (mode == PA_MODE_NB)
+        hipLaunchKernelGGL(shuffle_kernel<PA_MODE_NB>, dim3(v.blk_start[4]), dim3(kThreads), 0, st, d_qc, v, d_bp);
+    else
+        hipLaunchKernelGGL(shuffle_kernel<PA_MODE_SM>, dim3(v.blk_start[4]), dim3(kThreads), 0, st, d_qc, v, d_bp);
+    PA_LAUNCH_CHECK();
+    return PA_OK;
+}
+
+extern "C" int pa_inverse_bitshuffle(const uint8_t* d_bp, const pa_layout* L, int mode, int32_t* d_qc,
+                                     void* stream) {
+    if (!L || !d_qc || !d_bp || (mode != PA_MODE_NB && mode != PA_MODE_SM)) return PA_ERR_ARG;
+    Levels v = make_levels(L);
+    hipStream_t st = as_stream(stream);
+    if (mode == PA_MODE_NB)
+        hipLaunchKernelGGL((unshuffle_kernel<PA_MODE_NB, false>), dim3(v.blk_start[4]), dim3(kThreads), 0, st,
+                           d_bp, v, nullptr, nullptr, d_qc);
+    else
+        hipLaunchKernelGGL((unshuffle_kernel<PA_MODE_SM, false>), dim3(v.blk_start[4]), dim3(kThreads), 0, st,
+                           d_bp, v, nullptr, nullptr, d_qc);
+    PA_LAUNCH_CHECK();
+    return PA_OK;
+}
+
+extern "C" int pa_inverse_bitshuffle_window(const uint8_t* d_bp, const pa_layout* L, int mode,
+                                            const int32_t* d_begin, const int32_t* d_end, int32_t* d_qc,
+                                            void* stream) {
+    if (!L || !d_qc || !d_bp || !d_begin || !d_end || (mode != PA_MODE_NB && mode != PA_MODE_SM))
+        return PA_ERR_ARG;
+    Levels v = make_levels(L);
+    hipStream_t st = as_stream(stream);
+    if (mode == PA_MODE_NB)
+        hipLaunchKernelGGL((unshuffle_kernel<PA_MODE_NB, true>), dim3(v.blk_start[4]), dim3(kThreads), 0, st,
+                           d_bp, v, d_begin, d_end, d_qc);
+    else
+        hipLaunchKernelGGL((unshuffle_kernel<PA_MODE_SM, true>), dim3(v.blk_start[4]), dim3(kThreads), 0, st,
+                           d_bp, v, d_begin, d_end, d_qc);
+    PA_LAUNCH_CHECK();
+    return PA_OK;
+}

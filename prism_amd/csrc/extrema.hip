@@ -1,0 +1,105 @@
+// extrema.hip -- exact min / max of a field (StatBuffer::findMinMax,
+// compressor.cu:228-302) on gfx950.
+//
+// One streaming pass with 16-byte loads; wave64 then workgroup reduction,
+// folded into two 64-bit words with atomicMin / atomicMax on an
+// order-preserving integer image of the double value.  Min and max are
+// exact, so the result does not depend on the reduction order.
+#include "common.hpp"
+
+namespace pa {
+namespace {
+
+__device__ __forceinline__ unsigned long long order_key(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double key_value(unsigned long long k) {
+    const unsigned long long b = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+    return __longlong_as_double((long long)b);
+}
+
+__global__ void extrema_init_kernel(unsigned long long* keys) {
+    keys[0] = ~0ull;  // running min
+    keys[1] = 0ull;   // running max
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void extrema_kernel(const T* __restrict__ in, int64_t n,
+                                                      unsigned long long* __restrict__ keys) {
+    __shared__ double s_min[4], s_max[4];
+    constexpr int V = 16 / sizeof(T);
+    using Vec = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+    double mn = __builtin_inf(), mx = -__builtin_inf();
+    const int64_t nv = n / V;
+    const Vec* vin = reinterpret_cast<const Vec*>(in);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv; i += stride) {
+        const Vec v = vin[i];
+        const T* e = reinterpret_cast<const T*>(&v);
+#pragma unroll
+        for (int j = 0; j < V; ++j) {
+            mn = fmin(mn, (double)e[j]);
+            mx = fmax(mx, (double)e[j]);
+        }
+    }
+    for (int64_t i = nv * V + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+        mn = fmin(mn, (double)in[i]);
+        mx = fmax(mx, (double)in[i]);
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        mn = fmin(mn, __shfl_xor(mn, d, 64));
+        mx = fmax(mx, __shfl_xor(mx, d, 64));
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        s_min[wave] = mn;
+        s_max[wave] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) {
+            mn = fmin(mn, s_min[w]);
+            mx = fmax(mx, s_max[w]);
+        }
+        atomicMin(keys, order_key(mn));
+        atomicMax(keys + 1, order_key(mx));
+    }
+}
+
+__global__ void extrema_finish_kernel(const unsigned long long* keys, double* out) {
+    out[0] = key_value(keys[0]);
+    out[1] = key_value(keys[1]);
+}
+
+}  // namespace
+}  // namespace pa
+
+using namespace pa;
+
+extern "C" int pa_extrema(const void* d_data, int dtype, int64_t n, double* d_minmax, void* stream) {
+    if (!d_data || !d_minmax || n <= 0 || (dtype != PA_F32 && dtype != PA_F64)) return PA_ERR_ARG;
+    if (((uintptr_t)d_data & 15) != 0) return PA_ERR_ARG;
+    hipStream_t st = as_stream(stream);
+    // two 64-bit keys kept right after the two doubles of the result
+    unsigned long long* keys = reinterpret_cast<unsigned long long*>(d_minmax + 2);
+    hipLaunchKernelGGL(extrema_init_kernel, dim3(1), dim3(1), 0, st, keys);
+    PA_LAUNCH_CHECK();
+    const int64_t work = (n + 255) / 256;
+    const int blocks = (int)(work < 2048 ? work : 2048);
+    if (dtype == PA_F32)
+        hipLaunchKernelGGL(extrema_kernel<float>, dim3(blocks), dim3(256), 0, st, (const float*)d_data, n, keys);
+    else
+        hipLaunchKernelGGL(extrema_kernel<double>, dim3(blocks), dim3(256), 0, st, (const double*)d_data, n, keys);
+    PA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(extrema_finish_kernel, dim3(1), dim3(1), 0, st, keys, d_minmax);
+    PA_LAUNCH_CHECK();
+    return PA_OK;
+}
+
+extern "C" int pa_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}

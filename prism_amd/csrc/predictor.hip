@@ -1,0 +1,657 @@
+// predictor.hip -- multi-level cubic-spline prediction on gfx950.
+//
+// One workgroup (256 lanes = 4 wave64) per 16^3 tile.  The tile and its
+// +1 halo (17^3 values of T) live in LDS; each of the 4 levels (unit 8,4,2,1)
+// runs three stages, and every stage is a compile-time shape so the
+// point <-> lane mapping uses constant divisors only.  Points of one stage
+// read only points of earlier stages, so a stage is one parallel sweep
+// followed by one barrier.  Arithmetic is in T (FP64 for double fields),
+// evaluated in the reference's order with no FMA contraction, so the
+// compressor and the decompressor reproduce identical values.
+//
+// Reference: InterpolationPredictor.cu -- stage maps 1786-1809 / 23-106,
+// 1-D rule 1229-1352, multi-dimensional rule 1414-1750, level loop
+// 1823-2026, tile staging 997-1170, compaction 2028-2088, auto-tuner 110-967.
+#include "common.hpp"
+
+namespace pa {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kTileVals = kTS * kTS * kTS;
+
+enum Kind { BLUE, YELLOW, HOLLOW, HOLLOW_R, YELLOW_R, BLUE_R, LINE, FACE, CUBE };
+enum Mode { COMPRESS, DECOMPRESS, TUNE };
+
+__device__ __forceinline__ int sidx(int x, int y, int z) { return (z * kTS + y) * kTS + x; }
+
+template <typename T>
+struct Ctx {
+    T* s;             // LDS tile
+    int ox, oy, oz;   // tile origin
+    bool c1x, c1y, c1z;  // tile is not the last one on the axis
+    Geom g;
+    T eb_r, ebx2;     // per-level values of the current level
+    int32_t* qc_out;
+    const int32_t* qc_in;
+    bool natural;     // tuner candidates 9..11
+    double acc;       // tuner partial sum of this lane
+};
+
+template <typename T>
+__device__ __forceinline__ T cubic(T a, T b, T c, T d, bool natural) {
+    if (natural) return ((((T)-3 * a + (T)23 * b) + (T)23 * c) - (T)3 * d) / (T)40;
+    return (((-a + (T)9 * b) + (T)9 * c) - d) / (T)16;
+}
+template <typename T>
+__device__ __forceinline__ T quad_l(T t0, T t1, T t2) {
+    return ((-t0 + (T)6 * t1) + (T)3 * t2) / (T)8;
+}
+template <typename T>
+__device__ __forceinline__ T quad_r(T t1, T t2, T t3) {
+    return (((T)3 * t1 + (T)6 * t2) - t3) / (T)8;
+}
+template <typename T>
+__device__ __forceinline__ T lin(T t1, T t2) {
+    return (t1 + t2) / (T)2;
+}
+
+// get_interp_order (InterpolationPredictor.cu:1538-1545)
+template <int U>
+__device__ __forceinline__ int interp_order(int c, int g, int N, bool c1) {
+    const int b = (c >= 3 * U) ? 3 : 1;
+    const int f = ((c + 3 * U <= 16) && (c1 || (g + 3 * U < N))) ? 3 : ((c1 || (g + U < N)) ? 1 : 0);
+    return (b == 3) ? ((f == 3) ? 4 : ((f == 1) ? 3 : 0)) : ((f == 3) ? 2 : ((f == 1) ? 1 : 0));
+}
+
+// 1-D rule along axis A (0 = x, 1 = y, 2 = z), InterpolationPredictor.cu:1269-1326
+template <typename T, int U, int A>
+__device__ __forceinline__ T pred_1d(const Ctx<T>& t, int x, int y, int z) {
+    const int c = A == 0 ? x : (A == 1 ? y : z);
+    const int o = A == 0 ? t.ox : (A == 1 ? t.oy : t.oz);
+    const int N = A == 0 ? t.g.nx : (A == 1 ? t.g.ny : t.g.nz);
+    const bool c1 = A == 0 ? t.c1x : (A == 1 ? t.c1y : t.c1z);
+    const int gc = o + c;
+    const int i0 = c - 3 * U >= 0 ? c - 3 * U : 0;
+    const int i1 = c - U >= 0 ? c - U : 0;
+    const int i2 = c + U < kTS ? c + U : 0;
+    const int i3 = c + 3 * U < kTS ? c + 3 * U : 0;
+    const int st = A == 0 ? 1 : (A == 1 ? kTS : kTS * kTS);
+    const int base = sidx(x, y, z) - c * st;
+    const T t0 = t.s[base + i0 * st], t1 = t.s[base + i1 * st];
+    const T t2 = t.s[base + i2 * st], t3 = t.s[base + i3 * st];
+    const bool c2 = c >= 3 * U, c3 = c + 3 * U <= 16, c4 = gc + 3 * U < N, c5 = gc + U < N;
+    T pred = t1;
+    if ((c1 && !c2 && !c3) || (!c1 && !c2 && !(c3 && c4) && c5))
+        pred = lin(t1, t2);
+    else if ((c1 && !c2 && c3) || (!c1 && !c2 && c3 && c4))
+        pred = quad_r(t1, t2, t3);
+    else if ((c1 && c2 && !c3) || (!c1 && c2 && !(c3 && c4) && c5))
+        pred = quad_l(t0, t1, t2);
+    else if ((c1 && c2 && c3) || (!c1 && c2 && c3 && c4))
+        pred = cubic(t0, t1, t2, t3, t.natural);
+    return pred;
+}
+
+template <int U>
+__device__ __forceinline__ void taps(int c, int (&id)[4]) {
+    id[0] = (c - 3 * U >= 0) ? c - 3 * U : 0;
+    id[1] = (c - U >= 0) ? c - U : 0;
+    id[2] = (c + U <= 16) ? c + U : 0;
+    id[3] = (c + 3 * U <= 16) ? c + 3 * U : 0;
+}
+
+// FACE rule, InterpolationPredictor.cu:1546-1669
+template <typename T, int U>
+__device__ __forceinline__ T pred_face(const Ctx<T>& t, int x, int y, int z) {
+    int idz[4], idy[4], idx[4];
+    taps<U>(z, idz);
+    taps<U>(y, idy);
+    taps<U>(x, idx);
+    T s1[4], s2[4], pred;
+    int i1, i2;
+    if ((x % (2 * U)) == 0) {  // directions z (1) and y (2)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) s1[i] = t.s[sidx(x, y, idz[i])];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) s2[i] = t.s[sidx(x, idy[i], z)];
+        pred = t.s[sidx(x, idy[1], idz[1])];
+        i1 = interp_order<U>(z, t.oz + z, t.g.nz, t.c1z);
+        i2 = interp_order<U>(y, t.oy + y, t.g.ny, t.c1y);
+    } else if ((y % (2 * U)) == 0) {  // directions z (1) and x (2)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) s1[i] = t.s[sidx(x, y, idz[i])];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) s2[i] = t.s[sidx(idx[i], y, z)];
+        pred = t.s[sidx(idx[1], y, idz[1])];
+        i1 = interp_order<U>(z, t.oz + z, t.g.nz, t.c1z);
+        i2 = interp_order<U>(x, t.ox + x, t.g.nx, t.c1x);
+    } else {  // directions y (1) and x (2)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) s1[i] = t.s[sidx(x, idy[i], z)];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) s2[i] = t.s[sidx(idx[i], y, z)];
+        pred = t.s[sidx(idx[1], idy[1], z)];
+        i1 = interp_order<U>(y, t.oy + y, t.g.ny, t.c1y);
+        i2 = interp_order<U>(x, t.ox + x, t.g.nx, t.c1x);
+    }
+    const bool nat = t.natural;
+    if (i1 == 4 && i2 == 4) {
+        pred = (cubic(s1[0], s1[1], s1[2], s1[3], nat) + cubic(s2[0], s2[1], s2[2], s2[3], nat)) / (T)2;
+    } else if (i1 != 4 && i2 == 4) {
+        pred = cubic(s2[0], s2[1], s2[2], s2[3], nat);
+    } else if (i1 == 4 && i2 != 4) {
+        pred = cubic(s1[0], s1[1], s1[2], s1[3], nat);
+    } else if (i1 == 3 && i2 == 3) {
+        pred = quad_l(s2[0], s2[1], s2[2]);
+        pred += quad_l(s1[0], s1[1], s1[2]);
+        pred /= (T)2;
+    } else if (i1 == 3 && i2 == 2) {
+        pred = quad_r(s2[1], s2[2], s2[3]);
+        pred += quad_l(s1[0], s1[1], s1[2]);
+        pred /= (T)2;
+    } else if (i1 == 3 && i2 < 2) {
+        pred = quad_l(s1[0], s1[1], s1[2]);
+    } else if (i1 == 2 && i2 == 3) {
+        pred = quad_r(s1[1], s1[2], s1[3]);
+        pred += quad_l(s2[0], s2[1], s2[2]);
+        pred /= (T)2;
+    } else if (i1 == 2 && i2 == 2) {
+        pred = quad_r(s1[1], s1[2], s1[3]);
+        pred += quad_r(s2[1], s2[2], s2[3]);
+        pred /= (T)2;
+    } else if (i1 == 2 && i2 < 2) {
+        pred = quad_r(s1[1], s1[2], s1[3]);
+    } else if (i1 <= 1 && i2 == 3) {
+        pred = quad_l(s2[0], s2[1], s2[2]);
+    } else if (i1 <= 1 && i2 == 2) {
+        pred = quad_r(s2[1], s2[2], s2[3]);
+    } else if (i1 == 1 && i2 == 1) {
+        pred = lin(s2[1], s2[2]);
+        pred += lin(s1[1], s1[2]);
+        pred /= (T)2;
+    } else if (i1 == 1 && i2 < 1) {
+        pred = lin(s1[1], s1[2]);
+    } else if (i1 == 0 && i2 == 1) {
+        pred = lin(s2[1], s2[2]);
+    } else {
+        pred = (s1[1] + s2[1]) - pred;
+    }
+    return pred;
+}
+
+// CUBE rule, InterpolationPredictor.cu:1671-1723
+template <typename T, int U>
+__device__ __forceinline__ T pred_cube(const Ctx<T>& t, int x, int y, int z) {
+    int idz[4], idy[4], idx[4];
+    taps<U>(z, idz);
+    taps<U>(y, idy);
+    taps<U>(x, idx);
+    const int iz = interp_order<U>(z, t.oz + z, t.g.nz, t.c1z);
+    const int iy = interp_order<U>(y, t.oy + y, t.g.ny, t.c1y);
+    const int ix = interp_order<U>(x, t.ox + x, t.g.nx, t.c1x);
+    T tx[4], ty[4], tz[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) tx[i] = t.s[sidx(idx[i], y, z)];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) tz[i] = iz == 4 ? t.s[sidx(x, y, idz[i])] : (T)0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) ty[i] = iy == 4 ? t.s[sidx(x, idy[i], z)] : (T)0;
+    const bool nat = t.natural;
+    const T px1 = cubic(tx[0], tx[1], tx[2], tx[3], nat);
+    const T py1 = cubic(ty[0], ty[1], ty[2], ty[3], nat);
+    const T pz1 = cubic(tz[0], tz[1], tz[2], tz[3], nat);
+    T pred = tx[1];
+    if (iz == 4 && iy == 4 && ix == 4) pred = ((px1 + py1) + pz1) / (T)3;
+    else if (iz == 4 && iy == 4) pred = (pz1 + py1) / (T)2;
+    else if (iz == 4 && ix == 4) pred = (pz1 + px1) / (T)2;
+    else if (iy == 4 && ix == 4) pred = (py1 + px1) / (T)2;
+    else if (iz == 4) pred = pz1;
+    else if (iy == 4) pred = py1;
+    else if (ix == 4) pred = px1;
+    else if (ix == 3) pred = quad_l(tx[0], tx[1], tx[2]);
+    else if (ix == 2) pred = quad_r(tx[1], tx[2], tx[3]);
+    else if (ix == 1) pred = lin(tx[1], tx[2]);
+    return pred;
+}
+
+template <typename T, int U, int K, int M>
+__device__ __forceinline__ void visit(Ctx<T>& t, int x, int y, int z, bool inclusive) {
+    const int gx = t.ox + x, gy = t.oy + y, gz = t.oz + z;
+    bool ok = inclusive ? (x <= 16 && y <= 16 && z <= 16)
+                        : (x < 16 && y < 16 && z < 16 + (t.c1z ? 0 : 1));
+    ok = ok && gx < t.g.nx && gy < t.g.ny && gz < t.g.nz;
+    if (!ok) return;
+    T pred;
+    if constexpr (K == BLUE || K == BLUE_R) {
+        pred = pred_1d<T, U, 2>(t, x, y, z);
+    } else if constexpr (K == YELLOW || K == YELLOW_R) {
+        pred = pred_1d<T, U, 1>(t, x, y, z);
+    } else if constexpr (K == HOLLOW || K == HOLLOW_R) {
+        pred = pred_1d<T, U, 0>(t, x, y, z);
+    } else if constexpr (K == LINE) {
+        if ((z % (2 * U)) > 0) pred = pred_1d<T, U, 2>(t, x, y, z);
+        else if ((y % (2 * U)) > 0) pred = pred_1d<T, U, 1>(t, x, y, z);
+        else pred = pred_1d<T, U, 0>(t, x, y, z);
+    } else if constexpr (K == FACE) {
+        pred = pred_face<T, U>(t, x, y, z);
+    } else {
+        pred = pred_cube<T, U>(t, x, y, z);
+    }
+    T& v = t.s[sidx(x, y, z)];
+    constexpr int LEVEL = U == 1 ? 0 : (U == 2 ? 1 : (U == 4 ? 2 : 3));
+    if constexpr (M == TUNE) {
+        T e = v - pred;
+        t.acc += (double)(e < (T)0 ? -e : e);
+    } else if constexpr (M == COMPRESS) {
+        const T err = v - pred;
+        T code = (err < (T)0 ? -err : err) * t.eb_r + (T)1;
+        code = err < (T)0 ? -code : code;
+        const int q = (int)(code / (T)2);
+        code = (T)q;
+        if (x < 16 && y < 16 && z < 16) t.qc_out[lm_index<LEVEL>(t.g, gx, gy, gz)] = q;
+        v = pred + code * t.ebx2;
+    } else {
+        const int32_t q = t.qc_in[lm_index<LEVEL>(t.g, gx, gy, gz)];
+        v = pred + (T)q * t.ebx2;
+    }
+}
+
+// One stage: the reference's thread-visiting order (it defines the tuner's
+// lane assignment p mod 256; other modes do not depend on it).
+template <typename T, int U, int K, int M>
+__device__ __forceinline__ void stage(Ctx<T>& t, bool inclusive) {
+    constexpr int N = 16 / (2 * U);
+    const int tid = threadIdx.x;
+    if constexpr (K <= BLUE_R) {
+        constexpr int X0 = (K == HOLLOW || K == HOLLOW_R) ? U : 0;
+        constexpr int XS = (K == YELLOW_R || K == BLUE_R) ? U : 2 * U;
+        constexpr int XC = (K == HOLLOW || K == HOLLOW_R) ? N : ((K == YELLOW_R || K == BLUE_R) ? 2 * N + 1 : N + 1);
+        constexpr int Y0 = (K == YELLOW || K == YELLOW_R) ? U : 0;
+        constexpr int YS = (K == HOLLOW || K == BLUE_R) ? U : 2 * U;
+        constexpr int YC = (K == YELLOW || K == YELLOW_R) ? N : ((K == HOLLOW || K == BLUE_R) ? 2 * N + 1 : N + 1);
+        constexpr int Z0 = (K == BLUE || K == BLUE_R) ? U : 0;
+        constexpr int ZS = (K == YELLOW || K == HOLLOW) ? U : 2 * U;
+        constexpr int ZC = (K == BLUE || K == BLUE_R) ? N : ((K == YELLOW || K == HOLLOW) ? 2 * N + 1 : N + 1);
+        constexpr int TOTAL = XC * YC * ZC;
+        for (int p = tid; p < TOTAL; p += kThreads) {
+            const int ix = p % XC, iy = (p / XC) % YC, iz = p / (XC * YC);
+            visit<T, U, K, M>(t, X0 + XS * ix, Y0 + YS * iy, Z0 + ZS * iz, inclusive);
+        }
+    } else if constexpr (K == LINE) {
+        constexpr int L = N * (N + 1) * (N + 1), Q = (N + 1) * (N + 1);
+        for (int p = tid; p < 3 * L; p += kThreads) {
+            const int grp = p / L, m = p % L;
+            const int i = m / Q, j = (m % Q) / (N + 1), k = (m % Q) % (N + 1);
+            int x, y, z;
+            if (grp == 0) { x = 2 * U * i + U; y = 2 * U * j; z = 2 * U * k; }
+            else if (grp == 1) { x = 2 * U * k; y = 2 * U * i + U; z = 2 * U * j; }
+            else { x = 2 * U * j; y = 2 * U * k; z = 2 * U * i + U; }
+            visit<T, U, K, M>(t, x, y, z, inclusive);
+        }
+    } else if constexpr (K == FACE) {
+        constexpr int L = N * N * (N + 1), Q = N * N;
+        for (int p = tid; p < 3 * L; p += kThreads) {
+            const int grp = p / L, m = p % L;
+            const int i = m / Q, j = (m % Q) / N, k = (m % Q) % N;
+            int x, y, z;
+            if (grp == 0) { x = 2 * U * i; y = 2 * U * j + U; z = 2 * U * k + U; }
+            else if (grp == 1) { x = 2 * U * k + U; y = 2 * U * i; z = 2 * U * j + U; }
+            else { x = 2 * U * j + U; y = 2 * U * k + U; z = 2 * U * i; }
+            visit<T, U, K, M>(t, x, y, z, inclusive);
+        }
+    } else {
+        constexpr int Q = N * N;
+        for (int p = tid; p < N * N * N; p += kThreads) {
+            const int i = p / Q, j = (p % Q) / N, k = (p % Q) % N;
+            visit<T, U, K, M>(t, 2 * U * i + U, 2 * U * j + U, 2 * U * k + U, inclusive);
+        }
+    }
+    __syncthreads();
+}
+
+// The three stages of one level; the last stage of level 0 uses the
+// exclusive border outside tuning (InterpolationPredictor.cu:1940-2025).
+template <typename T, int U, int M>
+__device__ __forceinline__ void level(Ctx<T>& t, bool md, bool rev) {
+    const bool last_incl = !(M != TUNE && U == 1);
+    if (md) {
+        stage<T, U, LINE, M>(t, true);
+        stage<T, U, FACE, M>(t, true);
+        stage<T, U, CUBE, M>(t, last_incl);
+    } else if (rev) {
+        stage<T, U, HOLLOW_R, M>(t, true);
+        stage<T, U, YELLOW_R, M>(t, true);
+        stage<T, U, BLUE_R, M>(t, last_incl);
+    } else {
+        stage<T, U, BLUE, M>(t, true);
+        stage<T, U, YELLOW, M>(t, true);
+        stage<T, U, HOLLOW, M>(t, last_incl);
+    }
+}
+
+template <typename T>
+struct LevelEb {
+    T eb_r[4], ebx2[4];
+};
+
+template <typename T>
+__device__ __forceinline__ void load_tile(T* s, const T* data, const Geom& g, int ox, int oy, int oz) {
+    for (int i = threadIdx.x; i < kTileVals; i += kThreads) {
+        const int x = i % kTS, y = (i / kTS) % kTS, z = i / (kTS * kTS);
+        const int gx = ox + x, gy = oy + y, gz = oz + z;
+        T v = (T)0;
+        if (gx < g.nx && gy < g.ny && gz < g.nz)
+            v = data[(int64_t)gx + (int64_t)gy * g.nx + (int64_t)gz * g.nx * g.ny];
+        s[i] = v;
+    }
+}
+
+// kernel `interpolation` (compress) and `interpolation_reverse` /
+// `interpolation_progressive_reverse` (decompress: anchors, or zero anchors
+// plus old for the delta form).
+template <typename T, int M>
+__global__ __launch_bounds__(kThreads) void spline_kernel(const T* __restrict__ data, Geom g,
+                                                          LevelEb<T> le, const uint8_t* __restrict__ flags8,
+                                                          const int32_t* __restrict__ d_maxlevel,
+                                                          int32_t* __restrict__ qc_out,
+                                                          const int32_t* __restrict__ qc_in,
+                                                          T* __restrict__ anchors,
+                                                          const T* __restrict__ anchors_in,
+                                                          const T* __restrict__ old, T* __restrict__ out) {
+    __shared__ T tile[kTileVals];
+    const int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+    Ctx<T> t;
+    t.s = tile;
+    t.ox = bx * 16;
+    t.oy = by * 16;
+    t.oz = bz * 16;
+    t.c1x = t.ox + 16 < g.nx;
+    t.c1y = t.oy + 16 < g.ny;
+    t.c1z = t.oz + 16 < g.nz;
+    t.g = g;
+    t.qc_out = qc_out;
+    t.qc_in = qc_in;
+    t.natural = false;
+    t.acc = 0.0;
+    uint8_t fl[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) fl[i] = flags8[i];
+    const int maxlevel = d_maxlevel ? *d_maxlevel : 4;
+
+    if constexpr (M == COMPRESS) {
+        load_tile(tile, data, g, t.ox, t.oy, t.oz);
+        __syncthreads();
+        if (threadIdx.x == 0)
+            anchors[bx + (int64_t)by * g.gx + (int64_t)bz * g.gx * g.gy] = tile[0];
+    } else {
+        for (int i = threadIdx.x; i < kTileVals; i += kThreads) tile[i] = (T)0;
+        __syncthreads();
+        if (anchors_in && threadIdx.x < 8) {  // x_reset_scratch_data 1051-1061
+            const int ax = bx + (threadIdx.x & 1), ay = by + ((threadIdx.x >> 1) & 1),
+                      az = bz + (threadIdx.x >> 2);
+            if (ax < g.gx && ay < g.gy && az < g.gz)
+                tile[sidx(16 * (threadIdx.x & 1), 16 * ((threadIdx.x >> 1) & 1), 16 * (threadIdx.x >> 2))] =
+                    anchors_in[ax + (int64_t)ay * g.gx + (int64_t)az * g.gx * g.gy];
+        }
+        __syncthreads();
+    }
+
+    if (3 <= maxlevel) { t.eb_r = le.eb_r[3]; t.ebx2 = le.ebx2[3]; level<T, 8, M>(t, fl[3], fl[7]); }
+    if (2 <= maxlevel) { t.eb_r = le.eb_r[2]; t.ebx2 = le.ebx2[2]; level<T, 4, M>(t, fl[2], fl[6]); }
+    if (1 <= maxlevel) { t.eb_r = le.eb_r[1]; t.ebx2 = le.ebx2[1]; level<T, 2, M>(t, fl[1], fl[5]); }
+    if (0 <= maxlevel) { t.eb_r = le.eb_r[0]; t.ebx2 = le.ebx2[0]; level<T, 1, M>(t, fl[0], fl[4]); }
+
+    if constexpr (M == DECOMPRESS) {
+        for (int i = threadIdx.x; i < 4096; i += kThreads) {
+            const int x = i & 15, y = (i >> 4) & 15, z = i >> 8;
+            const int gx = t.ox + x, gy = t.oy + y, gz = t.oz + z;
+            if (gx < g.nx && gy < g.ny && gz < g.nz) {
+                const int64_t gi = (int64_t)gx + (int64_t)gy * g.nx + (int64_t)gz * g.nx * g.ny;
+                const T v = tile[sidx(x, y, z)];
+                out[gi] = old ? v + old[gi] : v;
+            }
+        }
+    }
+}
+
+// Compress-side zero fill of the code array regions the predictor does not
+// write (anchor slots, pads between levels, phantom tail).
+__global__ void fill_i32(int32_t* p, int64_t n, int32_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// ---------------------------------------------------------------- tuner --
+// Grid (sample block, candidate).  Lane p mod 256 of each stage accumulates
+// |x - pred| in double in visiting order; lanes are added by a halving tree.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void tune_kernel(const T* __restrict__ data, Geom g, int sx0,
+                                                        int sy0, int sz0, int nsx, int nsy,
+                                                        double* __restrict__ part) {
+    __shared__ T tile[kTileVals];
+    __shared__ double red[kThreads];
+    const int bid = blockIdx.x, cand = blockIdx.y;
+    Ctx<T> t;
+    t.s = tile;
+    t.ox = sx0 + (bid % nsx) * 128;
+    t.oy = sy0 + ((bid / nsx) % nsy) * 128;
+    t.oz = sz0 + (bid / nsx / nsy) * 128;
+    t.c1x = t.ox + 16 < g.nx;
+    t.c1y = t.oy + 16 < g.ny;
+    t.c1z = t.oz + 16 < g.nz;
+    t.g = g;
+    t.qc_out = nullptr;
+    t.qc_in = nullptr;
+    t.acc = 0.0;
+    load_tile(tile, data, g, t.ox, t.oy, t.oz);
+    __syncthreads();
+    // candidate table of auto_tuning_interpolation (pre_compute_att 110-141, 898-962)
+    t.natural = cand >= 9 && cand <= 11;
+    switch (cand) {
+        case 0: level<T, 8, TUNE>(t, false, false); break;
+        case 1: level<T, 8, TUNE>(t, false, true); break;
+        case 2: level<T, 8, TUNE>(t, true, true); break;
+        case 3: level<T, 4, TUNE>(t, false, false); break;
+        case 4: level<T, 4, TUNE>(t, false, true); break;
+        case 5: level<T, 4, TUNE>(t, true, true); break;
+        case 6: case 9: level<T, 2, TUNE>(t, false, false); break;
+        case 7: case 10: level<T, 2, TUNE>(t, false, true); break;
+        case 8: case 11: level<T, 2, TUNE>(t, true, true); break;
+        case 12: level<T, 1, TUNE>(t, false, false); break;
+        case 13: level<T, 1, TUNE>(t, false, true); break;
+        default: level<T, 1, TUNE>(t, true, true); break;
+    }
+    red[threadIdx.x] = t.acc;
+    __syncthreads();
+    for (int h = kThreads / 2; h >= 1; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(int64_t)bid * 15 + cand] = red[0];
+}
+
+// Sum over sample blocks in block order, then the host's pick rule
+// (InterpolationPredictor.cu:2306-2355) on the device.
+__global__ void tune_pick_kernel(const double* __restrict__ part, int nblocks, double* __restrict__ scores,
+                                 uint8_t* __restrict__ flags) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double e[15];
+    for (int c = 0; c < 15; ++c) {
+        double s = 0.0;
+        for (int b = 0; b < nblocks; ++b) s += part[(int64_t)b * 15 + c];
+        e[c] = s;
+        if (scores) scores[c] = s;
+    }
+    uint8_t f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double best;
+    if (e[0] > e[1]) { best = e[1]; f[7] = 1; } else { best = e[0]; f[7] = 0; }
+    f[3] = e[2] < best;
+    if (e[3] > e[4]) { best = e[4]; f[6] = 1; } else { best = e[3]; f[6] = 0; }
+    f[2] = e[5] < best;
+    best = e[6];
+    int idx = 6;
+    for (int i = 6; i < 12; i++) if (e[i] < best) { best = e[i]; idx = i; }
+    f[1] = (idx == 8 || idx == 11);
+    f[5] = (idx % 3) != 0;
+    best = e[12];
+    idx = 12;
+    for (int i = 12; i < 15; i++) if (e[i] < best) { best = e[i]; idx = i; }
+    f[0] = (idx == 14);
+    f[4] = (idx % 3) != 0;
+    for (int i = 0; i < 8; ++i) flags[i] = f[i];
+}
+
+__global__ void zero_flags_kernel(double* scores, uint8_t* flags) {
+    const int i = threadIdx.x;
+    if (i < 8) flags[i] = 0;
+    if (scores && i < 15) scores[i] = 0.0;
+}
+
+// ------------------------------------------------------------- host side --
+double alpha_of(double rel_eb) {  // InterpolationPredictor.cu:2248-2271
+    const double a1 = 2.0, a2 = 1.75, a3 = 1.5, a4 = 1.25, a5 = 1;
+    const double e1 = 1e-1, e2 = 1e-2, e3 = 1e-3, e4 = 1e-4, e5 = 1e-5;
+    if (rel_eb >= e1) return a1;
+    if (rel_eb >= e2) return a2 + (a1 - a2) * (rel_eb - e2) / (e1 - e2);
+    if (rel_eb >= e3) return a3 + (a2 - a3) * (rel_eb - e3) / (e2 - e3);
+    if (rel_eb >= e4) return a4 + (a3 - a4) * (rel_eb - e4) / (e3 - e4);
+    if (rel_eb >= e5) return a5 + (a4 - a5) * (rel_eb - e5) / (e4 - e5);
+    return a5;
+}
+
+// calc_eb (InterpolationPredictor.cu:1825-1839): every step rounds to T.
+template <typename T>
+LevelEb<T> level_ebs(double eb, double rel_eb) {
+    LevelEb<T> le;
+    const T ebx2 = (T)(eb * 2), eb_r = (T)(1 / eb);
+    const double alpha = alpha_of(rel_eb), beta = 4.0;
+    for (int lv = 0; lv < 4; ++lv) {
+        const int unit = 1 << lv;
+        T r = eb_r, x2 = ebx2;
+        int temp = 1;
+        while (temp < unit) {
+            temp *= 2;
+            r = (T)((double)r * alpha);
+            x2 = (T)((double)x2 / alpha);
+        }
+        if ((double)x2 < (double)ebx2 / beta) {
+            x2 = (T)((double)ebx2 / beta);
+            r = (T)((double)eb_r * beta);
+        }
+        le.eb_r[lv] = r;
+        le.ebx2[lv] = x2;
+    }
+    return le;
+}
+
+template <typename T, int M>
+int launch_spline(const pa_layout* L, const T* data, double eb, double rel_eb, const uint8_t* flags8,
+                  const int32_t* d_maxlevel, int32_t* qc_out, const int32_t* qc_in, T* anchors,
+                  const T* anchors_in, const T* old, T* out, hipStream_t st) {
+    Geom g = make_geom(L);
+    if (g.gy > 65535 || g.gz > 65535) return PA_ERR_ARG;
+    dim3 grid(g.gx, g.gy, g.gz);
+    LevelEb<T> le = level_ebs<T>(eb, rel_eb);
+    hipLaunchKernelGGL((spline_kernel<T, M>), grid, dim3(kThreads), 0, st, data, g, le, flags8, d_maxlevel,
+                       qc_out, qc_in, anchors, anchors_in, old, out);
+    PA_LAUNCH_CHECK();
+    return PA_OK;
+}
+
+// calc_start_size with the reference's unsigned arithmetic (2277-2283):
+// false where that arithmetic wraps (any axis shorter than 17).
+bool tune_axis(uint32_t dim, int* start, int* size) {
+    const uint32_t mid = dim / 2;
+    if (mid < 8) return false;
+    const int64_t tnum = (int64_t)dim - 8 - 1 - mid;
+    if (tnum < 0) return false;
+    const int k = (int)((mid - 8) / 128);
+    *start = (int)mid - k * 128;
+    *size = k + (int)(tnum / 128) + 1;
+    return true;
+}
+
+bool valid_dtype(int d) { return d == PA_F32 || d == PA_F64; }
+
+}  // namespace
+}  // namespace pa
+
+using namespace pa;
+
+extern "C" size_t pa_autotune_workspace(const pa_layout* L) {
+    int s[3], n[3];
+    if (!L || !tune_axis(L->nx, &s[0], &n[0]) || !tune_axis(L->ny, &s[1], &n[1]) ||
+        !tune_axis(L->nz, &s[2], &n[2]))
+        return 0;
+    return sizeof(double) * 15 * (size_t)n[0] * n[1] * n[2];
+}
+
+extern "C" int pa_autotune(const void* d_data, int dtype, const pa_layout* L, double* d_scores15,
+                           uint8_t* d_flags8, void* d_ws, size_t ws_bytes, void* stream) {
+    if (!L || !d_data || !d_flags8 || !valid_dtype(dtype)) return PA_ERR_ARG;
+    hipStream_t st = as_stream(stream);
+    int s[3], n[3];
+    if (!tune_axis(L->nx, &s[0], &n[0]) || !tune_axis(L->ny, &s[1], &n[1]) || !tune_axis(L->nz, &s[2], &n[2])) {
+        hipLaunchKernelGGL(zero_flags_kernel, dim3(1), dim3(64), 0, st, d_scores15, d_flags8);
+        PA_LAUNCH_CHECK();
+        return PA_OK;
+    }
+    const int nblocks = n[0] * n[1] * n[2];
+    if (ws_bytes < pa_autotune_workspace(L) || !d_ws) return PA_ERR_WORKSPACE;
+    double* part = static_cast<double*>(d_ws);
+    Geom g = make_geom(L);
+    dim3 grid(nblocks, 15);
+    if (dtype == PA_F32)
+        hipLaunchKernelGGL(tune_kernel<float>, grid, dim3(kThreads), 0, st, (const float*)d_data, g, s[0], s[1],
+                           s[2], n[0], n[1], part);
+    else
+        hipLaunchKernelGGL(tune_kernel<double>, grid, dim3(kThreads), 0, st, (const double*)d_data, g, s[0],
+                           s[1], s[2], n[0], n[1], part);
+    PA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(tune_pick_kernel, dim3(1), dim3(64), 0, st, part, nblocks, d_scores15, d_flags8);
+    PA_LAUNCH_CHECK();
+    return PA_OK;
+}
+
+extern "C" int pa_spline_construct(const void* d_data, int dtype, const pa_layout* L, double eb,
+                                   double rel_eb, const uint8_t* d_flags8, int32_t* d_qc, void* d_anchors,
+                                   void* stream) {
+    if (!L || !d_data || !d_flags8 || !d_qc || !d_anchors || !valid_dtype(dtype) || !(eb > 0))
+        return PA_ERR_ARG;
+    hipStream_t st = as_stream(stream);
+    // codes the predictor does not own stay zero: clear the whole array first
+    const int64_t n = L->code_len;
+    int blocks = (int)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096);
+    hipLaunchKernelGGL(fill_i32, dim3(blocks), dim3(256), 0, st, d_qc, n, 0);
+    PA_LAUNCH_CHECK();
+    if (dtype == PA_F32)
+        return launch_spline<float, COMPRESS>(L, (const float*)d_data, eb, rel_eb, d_flags8, nullptr, d_qc,
+                                              nullptr, (float*)d_anchors, nullptr, nullptr, nullptr, st);
+    return launch_spline<double, COMPRESS>(L, (const double*)d_data, eb, rel_eb, d_flags8, nullptr, d_qc,
+                                           nullptr, (double*)d_anchors, nullptr, nullptr, nullptr, st);
+}
+
+extern "C" int pa_spline_reconstruct(const int32_t* d_qc, const void* d_anchors, int dtype, const pa_layout* L,
+                                     double eb, double rel_eb, const uint8_t* d_flags8,
+                                     const int32_t* d_maxlevel, void* d_out, void* stream) {
+    if (!L || !d_qc || !d_anchors || !d_flags8 || !d_out || !valid_dtype(dtype) || !(eb > 0)) return PA_ERR_ARG;
+    hipStream_t st = as_stream(stream);
+    if (dtype == PA_F32)
+        return launch_spline<float, DECOMPRESS>(L, nullptr, eb, rel_eb, d_flags8, d_maxlevel, nullptr, d_qc,
+                                                nullptr, (const float*)d_anchors, nullptr, (float*)d_out, st);
+    return launch_spline<double, DECOMPRESS>(L, nullptr, eb, rel_eb, d_flags8, d_maxlevel, nullptr, d_qc, nullptr,
+                                             (const double*)d_anchors, nullptr, (double*)d_out, st);
+}
+
+extern "C" int pa_spline_delta_reconstruct(const int32_t* d_qc, int dtype, const pa_layout* L, double eb,
+                                           double rel_eb, const uint8_t* d_flags8, const int32_t* d_maxlevel,
+                                           const void* d_old, void* d_out, void* stream) {
+    if (!L || !d_qc || !d_flags8 || !d_old || !d_out || !valid_dtype(dtype) || !(eb > 0)) return PA_ERR_ARG;
+    hipStream_t st = as_stream(stream);
+    if (dtype == PA_F32)
+        return launch_spline<float, DECOMPRESS>(L, nullptr, eb, rel_eb, d_flags8, d_maxlevel, nullptr, d_qc,
+                                                nullptr, nullptr, (const float*)d_old, (float*)d_out, st);
+    return launch_spline<double, DECOMPRESS>(L, nullptr, eb, rel_eb, d_flags8, d_maxlevel, nullptr, d_qc, nullptr,
+                                             nullptr, (const double*)d_old, (double*)d_out, st);
+}

@@ -1,0 +1,136 @@
+// strategy.hip -- progressive load strategy (which bit-planes of each level
+// to read for a target error) on gfx950.
+//
+// The reference (dataloader.cu:30-177) enumerates the 33^4 plane counts with
+// 33 blocks x 1024 threads, keeps each thread's first strict maximum, then
+// reduces with halving trees that keep the left operand on ties.  Such a tree
+// returns the tied index with the smallest bit-reversed value, so the same
+// winner is the maximum of one 64-bit key
+//     value << 22 | (63 - rev6(b0)) << 16 | (1023 - rev10(tid)) << 6 | (63 - k)
+// (pos = tid + 1024k), which any reduction order finds.  No grid barrier and
+// no co-residency assumption: waves fold their maxima with a 64-bit atomicMax.
+#include "common.hpp"
+
+namespace pa {
+namespace {
+
+// ceil(2^(32-b) * F_lr), F = sum of p^(12-3lr-{1,2,3}), p = 1 (lr 0) or 1.25:
+// the exact integers of the reference's double evaluation (dataloader.cu:59-68)
+__device__ __forceinline__ uint64_t plane_cost(int lr, int b) {
+    const uint64_t num = lr == 0 ? 3u : (lr == 1 ? 953125u : (lr == 2 ? 7625u : 61u));
+    const int den = lr == 0 ? 0 : (lr == 1 ? 16 : (lr == 2 ? 10 : 4));
+    if (b >= 32) return 0;
+    const int sh = 32 - b - den;
+    if (sh >= 0) return num << sh;
+    const uint64_t d = 1ull << (-sh);
+    return (num + d - 1) / d;
+}
+
+__global__ void strategy_reset_kernel(unsigned long long* key) { *key = 0ull; }
+
+__global__ __launch_bounds__(256) void strategy_search_kernel(const uint64_t* __restrict__ cum,
+                                                              uint64_t target_cost,
+                                                              unsigned long long* __restrict__ best) {
+    __shared__ uint64_t s_val[4][33], s_cost[4][33];
+    const int tid = threadIdx.x;
+    if (tid < 4 * 33) {
+        const int lr = tid / 33, b = tid % 33;
+        uint64_t v;
+        if (b == 0) v = cum[lr * 32 + 31];  // cumulative quirk, dataloader.cu:47
+        else if (b == 32) v = 0;
+        else v = cum[lr * 32 + 31] - cum[lr * 32 + b - 1];
+        s_val[lr][b] = v;
+        s_cost[lr][b] = plane_cost(lr, b);
+    }
+    __syncthreads();
+    const int b0 = blockIdx.x;
+    const uint64_t c0 = s_cost[0][b0], v0 = s_val[0][b0];
+    const uint64_t tie0 = (uint64_t)(63u - (__brev((uint32_t)b0) >> 26)) << 16;
+    unsigned long long best_key = 0ull;
+    for (int pos = tid; pos < 33 * 33 * 33; pos += 256) {
+        const int b1 = pos / 1089, b2 = (pos / 33) % 33, b3 = pos % 33;
+        const uint64_t tc = c0 + s_cost[1][b1] + s_cost[2][b2] + s_cost[3][b3];
+        if (tc > target_cost) continue;
+        const uint64_t v = v0 + s_val[1][b1] + s_val[2][b2] + s_val[3][b3];
+        if (v == 0) continue;
+        const uint64_t key = (v << 22) | tie0 |
+                             ((uint64_t)(1023u - (__brev((uint32_t)(pos & 1023)) >> 22)) << 6) |
+                             (uint64_t)(63 - (pos >> 10));
+        if (key > best_key) best_key = key;
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const unsigned long long o = __shfl_xor(best_key, d, 64);
+        best_key = o > best_key ? o : best_key;
+    }
+    if ((tid & 63) == 0 && best_key) atomicMax(best, best_key);
+}
+
+// begin <- end; end <- max(end, choice); maxlevel = largest level that
+// advanced (dataloader.cu:135-163), 4 when none did
+__global__ void strategy_apply_kernel(const unsigned long long* __restrict__ best, int32_t* begin, int32_t* end,
+                                      int32_t* maxlevel) {
+    if (threadIdx.x != 0) return;
+    const unsigned long long key = *best;
+    int ch[4] = {32, 32, 32, 32};
+    if (key) {
+        const int k = 63 - (int)(key & 63);
+        const int t = (int)(__brev(1023u - (uint32_t)((key >> 6) & 1023)) >> 22);
+        const int b0 = (int)(__brev(63u - (uint32_t)((key >> 16) & 63)) >> 26);
+        const int pos = t + 1024 * k;
+        ch[0] = b0;
+        ch[1] = pos / 1089;
+        ch[2] = (pos / 33) % 33;
+        ch[3] = pos % 33;
+    }
+    int ml = 4;
+    for (int i = 0; i < 4; ++i) begin[i] = end[i];
+    for (int lr = 3; lr >= 0; --lr) {
+        if (begin[lr] < ch[lr]) {
+            end[lr] = ch[lr];
+            ml = 3 - lr;
+        } else {
+            end[lr] = begin[lr];
+        }
+    }
+    if (maxlevel) *maxlevel = ml;
+}
+
+// loadall (dataloader.cu:21-28)
+__global__ void load_all_kernel(int32_t* begin, int32_t* end, int32_t* maxlevel) {
+    if (threadIdx.x != 0) return;
+    for (int i = 0; i < 4; ++i) begin[i] = end[i];
+    for (int i = 0; i < 4; ++i) end[i] = 32;
+    const int ml = begin[0] < 32 ? 3 : begin[1] < 32 ? 2 : begin[2] < 32 ? 1 : begin[3] < 32 ? 0 : 4;
+    if (maxlevel) *maxlevel = ml;
+}
+
+}  // namespace
+}  // namespace pa
+
+using namespace pa;
+
+extern "C" size_t pa_strategy_workspace(void) { return 64; }
+
+extern "C" int pa_find_strategy(const uint64_t* d_cum, int32_t* d_begin, int32_t* d_end, int32_t* d_maxlevel,
+                                double eb, double target_error, void* d_ws, size_t ws_bytes, void* stream) {
+    if (!d_cum || !d_begin || !d_end || !(eb > 0)) return PA_ERR_ARG;
+    if (!d_ws || ws_bytes < pa_strategy_workspace()) return PA_ERR_WORKSPACE;
+    hipStream_t st = as_stream(stream);
+    if (target_error * 0.954 >= eb) {  // findStrategy_h 181, 193
+        const double q = target_error * 0.954 / eb;
+        if (!(q < 1.8e19)) return PA_ERR_ARG;
+        const uint64_t target_cost = (uint64_t)__builtin_floor(q) - 1;
+        unsigned long long* key = static_cast<unsigned long long*>(d_ws);
+        hipLaunchKernelGGL(strategy_reset_kernel, dim3(1), dim3(1), 0, st, key);
+        PA_LAUNCH_CHECK();
+        hipLaunchKernelGGL(strategy_search_kernel, dim3(33), dim3(256), 0, st, d_cum, target_cost, key);
+        PA_LAUNCH_CHECK();
+        hipLaunchKernelGGL(strategy_apply_kernel, dim3(1), dim3(64), 0, st, key, d_begin, d_end, d_maxlevel);
+        PA_LAUNCH_CHECK();
+    } else {
+        hipLaunchKernelGGL(load_all_kernel, dim3(1), dim3(64), 0, st, d_begin, d_end, d_maxlevel);
+        PA_LAUNCH_CHECK();
+    }
+    return PA_OK;
+}

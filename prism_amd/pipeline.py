@@ -1,0 +1,224 @@
+"""Host orchestration of the compress / decompress / progressive-retrieve path.
+
+Follows the reference's Compressor flow (compressor.cu:120-190): extrema ->
+auto-tune -> predict -> bit-plane shuffle -> lossless encode -> trailer, and
+the inverse.  Every stage is a call into libprism_amd.so; torch is used only
+to allocate device memory and to name the current HIP stream.  All stage
+calls are stream-ordered; host synchronisation happens only where a host
+value is needed (the value range of a REL-mode field, the stream length).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import torch
+
+from . import _lib
+from ._lib import PA_F32, PA_F64, PA_MODE_NB, PA_MODE_SM, check, layout, lib
+
+DTYPES = {torch.float32: PA_F32, torch.float64: PA_F64}
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _p(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _dev():
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _empty(nbytes: int, dtype=torch.uint8):
+    item = torch.empty(0, dtype=dtype).element_size()
+    return torch.empty(max(1, (int(nbytes) + item - 1) // item), dtype=dtype, device=_dev())
+
+
+def shape_xyz(t: torch.Tensor):
+    """(nx, ny, nz) of a [nz][ny][nx] (or lower-rank, x fastest) tensor."""
+    s = list(t.shape)
+    while len(s) < 3:
+        s.insert(0, 1)
+    if len(s) != 3:
+        raise ValueError("fields are 1-D, 2-D or 3-D")
+    return s[2], s[1], s[0]
+
+
+@dataclass
+class Compressed:
+    stream: torch.Tensor          # uint8, exact length, on the device
+    shape: tuple                  # (nx, ny, nz)
+    dtype: torch.dtype
+    eb_input: float               # rel_eb (REL) or eb (ABS)
+    rel: bool
+    progressive: bool
+    range: float
+    flags: list = field(default_factory=list)
+
+    @property
+    def nbytes(self) -> int:
+        return int(self.stream.numel())
+
+    def eb_abs(self) -> float:
+        return self.eb_input * self.range if self.rel else self.eb_input
+
+
+def extrema(data: torch.Tensor):
+    mm = torch.empty(4, dtype=torch.float64, device=data.device)
+    check(lib().pa_extrema(_p(data), DTYPES[data.dtype], data.numel(), _p(mm), _stream()), "pa_extrema")
+    return mm
+
+
+def compress(data: torch.Tensor, eb: float, rel: bool = True, progressive: bool = True,
+             return_parts: bool = False):
+    """Compresses a contiguous device field (T[nz][ny][nx])."""
+    if not data.is_cuda or data.dtype not in DTYPES:
+        raise ValueError("data must be a float32/float64 CUDA tensor")
+    data = data.contiguous()
+    nx, ny, nz = shape_xyz(data)
+    L = layout(nx, ny, nz)
+    dt = DTYPES[data.dtype]
+    st = _stream()
+    rng = 1.0
+    rng_dev = None
+    if rel:
+        mm = extrema(data)
+        rng_dev = (mm[1:2] - mm[0:1]).contiguous()
+        lo, hi = mm[:2].tolist()
+        rng = hi - lo
+        eb_abs, rel_eb = eb * rng, eb
+    else:
+        eb_abs, rel_eb = eb, 0.0
+    if not eb_abs > 0:
+        raise ValueError("error bound must be positive (constant field in REL mode?)")
+    flags = torch.empty(8, dtype=torch.uint8, device=data.device)
+    scores = torch.empty(15, dtype=torch.float64, device=data.device)
+    tws = _empty(max(8, lib().pa_autotune_workspace(C.byref(L))))
+    check(lib().pa_autotune(_p(data), dt, C.byref(L), _p(scores), _p(flags), _p(tws), tws.numel(), st),
+          "pa_autotune")
+    qc = torch.empty(L.code_len, dtype=torch.int32, device=data.device)
+    anchors = torch.empty(L.n4, dtype=data.dtype, device=data.device)
+    check(lib().pa_spline_construct(_p(data), dt, C.byref(L), eb_abs, rel_eb, _p(flags), _p(qc), _p(anchors), st),
+          "pa_spline_construct")
+    bp = _empty(L.host_size)
+    mode = PA_MODE_NB if progressive else PA_MODE_SM
+    check(lib().pa_bitshuffle(_p(qc), C.byref(L), mode, _p(bp), st), "pa_bitshuffle")
+    bound = lib().pa_compress_bound(C.byref(L), dt)
+    out = _empty(bound)
+    cum = torch.empty(128, dtype=torch.int64, device=data.device)
+    sizes = torch.empty(2, dtype=torch.int64, device=data.device)
+    ews = _empty(lib().pa_encode_workspace(C.byref(L)))
+    check(lib().pa_lossless_encode(_p(bp), C.byref(L), _p(out), _p(cum), _p(sizes), _p(ews), ews.numel(), st),
+          "pa_lossless_encode")
+    total_t = sizes[1:2]
+    check(lib().pa_compress_merge(_p(out), C.byref(L), dt, _p(sizes), _p(flags), _p(rng_dev), _p(cum),
+                                  _p(anchors), _p(total_t), st), "pa_compress_merge")
+    total = int(total_t.item())
+    c = Compressed(stream=out[:total], shape=(nx, ny, nz), dtype=data.dtype, eb_input=eb, rel=rel,
+                   progressive=progressive, range=rng, flags=flags.tolist())
+    if return_parts:
+        return c, {"qc": qc, "anchors": anchors, "bp": bp, "cum": cum, "flags": flags, "scores": scores,
+                   "layout": L}
+    return c
+
+
+def _trailer(c: Compressed, L):
+    item = 4 if c.dtype == torch.float32 else 8
+    tsize = L.trailer_fixed + L.n4 * item
+    t0 = c.nbytes - tsize
+    if t0 < 8 + 2 * L.nchunks or t0 % 8:
+        raise _lib.PrismError("stream too short or misaligned for this shape")
+    s = c.stream
+    flags = s[t0:t0 + 8]
+    cum = s[t0 + 16:t0 + 1040].view(torch.int64)
+    anchors = s[t0 + 1040:t0 + 1040 + L.n4 * item].view(c.dtype)
+    return t0, flags, cum, anchors
+
+
+def stream_range(c: Compressed) -> float:
+    """The value range stored in the trailer (compressor.cu:146, 204)."""
+    nx, ny, nz = c.shape
+    L = layout(nx, ny, nz)
+    t0, _, _, _ = _trailer(c, L)
+    return float(c.stream[t0 + 8:t0 + 16].view(torch.float64).item())
+
+
+def decompress(c: Compressed) -> torch.Tensor:
+    """Full decompression of a plain (SM) stream, decompress_pipeline
+    (compressor.cu:153-164)."""
+    nx, ny, nz = c.shape
+    L = layout(nx, ny, nz)
+    dt = DTYPES[c.dtype]
+    st = _stream()
+    t0, flags, cum, anchors = _trailer(c, L)
+    rng = stream_range(c)
+    eb_abs = c.eb_input * rng if c.rel else c.eb_input
+    rel_eb = c.eb_input if c.rel else 0.0
+    bp = _empty(L.host_size)
+    ws = _empty(lib().pa_decode_workspace(C.byref(L)))
+    check(lib().pa_lossless_decode(_p(c.stream), c.nbytes, C.byref(L), _p(bp), _p(ws), ws.numel(), st), "pa_lossless_decode")
+    qc = torch.empty(L.code_len, dtype=torch.int32, device=c.stream.device)
+    mode = PA_MODE_NB if c.progressive else PA_MODE_SM
+    check(lib().pa_inverse_bitshuffle(_p(bp), C.byref(L), mode, _p(qc), st), "pa_inverse_bitshuffle")
+    out = torch.empty((nz, ny, nx), dtype=c.dtype, device=c.stream.device)
+    check(lib().pa_spline_reconstruct(_p(qc), _p(anchors), dt, C.byref(L), eb_abs, rel_eb, _p(flags), None,
+                                      _p(out), st), "pa_spline_reconstruct")
+    return out
+
+
+class Retriever:
+    """Progressive retrieval of one compressed field, keeping its state
+    (begin/end per level, decoded bit-planes, current reconstruction)
+    resident on the device; each retrieve() refines from the current level
+    (decompress_progressive_pipeline, compressor.cu:166-190)."""
+
+    def __init__(self, c: Compressed):
+        if not c.progressive:
+            raise ValueError("progressive retrieval needs a stream compressed with progressive=True")
+        self.c = c
+        nx, ny, nz = c.shape
+        self.L = layout(nx, ny, nz)
+        self.dt = DTYPES[c.dtype]
+        dev = c.stream.device
+        self.t0, self.flags, self.cum, self.anchors = _trailer(c, self.L)
+        self.range = stream_range(c)
+        self.eb_abs = c.eb_input * self.range if c.rel else c.eb_input
+        self.rel_eb = c.eb_input if c.rel else 0.0
+        self.state = torch.zeros(16, dtype=torch.int32, device=dev)  # begin[4] end[4] maxlevel
+        self.begin, self.end, self.maxlevel = self.state[0:4], self.state[4:8], self.state[8:9]
+        self.bp = _empty(self.L.host_size)
+        self.qc = torch.empty(self.L.code_len, dtype=torch.int32, device=dev)
+        self.dws = _empty(lib().pa_decode_workspace(C.byref(self.L)))
+        self.sws = _empty(lib().pa_strategy_workspace())
+        self.bytes_read = torch.zeros(1, dtype=torch.int64, device=dev)
+        self.out = None
+        self.rounds = 0
+
+    def retrieve(self, target: float) -> torch.Tensor:
+        """Refines to the target error bound (relative in REL mode)."""
+        st = _stream()
+        L = self.L
+        target_abs = target * self.range if self.c.rel else target
+        check(lib().pa_find_strategy(_p(self.cum), _p(self.begin), _p(self.end), _p(self.maxlevel), self.eb_abs,
+                                     target_abs, _p(self.sws), self.sws.numel(), st), "pa_find_strategy")
+        check(lib().pa_lossless_decode_window(_p(self.c.stream), self.c.nbytes, C.byref(L), _p(self.begin), _p(self.end),
+                                              _p(self.bp), _p(self.bytes_read), int(self.rounds > 0),
+                                              _p(self.dws), self.dws.numel(), st), "pa_lossless_decode_window")
+        check(lib().pa_inverse_bitshuffle_window(_p(self.bp), C.byref(L), PA_MODE_NB, _p(self.begin),
+                                                 _p(self.end), _p(self.qc), st), "pa_inverse_bitshuffle_window")
+        nx, ny, nz = self.c.shape
+        new = torch.empty((nz, ny, nx), dtype=self.c.dtype, device=self.qc.device)
+        if self.out is None:
+            check(lib().pa_spline_reconstruct(_p(self.qc), _p(self.anchors), self.dt, C.byref(L), self.eb_abs,
+                                              self.rel_eb, _p(self.flags), _p(self.maxlevel), _p(new), st),
+                  "pa_spline_reconstruct")
+        else:
+            check(lib().pa_spline_delta_reconstruct(_p(self.qc), self.dt, C.byref(L), self.eb_abs, self.rel_eb,
+                                                    _p(self.flags), _p(self.maxlevel), _p(self.out), _p(new),
+                                                    st), "pa_spline_delta_reconstruct")
+        self.out = new
+        self.rounds += 1
+        return new

@@ -1,0 +1,267 @@
+"""Parity of the HIP path with the CPU oracle, stage by stage and end to end.
+
+Integer and byte outputs (codes, bit-planes, streams, strategy) must be
+bit-exact; floating-point outputs (reconstructions, tuner scores) must be
+bit-exact too, because both sides evaluate the same IEEE operations in the
+same order with no FMA contraction.
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def announce(*a):
+    print("CASE", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def pa():
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    import prism_amd
+    from prism_amd import _lib, pipeline
+    prism_amd.lib()
+    return _lib, pipeline
+
+
+def smooth(shape, dtype, seed=0, noise=1e-3):
+    nz, ny, nx = shape
+    rng = np.random.default_rng(seed)
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    f = (np.sin(x / 5.3 + rng.random()) * np.cos(y / 7.1) + 0.5 * np.sin(z / 3.7 + 1.0) +
+         0.2 * np.cos((x + y) / 11.0) + rng.normal(0, noise, shape))
+    return np.ascontiguousarray(f.astype(dtype))
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def P(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def S():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+SHAPES = [(33, 47, 29), (16, 16, 16), (1, 1, 1000), (1, 40, 70), (35, 18, 20), (64, 64, 64), (17, 17, 17)]
+FLAGSETS = [[0] * 8, [1] * 4 + [0] * 4, [0] * 4 + [1] * 4, [1, 0, 1, 0, 0, 1, 0, 1]]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_extrema(pa, dtype):
+    _lib, pipe = pa
+    rng = np.random.default_rng(1)
+    for n in (1, 3, 1000, (1 << 20) + 5):
+        announce("extrema", dtype.__name__, n)
+        a = rng.normal(0, 10, n).astype(dtype)
+        mm = host(pipe.extrema(dev(a)))
+        assert mm[0] == float(a.min()) and mm[1] == float(a.max())
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_predictor_parity(pa, oracle, dtype, shape):
+    _lib, pipe = pa
+    nz, ny, nx = shape
+    data = smooth(shape, dtype, seed=nx)
+    L = _lib.layout(nx, ny, nz)
+    dt = _lib.PA_F32 if dtype == np.float32 else _lib.PA_F64
+    for flags in FLAGSETS:
+        for rel in (1e-2, 1e-5):
+            announce("predictor", shape, dtype.__name__, flags, rel)
+            rg = float(data.max()) - float(data.min())
+            eb = rel * rg
+            qc_o, an_o = oracle.spline_construct(data, eb, rel, flags)
+            d = dev(data)
+            fl = dev(np.array(flags, np.uint8))
+            qc = torch.empty(L.code_len, dtype=torch.int32, device="cuda")
+            an = torch.empty(L.n4, dtype=d.dtype, device="cuda")
+            _lib.check(_lib.lib().pa_spline_construct(P(d), dt, C.byref(L), eb, rel, P(fl), P(qc), P(an), S()))
+            assert np.array_equal(host(qc), qc_o)
+            assert np.array_equal(host(an), an_o)
+            rec_o = oracle.spline_reconstruct(qc_o, an_o, shape, eb, rel, flags)
+            out = torch.empty(shape, dtype=d.dtype, device="cuda")
+            _lib.check(_lib.lib().pa_spline_reconstruct(P(qc), P(an), dt, C.byref(L), eb, rel, P(fl), None,
+                                                         P(out), S()))
+            assert np.array_equal(host(out).view(np.uint8), rec_o.view(np.uint8))
+            # delta form with a partial level range
+            ml = torch.tensor([2], dtype=torch.int32, device="cuda")
+            half = qc_o // 3
+            old = rec_o * dtype(0.5)
+            dl_o = oracle.spline_delta_reconstruct(half, old, eb, rel, flags, maxlevel=2)
+            out2 = torch.empty(shape, dtype=d.dtype, device="cuda")
+            half_d, old_d = dev(half), dev(old)  # keep alive until the kernel has run
+            _lib.check(_lib.lib().pa_spline_delta_reconstruct(P(half_d), dt, C.byref(L), eb, rel, P(fl), P(ml),
+                                                               P(old_d), P(out2), S()))
+            assert np.array_equal(host(out2).view(np.uint8), dl_o.view(np.uint8))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", [(40, 40, 40), (20, 70, 33), (35, 300, 17), (17, 17, 17), (1, 64, 64)])
+def test_tuner_parity(pa, oracle, dtype, shape):
+    _lib, pipe = pa
+    nz, ny, nx = shape
+    announce("tuner", shape, dtype.__name__)
+    data = smooth(shape, dtype, seed=3, noise=0.05)
+    sc_o, fl_o, rc = oracle.autotune(data)
+    L = _lib.layout(nx, ny, nz)
+    dt = _lib.PA_F32 if dtype == np.float32 else _lib.PA_F64
+    d = dev(data)
+    fl = torch.empty(8, dtype=torch.uint8, device="cuda")
+    sc = torch.empty(15, dtype=torch.float64, device="cuda")
+    ws = torch.empty(max(8, _lib.lib().pa_autotune_workspace(C.byref(L))), dtype=torch.uint8, device="cuda")
+    _lib.check(_lib.lib().pa_autotune(P(d), dt, C.byref(L), P(sc), P(fl), P(ws), ws.numel(), S()))
+    assert np.array_equal(host(sc).view(np.uint64), sc_o.view(np.uint64))
+    assert list(host(fl)) == list(fl_o)
+
+
+def _codes_for(oracle, shape, dtype, rel=1e-4, seed=0):
+    data = smooth(shape, dtype, seed=seed)
+    rg = float(data.max()) - float(data.min())
+    qc, _ = oracle.spline_construct(data, rel * rg, rel, [1, 0, 1, 0, 0, 1, 0, 1])
+    return qc
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("shape", SHAPES + [(100, 30, 1), (300, 1, 1)])
+def test_shuffle_parity(pa, oracle, shape, mode):
+    _lib, pipe = pa
+    nz, ny, nx = shape
+    announce("shuffle", shape, mode)
+    L = _lib.layout(nx, ny, nz)
+    Lo = oracle.layout(nx, ny, nz)
+    qc_o = _codes_for(oracle, shape, np.float64)
+    bp_o = oracle.bitshuffle(qc_o, Lo, mode)
+    qc = dev(qc_o)
+    bp = torch.full((L.host_size,), 0xCD, dtype=torch.uint8, device="cuda")
+    _lib.check(_lib.lib().pa_bitshuffle(P(qc), C.byref(L), mode, P(bp), S()))
+    assert np.array_equal(host(bp), bp_o)
+    back = torch.zeros(L.code_len, dtype=torch.int32, device="cuda")
+    _lib.check(_lib.lib().pa_inverse_bitshuffle(P(bp), C.byref(L), mode, P(back), S()))
+    assert np.array_equal(host(back)[L.n4:L.real_end], qc_o[L.n4:L.real_end])
+    for b, e in (([0] * 4, [5, 9, 32, 1]), ([3, 0, 7, 31], [20, 32, 7, 32])):
+        w_o = oracle.inverse_bitshuffle(bp_o, Lo, mode, b, e)
+        be = dev(np.array(b + e, np.int32))
+        w = torch.zeros(L.code_len, dtype=torch.int32, device="cuda")
+        _lib.check(_lib.lib().pa_inverse_bitshuffle_window(P(bp), C.byref(L), mode, P(be[0:4]), P(be[4:8]), P(w),
+                                                            S()))
+        assert np.array_equal(host(w)[L.n4:L.real_end], w_o[L.n4:L.real_end])
+
+
+def _gpu_encode(_lib, L, bp_np):
+    bp = dev(bp_np)
+    bound = _lib.lib().pa_compress_bound(C.byref(L), 8)
+    out = torch.zeros(bound, dtype=torch.uint8, device="cuda")
+    cum = torch.zeros(128, dtype=torch.int64, device="cuda")
+    sz = torch.zeros(2, dtype=torch.int64, device="cuda")
+    ws = torch.empty(_lib.lib().pa_encode_workspace(C.byref(L)), dtype=torch.uint8, device="cuda")
+    _lib.check(_lib.lib().pa_lossless_encode(P(bp), C.byref(L), P(out), P(cum), P(sz), P(ws), ws.numel(), S()))
+    n = int(host(sz)[0])
+    return out, host(out)[:n], host(cum).view(np.uint64)
+
+
+@pytest.mark.parametrize("shape", [(33, 47, 29), (64, 64, 64), (300, 1, 1), (1, 100, 70), (128, 128, 128)])
+@pytest.mark.parametrize("kind", ["codes", "sparse", "dense", "zeros"])
+def test_codec_parity(pa, oracle, shape, kind):
+    _lib, pipe = pa
+    nz, ny, nx = shape
+    announce("codec", shape, kind)
+    L = _lib.layout(nx, ny, nz)
+    Lo = oracle.layout(nx, ny, nz)
+    rng = np.random.default_rng(2)
+    if kind == "codes":
+        bp_np = oracle.bitshuffle(_codes_for(oracle, shape, np.float32), Lo, 0)
+    elif kind == "sparse":
+        bp_np = np.where(rng.random(L.host_size) < 0.03, rng.integers(1, 256, L.host_size), 0).astype(np.uint8)
+    elif kind == "dense":
+        bp_np = np.where(rng.random(L.host_size) < 0.6, rng.integers(1, 256, L.host_size), 0).astype(np.uint8)
+    else:
+        bp_np = np.zeros(L.host_size, np.uint8)
+    s_o, cum_o = oracle.lossless_encode(bp_np, Lo)
+    out, s_g, cum_g = _gpu_encode(_lib, L, bp_np)
+    pad = (-s_g.size) % 8
+    assert s_g.size + pad == s_o.size
+    assert np.array_equal(s_g, s_o[:s_g.size])
+    assert np.array_equal(cum_g, cum_o)
+    bp = torch.full((L.host_size,), 0x5A, dtype=torch.uint8, device="cuda")
+    ws = torch.empty(_lib.lib().pa_decode_workspace(C.byref(L)), dtype=torch.uint8, device="cuda")
+    _lib.check(_lib.lib().pa_lossless_decode(P(out), out.numel(), C.byref(L), P(bp), P(ws), ws.numel(), S()))
+    assert np.array_equal(host(bp), bp_np)
+    assert int(host(ws[:4].view(torch.int32))[0]) == 0
+    # windowed decode: every device byte of the requested planes is restored
+    b, e = [1, 0, 6, 30], [9, 32, 6, 32]
+    be = dev(np.array(b + e, np.int32))
+    bp2 = torch.zeros(L.host_size, dtype=torch.uint8, device="cuda")
+    nread = torch.zeros(1, dtype=torch.int64, device="cuda")
+    _lib.check(_lib.lib().pa_lossless_decode_window(P(out), out.numel(), C.byref(L), P(be[0:4]), P(be[4:8]), P(bp2), P(nread),
+                                                     0, P(ws), ws.numel(), S()))
+    got = host(bp2)
+    ref, nb = oracle.lossless_decode_window(s_o, Lo, b, e, corrected=True)
+    assert int(host(nread)[0]) == nb
+    for lr in range(4):
+        lv = 3 - lr
+        if b[lr] < e[lr]:
+            lo = L.dev_base[lv] + b[lr] * L.dev_stride[lv]
+            hi = L.dev_base[lv] + e[lr] * L.dev_stride[lv]
+            assert np.array_equal(got[lo:hi], bp_np[lo:hi])
+
+
+def test_strategy_parity(pa, oracle):
+    _lib, pipe = pa
+    rng = np.random.default_rng(5)
+    ws = torch.empty(_lib.lib().pa_strategy_workspace(), dtype=torch.uint8, device="cuda")
+    for trial in range(12):
+        announce("strategy", trial)
+        sizes = rng.integers(2, 6000, 128) if trial % 3 else rng.choice([2, 4], 128)
+        cum = np.cumsum(sizes).astype(np.uint64)
+        cum_d = dev(cum.view(np.int64))
+        st = torch.zeros(9, dtype=torch.int32, device="cuda")
+        b_o, e_o = np.zeros(4, np.int32), np.zeros(4, np.int32)
+        eb = 1e-6 * (1 + trial)
+        for target in (1e-2, 1e-3, 3e-4, 1e-4, 2e-5, 1e-5, 5e-7):
+            b_o, e_o, ml_o, _ = oracle.find_strategy(cum, b_o, e_o, eb, target)
+            _lib.check(_lib.lib().pa_find_strategy(P(cum_d), P(st[0:4]), P(st[4:8]), P(st[8:9]), eb, target, P(ws),
+                                                    ws.numel(), S()))
+            h = host(st)
+            assert list(h[0:4]) == list(b_o) and list(h[4:8]) == list(e_o) and h[8] == ml_o, (trial, target)
+
+
+CONFIGS = [((33, 47, 29), np.float32, 1e-3), ((16, 16, 16), np.float64, 1e-2), ((1, 1, 5000), np.float64, 1e-4),
+           ((1, 70, 90), np.float32, 1e-4), ((64, 64, 64), np.float32, 1e-3), ((40, 50, 60), np.float64, 1e-6)]
+
+
+@pytest.mark.parametrize("shape,dtype,rel", CONFIGS)
+@pytest.mark.parametrize("progressive", [False, True])
+def test_end_to_end_stream_parity(pa, oracle, shape, dtype, rel, progressive):
+    _lib, pipe = pa
+    announce("e2e", shape, dtype.__name__, rel, progressive)
+    data = smooth(shape, dtype, seed=7)
+    s_o, fl_o = oracle.compress(data, rel, mode=0 if progressive else 1)
+    c = pipe.compress(dev(data), rel, rel=True, progressive=progressive)
+    s_g = host(c.stream)
+    assert s_g.size == s_o.size
+    assert np.array_equal(s_g, s_o)
+    if not progressive:
+        r_o = oracle.decompress(s_o, shape, dtype, rel)
+        r_g = host(pipe.decompress(c))
+        assert np.array_equal(r_g.view(np.uint8), r_o.view(np.uint8))
+    else:
+        po = oracle.Progressive(s_o, shape, dtype, rel, corrected=True)
+        rv = pipe.Retriever(c)
+        for t in (1e-1, 1e-2, 1e-3, 1e-4, rel, rel / 2):
+            r_o, ml_o, _ = po.retrieve(t)
+            r_g = host(rv.retrieve(t))
+            assert np.array_equal(r_g.view(np.uint8), r_o.view(np.uint8)), t
