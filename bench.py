@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Benchmark of the compress / progressive-retrieve path on one MI355X.
+
+Workload (default): a 384 x 384 x 256 float64 Miranda-like density field
+(synthetic), relative error bound 1e-6, compressed as a progressive stream,
+then retrieved progressively at 1e-2, 1e-3, 1e-4, 1e-5, 1e-6.
+
+A step is one compress of the field plus one full progressive retrieval
+(5 incremental rounds) from a fresh retrieval state, with the field already
+resident in HBM.  value = 2 * field_bytes / (compress_time + retrieve_time):
+the mean of the compress and decompress throughputs in GB/s (1e9 B/s) of
+field data, so a path that compresses and decompresses at X GB/s scores X.
+
+Printed as one JSON line on rank 0.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+METRIC = "cmp/decmp GB/s + CR @rel-eb 1e-6; batched-vs-plain GB/s; p99 retrieval lat"
+HBM_PEAK_GBPS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
+    ap.add_argument("--cpu-sample-planes", type=int, default=64,
+                    help="z-planes of the field timed on the host cores for the CPU baseline")
+    ap.add_argument("--shape", type=int, nargs=3, default=[384, 384, 256], metavar=("NX", "NY", "NZ"))
+    ap.add_argument("--rel-eb", type=float, default=1e-6)
+    ap.add_argument("--targets", type=float, nargs="+", default=[1e-2, 1e-3, 1e-4, 1e-5, 1e-6])
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build shards nothing)",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from prism_amd import fields, pipeline
+    from prism_amd._lib import layout
+
+    torch.cuda.init()
+    dev = torch.device("cuda", 0)
+    nx, ny, nz = args.shape
+    field = fields.density(nx, ny, nz, seed=2, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    L = layout(nx, ny, nz)
+    s = field.element_size()
+    field_bytes = field.numel() * s
+
+    def one_step(timer=None):
+        c = pipeline.compress(field, args.rel_eb, rel=True, progressive=True, timer=timer)
+        r = pipeline.Retriever(c)
+        for t in args.targets:
+            r.retrieve(t, timer=timer)
+        return c, r
+
+    for _ in range(args.warmup):
+        one_step()
+    torch.cuda.synchronize()
+
+    # timed region: compress and retrieve timed separately with host clocks
+    # around device-synchronised phases, summed per step
+    t_cmp = t_dec = 0.0
+    torch.cuda.synchronize()
+    t_all0 = time.perf_counter()
+    for _ in range(args.steps):
+        a = time.perf_counter()
+        c = pipeline.compress(field, args.rel_eb, rel=True, progressive=True)
+        torch.cuda.synchronize()
+        b = time.perf_counter()
+        r = pipeline.Retriever(c)
+        for t in args.targets:
+            r.retrieve(t)
+        torch.cuda.synchronize()
+        e = time.perf_counter()
+        t_cmp += b - a
+        t_dec += e - b
+    t_all = time.perf_counter() - t_all0
+    ms_per_step = 1e3 * t_all / args.steps
+    cmp_gbps = field_bytes * args.steps / t_cmp / 1e9
+    dec_gbps = field_bytes * args.steps / t_dec / 1e9
+    value = 2 * field_bytes * args.steps / (t_cmp + t_dec) / 1e9
+
+    # per-stage device time (HIP events on the launch stream), separate pass
+    timer = pipeline.StageTimer()
+    for _ in range(max(1, min(args.steps, 5))):
+        one_step(timer)
+    stage_ms = timer.results()
+
+    # quality and bytes per level, untimed
+    c = pipeline.compress(field, args.rel_eb, rel=True, progressive=True)
+    stream_bytes = c.nbytes
+    r = pipeline.Retriever(c)
+    f64 = field.to(torch.float64)
+    vrange = float(f64.max() - f64.min())
+    levels = []
+    prev = 0
+    for i, t in enumerate(args.targets):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        out = r.retrieve(t)
+        torch.cuda.synchronize()
+        lat = (time.perf_counter() - a) * 1e3
+        err = (out.to(torch.float64) - f64).abs()
+        linf = float(err.max())
+        mse = float((err * err).mean())
+        psnr = 20 * math.log10(vrange) - 10 * math.log10(mse) if mse > 0 else float("inf")
+        nread = int(r.bytes_read.item())
+        levels.append({"target": t, "bound_abs": t * c.range, "linf": linf, "linf_over_bound": linf / (t * c.range),
+                       "psnr_db": psnr, "payload_bytes_read": nread - prev, "ms": lat,
+                       "begin": r.begin.tolist(), "end": r.end.tolist()})
+        prev = nread
+
+    # roofline of the dominant kernel: per-launch algorithmic bytes over the
+    # launch's own event-timed duration, averaged over the launches
+    nrounds = len(args.targets)
+    codes = 4 * (L.n - L.n4)
+
+    def planes_bytes(lv_info):
+        return sum((lv_info["end"][3 - lv] - lv_info["begin"][3 - lv]) * L.dev_stride[lv] for lv in range(4))
+
+    def launch_bytes(name, k):
+        if name == "extrema":
+            return L.n * s
+        if name == "predict":
+            return L.n * s + codes + L.n4 * s
+        if name == "shuffle":
+            return codes + L.host_size
+        if name == "encode":
+            return L.host_size + stream_bytes
+        rnd = levels[k % nrounds]
+        if name == "decode":
+            return rnd["payload_bytes_read"] + planes_bytes(rnd)
+        if name == "unshuffle":
+            return planes_bytes(rnd) + 4 * L.code_len
+        if name == "reconstruct":
+            return codes + L.n * s + (L.n * s if k % nrounds else L.n4 * s)
+        return 0
+
+    passes = max(1, min(args.steps, 5))
+    stats = {}
+    for name, times in stage_ms.items():
+        byts = [launch_bytes(name, k) for k in range(len(times))]
+        avg = sum(times) / len(times)
+        stats[name] = {"avg_ms": avg, "launches_per_step": len(times) // passes,
+                       "avg_bytes": sum(byts) / len(byts),
+                       "gbps": (sum(byts) / (sum(times) * 1e-3) / 1e9) if sum(byts) else None}
+    dominant = max(stats, key=lambda k: stats[k]["avg_ms"] * stats[k]["launches_per_step"])
+    d = stats[dominant]
+    roofline = {"bound": "hbm", "kernel": dominant, "achieved": d["gbps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": (d["gbps"] / HBM_PEAK_GBPS) if d["gbps"] else None, "traffic": None,
+                "avg_launch_ms": d["avg_ms"], "algorithmic_bytes_per_launch": d["avg_bytes"]}
+
+    cpu = None
+    if args.cpu_baseline == "on" and int(os.environ.get("RANK", "0")) == 0:
+        cpu = cpu_baseline(field, args)
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (Miranda-like density: 1 + 0.5 tanh(8 plane waves) + N(0,1e-4), seed 2)",
+        "config": {"workload": "progressive density field 384x384x256 f64, rel eb 1e-6, retrieve 1e-2..1e-6",
+                   "shape_xyz": [nx, ny, nz], "rel_eb": args.rel_eb, "targets": args.targets},
+        "compress_GBps": cmp_gbps, "decompress_GBps": dec_gbps,
+        "compression_ratio": field_bytes / stream_bytes, "stream_bytes": stream_bytes,
+        "levels": levels, "stages": stats, "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+def cpu_baseline(field, args):
+    """The oracle (the reference algorithm restated in C) on the host cores,
+    on a slab of the same field: same step (compress + 5-round retrieval)."""
+    import numpy as np
+    import torch
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, here)
+    from oracle import oracle as O
+
+    planes = min(args.cpu_sample_planes, field.shape[0])
+    sample = np.ascontiguousarray(field[:planes].cpu().numpy())
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    nthreads = max(1, min(nthreads, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    stream, _ = O.compress(sample, args.rel_eb, mode=0, nthreads=nthreads)
+    t1 = time.perf_counter()
+    p = O.Progressive(stream, sample.shape, sample.dtype, args.rel_eb, corrected=True, nthreads=nthreads)
+    for t in args.targets:
+        p.retrieve(t)
+    t2 = time.perf_counter()
+    b = sample.nbytes
+    return {"value": 2 * b / ((t1 - t0) + (t2 - t1)) / 1e9, "unit": "GB/s", "cores": nthreads, "kind": "port",
+            "sample": f"first {planes} z-planes ({sample.shape[2]}x{sample.shape[1]}x{planes} f64) of the same field, "
+                      f"one compress + {len(args.targets)}-round retrieval",
+            "compress_s": t1 - t0, "retrieve_s": t2 - t1}
+
+
+if __name__ == "__main__":
+    main()

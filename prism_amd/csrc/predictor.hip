@@ -415,11 +415,17 @@ __global__ __launch_bounds__(kThreads) void spline_kernel(const T* __restrict__ 
     }
 }
 
-// Compress-side zero fill of the code array regions the predictor does not
-// write (anchor slots, pads between levels, phantom tail).
-__global__ void fill_i32(int32_t* p, int64_t n, int32_t v) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = v;
+// Zero the code-array entries the predictor does not write: the anchor
+// slots, the pads that 8-align each level region and the level-0 tail.
+struct Gaps {
+    int64_t lo[5], hi[5];
+};
+
+__global__ void zero_gaps_kernel(int32_t* p, Gaps g) {
+    for (int r = 0; r < 5; ++r)
+        for (int64_t i = g.lo[r] + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < g.hi[r];
+             i += (int64_t)gridDim.x * blockDim.x)
+            p[i] = 0;
 }
 
 // ---------------------------------------------------------------- tuner --
@@ -620,10 +626,20 @@ extern "C" int pa_spline_construct(const void* d_data, int dtype, const pa_layou
     if (!L || !d_data || !d_flags8 || !d_qc || !d_anchors || !valid_dtype(dtype) || !(eb > 0))
         return PA_ERR_ARG;
     hipStream_t st = as_stream(stream);
-    // codes the predictor does not own stay zero: clear the whole array first
-    const int64_t n = L->code_len;
-    int blocks = (int)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096);
-    hipLaunchKernelGGL(fill_i32, dim3(blocks), dim3(256), 0, st, d_qc, n, 0);
+    Gaps gaps;
+    const int64_t cnt[4] = {L->n - L->cnt[0], L->cnt[0] - L->cnt[1], L->cnt[1] - L->cnt[2], L->cnt[2] - L->n4};
+    gaps.lo[0] = 0;
+    gaps.hi[0] = L->n4;
+    for (int lv = 3; lv >= 1; --lv) {
+        gaps.lo[4 - lv] = L->prefix[lv] + cnt[lv];
+        gaps.hi[4 - lv] = L->prefix[lv - 1];
+    }
+    gaps.lo[4] = L->real_end;
+    gaps.hi[4] = L->code_len;
+    int64_t most = 0;
+    for (int r = 0; r < 5; ++r) most = gaps.hi[r] - gaps.lo[r] > most ? gaps.hi[r] - gaps.lo[r] : most;
+    const int blocks = (int)((most + 255) / 256 < 1024 ? (most + 255) / 256 + 1 : 1024);
+    hipLaunchKernelGGL(zero_gaps_kernel, dim3(blocks), dim3(256), 0, st, d_qc, gaps);
     PA_LAUNCH_CHECK();
     if (dtype == PA_F32)
         return launch_spline<float, COMPRESS>(L, (const float*)d_data, eb, rel_eb, d_flags8, nullptr, d_qc,

@@ -66,6 +66,53 @@ class Compressed:
         return self.eb_input * self.range if self.rel else self.eb_input
 
 
+class StageTimer:
+    """Per-stage device time from HIP events recorded on the current stream
+    (the stream every stage launches on).  Durations are read after a
+    synchronisation by `results()`."""
+
+    def __init__(self):
+        self.events = []  # (name, start, end)
+
+    def __call__(self, name):
+        timer = self
+
+        class _Span:
+            def __enter__(self_inner):
+                self_inner.s = torch.cuda.Event(enable_timing=True)
+                self_inner.e = torch.cuda.Event(enable_timing=True)
+                self_inner.s.record()
+                return self_inner
+
+            def __exit__(self_inner, *exc):
+                self_inner.e.record()
+                timer.events.append((name, self_inner.s, self_inner.e))
+                return False
+
+        return _Span()
+
+    def results(self):
+        torch.cuda.synchronize()
+        out = {}
+        for name, s, e in self.events:
+            out.setdefault(name, []).append(s.elapsed_time(e))
+        return out
+
+
+class _NoTimer:
+    def __call__(self, name):
+        return self
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        return False
+
+
+_NO_TIMER = _NoTimer()
+
+
 def extrema(data: torch.Tensor):
     mm = torch.empty(4, dtype=torch.float64, device=data.device)
     check(lib().pa_extrema(_p(data), DTYPES[data.dtype], data.numel(), _p(mm), _stream()), "pa_extrema")
@@ -73,8 +120,9 @@ def extrema(data: torch.Tensor):
 
 
 def compress(data: torch.Tensor, eb: float, rel: bool = True, progressive: bool = True,
-             return_parts: bool = False):
+             return_parts: bool = False, timer=None):
     """Compresses a contiguous device field (T[nz][ny][nx])."""
+    tm = timer or _NO_TIMER
     if not data.is_cuda or data.dtype not in DTYPES:
         raise ValueError("data must be a float32/float64 CUDA tensor")
     data = data.contiguous()
@@ -85,7 +133,8 @@ def compress(data: torch.Tensor, eb: float, rel: bool = True, progressive: bool 
     rng = 1.0
     rng_dev = None
     if rel:
-        mm = extrema(data)
+        with tm("extrema"):
+            mm = extrema(data)
         rng_dev = (mm[1:2] - mm[0:1]).contiguous()
         lo, hi = mm[:2].tolist()
         rng = hi - lo
@@ -97,25 +146,30 @@ def compress(data: torch.Tensor, eb: float, rel: bool = True, progressive: bool 
     flags = torch.empty(8, dtype=torch.uint8, device=data.device)
     scores = torch.empty(15, dtype=torch.float64, device=data.device)
     tws = _empty(max(8, lib().pa_autotune_workspace(C.byref(L))))
-    check(lib().pa_autotune(_p(data), dt, C.byref(L), _p(scores), _p(flags), _p(tws), tws.numel(), st),
-          "pa_autotune")
+    with tm("autotune"):
+        check(lib().pa_autotune(_p(data), dt, C.byref(L), _p(scores), _p(flags), _p(tws), tws.numel(), st),
+              "pa_autotune")
     qc = torch.empty(L.code_len, dtype=torch.int32, device=data.device)
     anchors = torch.empty(L.n4, dtype=data.dtype, device=data.device)
-    check(lib().pa_spline_construct(_p(data), dt, C.byref(L), eb_abs, rel_eb, _p(flags), _p(qc), _p(anchors), st),
-          "pa_spline_construct")
+    with tm("predict"):
+        check(lib().pa_spline_construct(_p(data), dt, C.byref(L), eb_abs, rel_eb, _p(flags), _p(qc), _p(anchors),
+                                        st), "pa_spline_construct")
     bp = _empty(L.host_size)
     mode = PA_MODE_NB if progressive else PA_MODE_SM
-    check(lib().pa_bitshuffle(_p(qc), C.byref(L), mode, _p(bp), st), "pa_bitshuffle")
+    with tm("shuffle"):
+        check(lib().pa_bitshuffle(_p(qc), C.byref(L), mode, _p(bp), st), "pa_bitshuffle")
     bound = lib().pa_compress_bound(C.byref(L), dt)
     out = _empty(bound)
     cum = torch.empty(128, dtype=torch.int64, device=data.device)
     sizes = torch.empty(2, dtype=torch.int64, device=data.device)
     ews = _empty(lib().pa_encode_workspace(C.byref(L)))
-    check(lib().pa_lossless_encode(_p(bp), C.byref(L), _p(out), _p(cum), _p(sizes), _p(ews), ews.numel(), st),
-          "pa_lossless_encode")
+    with tm("encode"):
+        check(lib().pa_lossless_encode(_p(bp), C.byref(L), _p(out), _p(cum), _p(sizes), _p(ews), ews.numel(), st),
+              "pa_lossless_encode")
     total_t = sizes[1:2]
-    check(lib().pa_compress_merge(_p(out), C.byref(L), dt, _p(sizes), _p(flags), _p(rng_dev), _p(cum),
-                                  _p(anchors), _p(total_t), st), "pa_compress_merge")
+    with tm("merge"):
+        check(lib().pa_compress_merge(_p(out), C.byref(L), dt, _p(sizes), _p(flags), _p(rng_dev), _p(cum),
+                                      _p(anchors), _p(total_t), st), "pa_compress_merge")
     total = int(total_t.item())
     c = Compressed(stream=out[:total], shape=(nx, ny, nz), dtype=data.dtype, eb_input=eb, rel=rel,
                    progressive=progressive, range=rng, flags=flags.tolist())
@@ -197,20 +251,33 @@ class Retriever:
         self.out = None
         self.rounds = 0
 
-    def retrieve(self, target: float) -> torch.Tensor:
+    def retrieve(self, target: float, timer=None) -> torch.Tensor:
         """Refines to the target error bound (relative in REL mode)."""
+        tm = timer or _NO_TIMER
         st = _stream()
         L = self.L
         target_abs = target * self.range if self.c.rel else target
-        check(lib().pa_find_strategy(_p(self.cum), _p(self.begin), _p(self.end), _p(self.maxlevel), self.eb_abs,
-                                     target_abs, _p(self.sws), self.sws.numel(), st), "pa_find_strategy")
-        check(lib().pa_lossless_decode_window(_p(self.c.stream), self.c.nbytes, C.byref(L), _p(self.begin), _p(self.end),
+        with tm("strategy"):
+            check(lib().pa_find_strategy(_p(self.cum), _p(self.begin), _p(self.end), _p(self.maxlevel),
+                                         self.eb_abs, target_abs, _p(self.sws), self.sws.numel(), st),
+                  "pa_find_strategy")
+        with tm("decode"):
+            check(lib().pa_lossless_decode_window(_p(self.c.stream), self.c.nbytes, C.byref(L), _p(self.begin), _p(self.end),
                                               _p(self.bp), _p(self.bytes_read), int(self.rounds > 0),
                                               _p(self.dws), self.dws.numel(), st), "pa_lossless_decode_window")
-        check(lib().pa_inverse_bitshuffle_window(_p(self.bp), C.byref(L), PA_MODE_NB, _p(self.begin),
-                                                 _p(self.end), _p(self.qc), st), "pa_inverse_bitshuffle_window")
+        with tm("unshuffle"):
+            check(lib().pa_inverse_bitshuffle_window(_p(self.bp), C.byref(L), PA_MODE_NB, _p(self.begin),
+                                                     _p(self.end), _p(self.qc), st), "pa_inverse_bitshuffle_window")
         nx, ny, nz = self.c.shape
         new = torch.empty((nz, ny, nx), dtype=self.c.dtype, device=self.qc.device)
+        with tm("reconstruct"):
+            self._reconstruct(new, st)
+        self.out = new
+        self.rounds += 1
+        return new
+
+    def _reconstruct(self, new, st):
+        L = self.L
         if self.out is None:
             check(lib().pa_spline_reconstruct(_p(self.qc), _p(self.anchors), self.dt, C.byref(L), self.eb_abs,
                                               self.rel_eb, _p(self.flags), _p(self.maxlevel), _p(new), st),
@@ -219,6 +286,3 @@ class Retriever:
             check(lib().pa_spline_delta_reconstruct(_p(self.qc), self.dt, C.byref(L), self.eb_abs, self.rel_eb,
                                                     _p(self.flags), _p(self.maxlevel), _p(self.out), _p(new),
                                                     st), "pa_spline_delta_reconstruct")
-        self.out = new
-        self.rounds += 1
-        return new
