@@ -95,19 +95,21 @@ __device__ __forceinline__ int block_scan(int v, int* lds, int* total) {
 
 constexpr uint64_t kEmpty = 0, kAgg = 1, kPrefix = 2;
 
+// Status words carry their own value, so relaxed agent-scope atomics are
+// enough: nothing else is published through them.
 __device__ __forceinline__ void publish(uint64_t* status, int64_t id, uint64_t flag, uint64_t value) {
-    __hip_atomic_store(status + id, (flag << 62) | value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(status + id, (flag << 62) | value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Decoupled look-back by wave 0: exclusive payload offset of chunk id.
+// Decoupled look-back by one wave64: exclusive payload offset of chunk id.
 // Chunk ids are handed out in arrival order, so every predecessor belongs to
-// a workgroup that is already running and will publish without waiting.
+// a wave that is already running and publishes its size without waiting.
 __device__ int64_t look_back(uint64_t* status, int64_t id) {
     const int lane = threadIdx.x & 63;
     int64_t excl = 0, top = id - 1;
     while (true) {
         const int64_t idx = top - lane;
-        const uint64_t st = idx >= 0 ? __hip_atomic_load(status + idx, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)
+        const uint64_t st = idx >= 0 ? __hip_atomic_load(status + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                                      : (kPrefix << 62);
         const uint64_t flag = st >> 62;
         const uint64_t val = st & ((1ull << 62) - 1);
@@ -137,18 +139,24 @@ __device__ int64_t look_back(uint64_t* status, int64_t id) {
     }
 }
 
-// repetition elimination of 8 bytes (packed little endian) given the byte
-// before them; returns the 8 flag bits
-__device__ __forceinline__ uint32_t re_flags(uint64_t v, uint32_t prev) {
-    uint32_t f = 0, p = prev;
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const uint32_t b = (uint32_t)(v >> (8 * j)) & 0xFFu;
-        f |= (uint32_t)(b != p) << j;
-        p = b;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(v, d, 64);
+        if (lane >= d) v += y;
     }
-    return f;
+    return v;
 }
+
+// 4-bit mask of the non-zero bytes of w
+__device__ __forceinline__ uint32_t nz_bytes(uint32_t w) {
+    const uint32_t h = (((w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | w) & 0x80808080u;
+    return ((h >> 7) & 1u) | ((h >> 14) & 2u) | ((h >> 21) & 4u) | ((h >> 28) & 8u);
+}
+
+// keep the ordering of LDS accesses of one wave across phases
+__device__ __forceinline__ void wave_sync() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); __builtin_amdgcn_wave_barrier(); }
 
 __global__ void encode_init_kernel(uint64_t* status, int64_t n, uint32_t* counter) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -156,165 +164,159 @@ __global__ void encode_init_kernel(uint64_t* status, int64_t n, uint32_t* counte
     if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0;
 }
 
-__global__ __launch_bounds__(kThreads) void encode_kernel(const uint8_t* __restrict__ bp, ChunkGeom g,
-                                                          uint8_t* __restrict__ out, uint64_t* __restrict__ cum,
-                                                          uint64_t* __restrict__ outsize,
-                                                          uint64_t* __restrict__ status,
-                                                          uint32_t* __restrict__ counter) {
-    __shared__ uint64_t s_in[kCS / 8];
-    __shared__ uint8_t s_out[kCS + 8];
-    __shared__ uint64_t s_b0[256];
-    __shared__ uint64_t s_b1[32];
-    __shared__ uint8_t s_b2[32];
-    __shared__ uint8_t s_b3[4];
-    __shared__ int s_scan[8];
-    __shared__ int64_t s_id, s_excl;
-    const int tid = threadIdx.x;
-    if (tid == 0) s_id = (int64_t)atomicAdd(counter, 1u);
-    __syncthreads();
-    const int64_t id = s_id;
+constexpr int kWaves = 4;  // chunks per workgroup, one per wave64
+
+struct __align__(16) WaveSlot {
+    uint16_t b0[kCS / 16];      // zero-elimination bitmap: 2 bytes per 16-byte segment
+    uint32_t out[kCS / 4 + 4];  // encoded bytes (+ slack read by the funnel-shifted store)
+};
+static_assert(sizeof(WaveSlot) % 16 == 0, "slots must keep 16-byte alignment");
+
+// One chunk per wave64.  Zero elimination streams the chunk in 16 steps of
+// 1 KiB (lane l owns 16-byte segment 64k+l, so every load is coalesced);
+// the three repetition-elimination rounds work on registers and shuffles.
+__global__ __launch_bounds__(64 * kWaves) void encode_kernel(const uint8_t* __restrict__ bp, ChunkGeom g,
+                                                             uint8_t* __restrict__ dst_stream,
+                                                             uint64_t* __restrict__ cum,
+                                                             uint64_t* __restrict__ outsize,
+                                                             uint64_t* __restrict__ status,
+                                                             uint32_t* __restrict__ counter) {
+    __shared__ WaveSlot slots[kWaves];
+    const int lane = threadIdx.x & 63;
+    WaveSlot& W = slots[threadIdx.x >> 6];
+    uint8_t* out = reinterpret_cast<uint8_t*>(W.out);
+    uint32_t got = 0;
+    if (lane == 0) got = atomicAdd(counter, 1u);
+    const int64_t id = (int64_t)__shfl(got, 0, 64);
+    if (id >= g.nchunks) return;
     const ChunkPos cp = chunk_pos(g, id);
     const int osize = cp.osize;
+    const uint8_t* src = bp + cp.base;
 
-    const uint64_t* src = reinterpret_cast<const uint64_t*>(bp + cp.base);
-    const int nwords = osize >> 3;
-    for (int i = tid; i < kCS / 8; i += kThreads) s_in[i] = i < nwords ? src[i] : 0ull;
-    __syncthreads();
-
-    // zero elimination: lane t owns bytes [64t, 64t+64)
-    const uint8_t* inb = reinterpret_cast<const uint8_t*>(s_in);
-    uint64_t mask = 0;
+    // zero elimination
+    int Z = 0;
+#pragma unroll 1
+    for (int k = 0; k < kCS / 1024; ++k) {
+        const int seg = 64 * k + lane;
+        const int b = 16 * seg;
+        uint32_t w[4] = {0u, 0u, 0u, 0u};
+        if (b < osize) {
+            const uint2 lo = *reinterpret_cast<const uint2*>(src + b);
+            w[0] = lo.x;
+            w[1] = lo.y;
+            if (b + 8 < osize) {
+                const uint2 hi = *reinterpret_cast<const uint2*>(src + b + 8);
+                w[2] = hi.x;
+                w[3] = hi.y;
+            }
+        }
+        const uint32_t m = nz_bytes(w[0]) | (nz_bytes(w[1]) << 4) | (nz_bytes(w[2]) << 8) | (nz_bytes(w[3]) << 12);
+        const int cnt = __popc(m);
+        const int incl = wave_incl_scan(cnt);
+        int pos = Z + incl - cnt;
 #pragma unroll
-    for (int w = 0; w < 8; ++w) {
-        const uint64_t v = s_in[tid * 8 + w];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) mask |= (uint64_t)(((v >> (8 * j)) & 0xFFu) != 0) << (8 * w + j);
+        for (int i = 0; i < 16; ++i)
+            if ((m >> i) & 1u) out[pos++] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
+        W.b0[seg] = (uint16_t)m;
+        Z += __shfl(incl, 63, 64);
     }
-    int Z;
-    int pos = block_scan(__popcll(mask), s_scan, &Z);
-    s_b0[tid] = mask;
     bool fail = Z >= kAvail;
-    if (!fail) {
-        uint64_t m = mask;
-        while (m) {
-            const int j = __ffsll((unsigned long long)m) - 1;
-            s_out[pos++] = inb[64 * tid + j];
-            m &= m - 1;
-        }
-    }
-    __syncthreads();
     int wpos = Z;
-    int csize;
-    if (Z == 0) {
-        csize = 2;
-    } else {
-        // round 1: B0 (2048 bytes) -> B1 (256 bytes); lane t owns B0 bytes [8t, 8t+8)
-        if (!fail) {
-            const uint64_t cur = s_b0[tid];
-            const uint32_t prev = tid ? (uint32_t)(s_b0[tid - 1] >> 56) : 0u;
-            const uint32_t f = re_flags(cur, prev);
-            reinterpret_cast<uint8_t*>(s_b1)[tid] = (uint8_t)f;
-            int R;
-            int p = block_scan(__popc(f), s_scan, &R);
-            fail = wpos + R > kAvail;
-            if (!fail) {
+    uint32_t b3 = 0;
+    wave_sync();
+    if (Z != 0) {
+        // round 1: B0 (2048 bytes) -> B1 (256 bytes); lane l owns B0 bytes [32l, 32l+32)
+        uint32_t v[8];
+        const uint4* b0v = reinterpret_cast<const uint4*>(W.b0) + 2 * lane;
+        const uint4 q0 = b0v[0], q1 = b0v[1];
+        v[0] = q0.x; v[1] = q0.y; v[2] = q0.z; v[3] = q0.w;
+        v[4] = q1.x; v[5] = q1.y; v[6] = q1.z; v[7] = q1.w;
+        uint32_t prev = __shfl_up(v[7] >> 24, 1, 64);
+        if (lane == 0) prev = 0;
+        uint32_t f1 = 0;
 #pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if ((f >> j) & 1u) s_out[wpos + p++] = (uint8_t)(cur >> (8 * j));
-            }
-            wpos += R;
-            __syncthreads();
+        for (int j = 0; j < 8; ++j) {
+            const uint32_t shifted = (v[j] << 8) | (j == 0 ? prev : (v[j - 1] >> 24));
+            f1 |= nz_bytes(v[j] ^ shifted) << (4 * j);
         }
-        // round 2: B1 (256 bytes) -> B2 (32 bytes); lanes 0..31
+        int cnt = __popc(f1);
+        int incl = wave_incl_scan(cnt);
+        int R = __shfl(incl, 63, 64);
+        fail = fail || wpos + R > kAvail;
         if (!fail) {
-            uint64_t cur = 0;
-            uint32_t f = 0;
-            if (tid < 32) {
-                cur = s_b1[tid];
-                const uint32_t prev = tid ? (uint32_t)(s_b1[tid - 1] >> 56) : 0u;
-                f = re_flags(cur, prev);
-                s_b2[tid] = (uint8_t)f;
-            }
-            int R;
-            int p = block_scan(__popc(f), s_scan, &R);
-            fail = wpos + R > kAvail;
-            if (!fail && tid < 32) {
+            int pos = wpos + incl - cnt;
 #pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if ((f >> j) & 1u) s_out[wpos + p++] = (uint8_t)(cur >> (8 * j));
-            }
-            wpos += R;
-            __syncthreads();
+            for (int i = 0; i < 32; ++i)
+                if ((f1 >> i) & 1u) out[pos++] = (uint8_t)(v[i >> 2] >> (8 * (i & 3)));
         }
-        // round 3: B2 (32 bytes) -> B3 (4 bytes); lanes 0..3
+        wpos += R;
+        // round 2: B1 (256 bytes, lane l holds bytes [4l, 4l+4) = f1) -> B2 (32 bytes)
+        uint32_t p2 = __shfl_up(f1 >> 24, 1, 64);
+        if (lane == 0) p2 = 0;
+        const uint32_t f2 = nz_bytes(f1 ^ ((f1 << 8) | p2));
+        cnt = __popc(f2);
+        incl = wave_incl_scan(cnt);
+        R = __shfl(incl, 63, 64);
+        fail = fail || wpos + R > kAvail;
         if (!fail) {
-            uint64_t cur = 0;
-            uint32_t f = 0;
-            if (tid < 4) {
-                cur = reinterpret_cast<const uint64_t*>(s_b2)[tid];
-                const uint32_t prev = tid ? (uint32_t)s_b2[8 * tid - 1] : 0u;
-                f = re_flags(cur, prev);
-                s_b3[tid] = (uint8_t)f;
-            }
-            int R;
-            int p = block_scan(__popc(f), s_scan, &R);
-            fail = wpos + R > kAvail;
-            if (!fail && tid < 4) {
+            int pos = wpos + incl - cnt;
 #pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if ((f >> j) & 1u) s_out[wpos + p++] = (uint8_t)(cur >> (8 * j));
-            }
-            wpos += R;
-            __syncthreads();
+            for (int i = 0; i < 4; ++i)
+                if ((f2 >> i) & 1u) out[pos++] = (uint8_t)(f1 >> (8 * i));
         }
-        if (!fail && wpos >= kAvail - 4) fail = true;
-        csize = fail ? osize : wpos + 6;
+        wpos += R;
+        // round 3: B2 byte m = nibbles of lanes 2m, 2m+1 -> B3 (4 bytes)
+        const uint32_t b2 = (__shfl(f2, (2 * lane) & 63, 64) | (__shfl(f2, (2 * lane + 1) & 63, 64) << 4)) & 0xFFu;
+        uint32_t p3 = __shfl_up(b2, 1, 64);
+        if (lane == 0) p3 = 0;
+        const bool keep = lane < 32 && b2 != p3;
+        const uint64_t m3 = __ballot(keep);
+        b3 = (uint32_t)m3;
+        R = __popcll(m3);
+        fail = fail || wpos + R > kAvail;
+        if (!fail && keep) out[wpos + __popcll(m3 & ((1ull << lane) - 1))] = (uint8_t)b2;
+        wpos += R;
+        fail = fail || wpos >= kAvail - 4;
     }
+    int csize = Z == 0 ? 2 : (fail ? osize : wpos + 6);
     const bool raw = fail || csize >= osize;
     if (raw) csize = osize;
-    if (!raw && tid == 0) {
+    if (!raw && lane == 0) {
         if (Z != 0) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) s_out[wpos + j] = s_b3[j];
+            for (int j = 0; j < 4; ++j) out[wpos + j] = (uint8_t)(b3 >> (8 * j));
         }
-        s_out[csize - 2] = (uint8_t)(osize & 0xFF);
-        s_out[csize - 1] = (uint8_t)((osize >> 8) & 0xFF);
+        out[csize - 2] = (uint8_t)(osize & 0xFF);
+        out[csize - 1] = (uint8_t)((osize >> 8) & 0xFF);
     }
 
-    // chunk offset
-    if (tid == 0) publish(status, id, id == 0 ? kPrefix : kAgg, (uint64_t)csize);
-    if (tid < 64) {
-        const int64_t ex = id == 0 ? 0 : look_back(status, id);
-        if (tid == 0) {
-            s_excl = ex;
-            if (id != 0) publish(status, id, kPrefix, (uint64_t)(ex + csize));
-        }
-    }
-    __syncthreads();
-    const int64_t excl = s_excl, incl = excl + csize;
+    // chunk offset by decoupled look-back
+    if (lane == 0) publish(status, id, id == 0 ? kPrefix : kAgg, (uint64_t)csize);
+    const int64_t excl = id == 0 ? 0 : look_back(status, id);
+    if (lane == 0 && id != 0) publish(status, id, kPrefix, (uint64_t)(excl + csize));
+    const int64_t incl_off = excl + csize;
+    wave_sync();
 
-    // payload store at an arbitrary byte offset: byte head, aligned words, byte tail
-    const uint8_t* sb = raw ? inb : s_out;
-    uint8_t* d = out + 8 + 2 * g.nchunks + excl;
+    // store at an arbitrary byte offset: head bytes, funnel-shifted words, tail bytes
+    uint8_t* d = dst_stream + 8 + 2 * g.nchunks + excl;
     int head = (int)((4 - ((uintptr_t)d & 3)) & 3);
     if (head > csize) head = csize;
-    if (tid < head) d[tid] = sb[tid];
+    const uint32_t* sw = raw ? reinterpret_cast<const uint32_t*>(src) : W.out;
+    const uint8_t* sb = raw ? src : out;
+    if (lane < head) d[lane] = sb[lane];
     const int nw = (csize - head) >> 2;
     uint32_t* dw = reinterpret_cast<uint32_t*>(d + head);
-    for (int w = tid; w < nw; w += kThreads) {
-        const int k = head + 4 * w;
-        dw[w] = (uint32_t)sb[k] | ((uint32_t)sb[k + 1] << 8) | ((uint32_t)sb[k + 2] << 16) |
-                ((uint32_t)sb[k + 3] << 24);
-    }
+    for (int w = lane; w < nw; w += 64)
+        dw[w] = head == 0 ? sw[w] : (sw[w] >> (8 * head)) | (sw[w + 1] << (32 - 8 * head));
     const int tail0 = head + 4 * nw;
-    if (tid < csize - tail0) d[tail0 + tid] = sb[tail0 + tid];
+    if (lane < csize - tail0) d[tail0 + lane] = sb[tail0 + lane];
 
-    if (tid == 0) {
-        reinterpret_cast<uint16_t*>(out + 8)[id] = (uint16_t)csize;
-        if (cp.last_of_plane) cum[(3 - cp.lv) * 32 + cp.plane] = (uint64_t)incl;
+    if (lane == 0) {
+        reinterpret_cast<uint16_t*>(dst_stream + 8)[id] = (uint16_t)csize;
+        if (cp.last_of_plane) cum[(3 - cp.lv) * 32 + cp.plane] = (uint64_t)incl_off;
         if (id == g.nchunks - 1) {
-            *reinterpret_cast<uint64_t*>(out) = (uint64_t)g.host_size;
-            *outsize = (uint64_t)(8 + 2 * g.nchunks + incl);
+            *reinterpret_cast<uint64_t*>(dst_stream) = (uint64_t)g.host_size;
+            *outsize = (uint64_t)(8 + 2 * g.nchunks + incl_off);
         }
     }
 }
@@ -526,8 +528,9 @@ extern "C" int pa_lossless_encode(const uint8_t* d_bp, const pa_layout* L, uint8
     if (blocks > 1024) blocks = 1024;
     hipLaunchKernelGGL(encode_init_kernel, dim3(blocks), dim3(256), 0, st, status, L->nchunks, counter);
     PA_LAUNCH_CHECK();
-    hipLaunchKernelGGL(encode_kernel, dim3((unsigned)L->nchunks), dim3(kThreads), 0, st, d_bp, g, d_out, d_cum,
-                       d_outsize, status, counter);
+    const unsigned eblocks = (unsigned)((L->nchunks + kWaves - 1) / kWaves);
+    hipLaunchKernelGGL(encode_kernel, dim3(eblocks), dim3(64 * kWaves), 0, st, d_bp, g, d_out, d_cum, d_outsize,
+                       status, counter);
     PA_LAUNCH_CHECK();
     return PA_OK;
 }
