@@ -37,6 +37,9 @@ def parse():
     ap.add_argument("--shape", type=int, nargs=3, default=[384, 384, 256], metavar=("NX", "NY", "NZ"))
     ap.add_argument("--rel-eb", type=float, default=1e-6)
     ap.add_argument("--targets", type=float, nargs="+", default=[1e-2, 1e-3, 1e-4, 1e-5, 1e-6])
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="run one device-to-device copy of this many MiB first (a known-size kernel for "
+                         "scaling HBM byte counters)")
     return ap.parse_args()
 
 
@@ -54,6 +57,13 @@ def main():
 
     torch.cuda.init()
     dev = torch.device("cuda", 0)
+    if args.calibrate_copy_mib > 0:
+        src = torch.ones(args.calibrate_copy_mib << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     nx, ny, nz = args.shape
     field = fields.density(nx, ny, nz, seed=2, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()

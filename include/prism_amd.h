@@ -131,8 +131,11 @@ int pa_lossless_decode(const uint8_t* d_in, int64_t in_len, const pa_layout* L, 
                        void* d_ws, size_t ws_bytes, void* stream);
 /* lossless_decode_progressive (zbpc.cu:560-599, kernel 368-473).  Decodes
  * every chunk that holds a byte of a requested device bit-plane (the covering
- * set; the reference's own selection loses plane heads, see the design
- * notes).  d_bytes_read (may be NULL) accumulates the payload bytes read.
+ * set).  The reference selects host chunks [begin*cb, end*cb) of each level;
+ * because host planes are longer than device planes (io.hpp:104-132 vs
+ * bitplane.cu:10-53) that selection misses the first bytes of a device plane
+ * in rounds after the first.  d_bytes_read (may be NULL) accumulates the
+ * payload bytes read.
  * The workspace must stay untouched between calls on the same stream when
  * offsets_ready != 0 (offsets are then reused). */
 int pa_lossless_decode_window(const uint8_t* d_in, int64_t in_len, const pa_layout* L,

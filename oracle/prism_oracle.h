@@ -10,7 +10,10 @@
  * Parity status of each function is stated next to it: "pinned" means a test
  * checks it against a closed form, an invariant or an independent emulation
  * of the reference's device code; "parity unpinned" means only the
- * reference's own run could pin the exact bits (see the notes there).
+ * reference's own run could pin the exact bits (see the notes there).  The
+ * reference holds no test vectors, fixtures or known-answer tests for this
+ * path, and it cannot be built here (CUDA only), so every pin below is a
+ * closed form, an invariant, an emulation or a brute force.
  *
  * Conventions (all citations are paths inside the reference tree):
  *   - arrays are T[nz][ny][nx], x fastest;
@@ -57,13 +60,19 @@ typedef struct po_layout {
     int64_t trailer_fixed; /* 1040 bytes before the anchors */
 } po_layout;
 
+/* Pinned: sizes of eight shapes against tests/golden/layout_sizes.json (the
+ * sizes restated from io.hpp:104-132 and bitplane.cu:10-53 by hand), host
+ * strides >= device strides, level counts = lattice point counts. */
 void po_layout_init(po_layout* L, uint32_t nx, uint32_t ny, uint32_t nz);
 
 /* Level-major position of grid point (gx,gy,gz) in the code array.
- * InterpolationPredictor.cu:2070-2083. */
+ * InterpolationPredictor.cu:2070-2083.  Pinned: equals the point's rank in
+ * raster order among the points of its level (brute force over all points). */
 int64_t po_lm_index(const po_layout* L, uint32_t gx, uint32_t gy, uint32_t gz);
 
-/* ---- error bound schedule, InterpolationPredictor.cu:2248-2271, 1825-1839 ---- */
+/* ---- error bound schedule, InterpolationPredictor.cu:2248-2271, 1825-1839 ----
+ * Pinned: alpha at its breakpoints and midpoints (closed form); per-level
+ * bound eb / min(alpha^k, 4) is met and more than half used on noise. */
 double po_alpha(double rel_eb);
 /* per-level (eb_r, ebx2) in T precision for unit u; is_double selects T */
 void po_level_eb_f32(float eb_r, float ebx2, double alpha, double beta, int unit,
@@ -71,14 +80,21 @@ void po_level_eb_f32(float eb_r, float ebx2, double alpha, double beta, int unit
 void po_level_eb_f64(double eb_r, double ebx2, double alpha, double beta, int unit,
                      double* cur_eb_r, double* cur_ebx2);
 
-/* ---- extrema, compressor.cu:228-302 ---- */
+/* ---- extrema, compressor.cu:228-302 ----
+ * Pinned: the trailer's range equals numpy's max - min (pipeline test). */
 void po_extrema_f32(const float* in, int64_t n, double out_minmax[2]);
 void po_extrema_f64(const double* in, int64_t n, double out_minmax[2]);
 
 /* ---- predictor: interpolation kernel, InterpolationPredictor.cu:2137-2173 ----
  * flags[0..3] = use_md[level_id], flags[4..7] = reverse[level_id].
  * qc must hold L->code_len int32 and is fully written (pads and the anchor
- * slots are set to 0).  anchors holds L->n4 values. */
+ * slots are set to 0).  anchors holds L->n4 values.
+ * Pinned (construct, reconstruct, delta): linear fields give zero codes and
+ * exact reconstruction under every flag setting; single-code impulse
+ * responses equal hand-derived stencil weights (1-D, face, cube); the bound
+ * holds; zero delta codes leave old unchanged.  Parity unpinned: whether
+ * nvcc contracts any of these products into FMAs in the reference build
+ * (this oracle and the HIP library evaluate without contraction). */
 void po_spline_construct_f32(const float* data, const po_layout* L, double eb, double rel_eb,
                              const uint8_t flags[8], int32_t* qc, float* anchors, int nthreads);
 void po_spline_construct_f64(const double* data, const po_layout* L, double eb, double rel_eb,
@@ -101,17 +117,27 @@ void po_spline_delta_reconstruct_f64(const int32_t* qc, const po_layout* L, doub
 
 /* ---- auto-tuner, InterpolationPredictor.cu:880-967 + 2273-2355 ----
  * scores[15] receive the candidate error sums in the reference's index order.
- * Returns 0 if tuning ran, 1 if the shape leaves it undefined (flags all 0). */
+ * Returns 0 if tuning ran, 1 if the shape leaves it undefined (flags all 0).
+ * Pinned: fields varying along x only (z only) select x-first (z-first)
+ * order at every level; shapes with an axis < 17 give all-false flags.
+ * Parity unpinned: the exact score bits, which the reference sums with float
+ * atomics in no fixed order (here: a fixed-order double sum). */
 int po_autotune_f32(const float* data, const po_layout* L, double rel_eb,
                     double scores[15], uint8_t flags[8]);
 int po_autotune_f64(const double* data, const po_layout* L, double rel_eb,
                     double scores[15], uint8_t flags[8]);
-/* Host flag pick from the 15 scores, InterpolationPredictor.cu:2306-2355. */
+/* Host flag pick from the 15 scores, InterpolationPredictor.cu:2306-2355.
+ * Pinned: strict/non-strict comparisons checked on tie cases. */
 void po_pick_flags(const double scores[15], uint8_t flags[8]);
 
 /* ---- bit-plane shuffle, bitplane.cu:55-253 (closed form), 256-601 ----
  * mode 0 = NB (negabinary, progressive), 1 = SM (sign-magnitude variant).
- * bp holds L->host_size bytes and is fully written. */
+ * bp holds L->host_size bytes and is fully written.
+ * Pinned: equals a lane-by-lane emulation of the reference's warp network
+ * (bitplane.cu:90-132, its inverse 292-318); NB/SM are involutions; single
+ * codes land on the predicted bytes and bits; windowed inverses add up.
+ * Parity unpinned: the phantom-tail bytes past the last real code, which the
+ * reference fills from reads past its code array (here: zero codes). */
 void po_bitshuffle(const int32_t* qc, const po_layout* L, int mode, uint8_t* bp);
 /* inverse; writes qc[n4 .. real_end) (other entries untouched) */
 void po_inverse_bitshuffle(const uint8_t* bp, const po_layout* L, int mode, int32_t* qc);
@@ -121,7 +147,9 @@ void po_inverse_bitshuffle_window(const uint8_t* bp, const po_layout* L, int mod
 
 /* ---- chunk codec, RZE.hpp:10-167 ----
  * Encodes one chunk of csize bytes; returns the encoded length, or -1 when
- * the chunk must be stored raw. out needs PO_CS bytes. */
+ * the chunk must be stored raw. out needs PO_CS bytes.
+ * Pinned: a hand-encoded example byte for byte, the all-zero chunk (footer
+ * only), raw storage of incompressible chunks, size limits, round trips. */
 int po_rze_encode_chunk(const uint8_t* in, int csize, uint8_t* out);
 /* Decodes one encoded chunk of clen bytes; returns decoded length or -1. */
 int po_rze_decode_chunk(const uint8_t* in, int clen, uint8_t* out);
@@ -129,7 +157,10 @@ int po_rze_decode_chunk(const uint8_t* in, int clen, uint8_t* out);
 /* ---- lossless stream, zbpc.cu:155-263 + 476-520 ----
  * out needs po_stream_bound() bytes.  Writes header, size table, payloads
  * and the zero pad to 8; returns the padded length.  cum[4*32] receives the
- * inclusive payload size through the last chunk of each (lr, plane). */
+ * inclusive payload size through the last chunk of each (lr, plane).
+ * Pinned: stream round trip, size table and cum table consistent with the
+ * payload, window decodes select exactly the covering chunks, and reading
+ * with the reference's selection down to the bound reads each byte once. */
 int64_t po_stream_bound(const po_layout* L);
 int64_t po_lossless_encode(const uint8_t* bp, const po_layout* L, uint8_t* out, uint64_t cum[128]);
 /* full decode, zbpc.cu:265-365: returns 0 on success, <0 on a malformed stream */
@@ -146,7 +177,12 @@ int64_t po_lossless_decode_window(const uint8_t* in, int64_t in_len, const po_la
 /* ---- strategy search, dataloader.cu:21-208 ----
  * Updates begin/end in place (begin <- end, end <- max(end, choice)) and
  * returns maxlevel.  uses_search is set to 1 when the search ran, 0 when
- * the 'load all' path was taken. */
+ * the 'load all' path was taken.
+ * Pinned: costs equal the double-precision evaluation of the reference's
+ * formula; the search equals a literal emulation of its 33 x 1024 tree
+ * reductions with their tie-breaks; update rules checked case by case.
+ * Parity unpinned: cost boundary cases where CUDA's pow/ceil could round an
+ * integer cost differently (exact integer arithmetic here). */
 int po_find_strategy(const uint64_t cum[128], int32_t begin[4], int32_t end[4], double eb,
                      double target_error, int* uses_search);
 /* raw search result for a given target cost (no begin/end update) */
@@ -154,7 +190,10 @@ void po_strategy_search(const uint64_t cum[128], uint64_t target_cost, int32_t c
                         uint64_t* best_value);
 uint64_t po_strategy_cost(int lr, int b);
 
-/* ---- whole pipelines (compressor.cu:120-190, prism.cu:9-151) ---- */
+/* ---- whole pipelines (compressor.cu:120-190, prism.cu:9-151) ----
+ * Pinned: stream layout fields (header size, flags, range, anchors = raw
+ * values at the 16-multiples, pad), bound met by decompress and by every
+ * progressive round (tests/test_oracle_pipeline.py). */
 /* compress: returns total stream length (payload + pad + trailer), writes
  * flags used into flags_out.  mode 0 = NB (-prog), 1 = SM.  abs_mode != 0
  * treats eb_or_rel as an absolute bound (range stored as 1.0). */

@@ -107,6 +107,48 @@ def test_error_bound(oracle, dtype, shape):
             assert err <= eb + 2 * ulp, (flags, rel, err / eb)
 
 
+def test_alpha_schedule(oracle):
+    """alpha is piecewise linear in rel_eb through (1e-1, 2), (1e-2, 1.75),
+    (1e-3, 1.5), (1e-4, 1.25), (1e-5, 1) and flat outside
+    (InterpolationPredictor.cu:2248-2271)."""
+    pts = [(1e-1, 2.0), (1e-2, 1.75), (1e-3, 1.5), (1e-4, 1.25), (1e-5, 1.0)]
+    for r, a in pts:
+        assert oracle.alpha(r) == pytest.approx(a, abs=1e-15)
+    for (r1, a1), (r0, a0) in zip(pts, pts[1:]):
+        assert oracle.alpha(0.5 * (r0 + r1)) == pytest.approx(0.5 * (a0 + a1), rel=1e-12)
+    assert oracle.alpha(0.7) == 2.0 and oracle.alpha(1e-9) == 1.0
+
+
+def _point_level(shape):
+    """Level of every grid point: k for unit 2^k, 4 for the anchors."""
+    z, y, x = np.meshgrid(*[np.arange(d) for d in shape], indexing="ij")
+    m = x | y | z | 16
+    return np.log2(m & -m).astype(int)
+
+
+@pytest.mark.parametrize("rel", [1e-1, 1e-3, 1e-6])
+def test_level_bounds_follow_alpha_beta(oracle, rel):
+    """Level k quantises with bound eb / min(alpha^k, beta = 4): on a noise
+    field every error stays within it and the largest error of each level
+    exceeds half of it, so a schedule that is off by a factor 2 (no beta
+    clamp, alpha inverted, one level shifted) fails."""
+    shape = (65, 65, 65)
+    data = np.random.default_rng(5).uniform(-1, 1, shape)
+    eb = rel * float(data.max() - data.min())
+    flags = np.zeros(8, np.uint8)
+    qc, an = oracle.spline_construct(data, eb, rel, flags)
+    rec = oracle.spline_reconstruct(qc, an, shape, eb, rel, flags)
+    err = np.abs(rec - data)
+    lv = _point_level(shape)
+    a = oracle.alpha(rel)
+    assert not err[lv == 4].any()
+    for k in range(4):
+        bound = eb / min(a ** k, 4.0)
+        e = err[lv == k]
+        assert e.max() <= bound * (1 + 1e-12), (k, e.max() / bound)
+        assert e.max() > 0.5 * bound, (k, e.max() / bound)
+
+
 def test_delta_reconstruction(oracle):
     """Zero delta codes leave the old field unchanged; delta + base codes
     reconstruct close to the full codes (differences are rounding only)."""
