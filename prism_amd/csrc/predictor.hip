@@ -215,54 +215,34 @@ __device__ __forceinline__ T pred_cube(const Ctx<T>& t, int x, int y, int z) {
     return pred;
 }
 
-template <typename T, int U, int K, int M>
-__device__ __forceinline__ void visit(Ctx<T>& t, int x, int y, int z, bool inclusive) {
-    const int gx = t.ox + x, gy = t.oy + y, gz = t.oz + z;
-    bool ok = inclusive ? (x <= 16 && y <= 16 && z <= 16)
-                        : (x < 16 && y < 16 && z < 16 + (t.c1z ? 0 : 1));
-    ok = ok && gx < t.g.nx && gy < t.g.ny && gz < t.g.nz;
-    if (!ok) return;
-    T pred;
-    if constexpr (K == BLUE || K == BLUE_R) {
-        pred = pred_1d<T, U, 2>(t, x, y, z);
-    } else if constexpr (K == YELLOW || K == YELLOW_R) {
-        pred = pred_1d<T, U, 1>(t, x, y, z);
-    } else if constexpr (K == HOLLOW || K == HOLLOW_R) {
-        pred = pred_1d<T, U, 0>(t, x, y, z);
+// Stage geometry: point p (0 <= p < total) of stage K at unit U, in the
+// reference's thread-visiting order (it defines the tuner's lane assignment
+// p mod 256; the other modes do not depend on it).
+template <int U, int K>
+__host__ __device__ constexpr int stage_total() {
+    constexpr int N = 16 / (2 * U);
+    if constexpr (K <= BLUE_R) {
+        constexpr int XC = (K == HOLLOW || K == HOLLOW_R) ? N : ((K == YELLOW_R || K == BLUE_R) ? 2 * N + 1 : N + 1);
+        constexpr int YC = (K == YELLOW || K == YELLOW_R) ? N : ((K == HOLLOW || K == BLUE_R) ? 2 * N + 1 : N + 1);
+        constexpr int ZC = (K == BLUE || K == BLUE_R) ? N : ((K == YELLOW || K == HOLLOW) ? 2 * N + 1 : N + 1);
+        return XC * YC * ZC;
     } else if constexpr (K == LINE) {
-        if ((z % (2 * U)) > 0) pred = pred_1d<T, U, 2>(t, x, y, z);
-        else if ((y % (2 * U)) > 0) pred = pred_1d<T, U, 1>(t, x, y, z);
-        else pred = pred_1d<T, U, 0>(t, x, y, z);
+        return 3 * N * (N + 1) * (N + 1);
     } else if constexpr (K == FACE) {
-        pred = pred_face<T, U>(t, x, y, z);
+        return 3 * N * N * (N + 1);
     } else {
-        pred = pred_cube<T, U>(t, x, y, z);
-    }
-    T& v = t.s[sidx(x, y, z)];
-    constexpr int LEVEL = U == 1 ? 0 : (U == 2 ? 1 : (U == 4 ? 2 : 3));
-    if constexpr (M == TUNE) {
-        T e = v - pred;
-        t.acc += (double)(e < (T)0 ? -e : e);
-    } else if constexpr (M == COMPRESS) {
-        const T err = v - pred;
-        T code = (err < (T)0 ? -err : err) * t.eb_r + (T)1;
-        code = err < (T)0 ? -code : code;
-        const int q = (int)(code / (T)2);
-        code = (T)q;
-        if (x < 16 && y < 16 && z < 16) t.qc_out[lm_index<LEVEL>(t.g, gx, gy, gz)] = q;
-        v = pred + code * t.ebx2;
-    } else {
-        const int32_t q = t.qc_in[lm_index<LEVEL>(t.g, gx, gy, gz)];
-        v = pred + (T)q * t.ebx2;
+        return N * N * N;
     }
 }
 
-// One stage: the reference's thread-visiting order (it defines the tuner's
-// lane assignment p mod 256; other modes do not depend on it).
-template <typename T, int U, int K, int M>
-__device__ __forceinline__ void stage(Ctx<T>& t, bool inclusive) {
+template <int U, int K>
+__host__ __device__ constexpr int stage_iters() {
+    return (stage_total<U, K>() + kThreads - 1) / kThreads;
+}
+
+template <int U, int K>
+__device__ __forceinline__ void stage_point(int p, int& x, int& y, int& z) {
     constexpr int N = 16 / (2 * U);
-    const int tid = threadIdx.x;
     if constexpr (K <= BLUE_R) {
         constexpr int X0 = (K == HOLLOW || K == HOLLOW_R) ? U : 0;
         constexpr int XS = (K == YELLOW_R || K == BLUE_R) ? U : 2 * U;
@@ -272,61 +252,396 @@ __device__ __forceinline__ void stage(Ctx<T>& t, bool inclusive) {
         constexpr int YC = (K == YELLOW || K == YELLOW_R) ? N : ((K == HOLLOW || K == BLUE_R) ? 2 * N + 1 : N + 1);
         constexpr int Z0 = (K == BLUE || K == BLUE_R) ? U : 0;
         constexpr int ZS = (K == YELLOW || K == HOLLOW) ? U : 2 * U;
-        constexpr int ZC = (K == BLUE || K == BLUE_R) ? N : ((K == YELLOW || K == HOLLOW) ? 2 * N + 1 : N + 1);
-        constexpr int TOTAL = XC * YC * ZC;
-        for (int p = tid; p < TOTAL; p += kThreads) {
-            const int ix = p % XC, iy = (p / XC) % YC, iz = p / (XC * YC);
-            visit<T, U, K, M>(t, X0 + XS * ix, Y0 + YS * iy, Z0 + ZS * iz, inclusive);
-        }
+        x = X0 + XS * (p % XC);
+        y = Y0 + YS * ((p / XC) % YC);
+        z = Z0 + ZS * (p / (XC * YC));
     } else if constexpr (K == LINE) {
         constexpr int L = N * (N + 1) * (N + 1), Q = (N + 1) * (N + 1);
-        for (int p = tid; p < 3 * L; p += kThreads) {
-            const int grp = p / L, m = p % L;
-            const int i = m / Q, j = (m % Q) / (N + 1), k = (m % Q) % (N + 1);
-            int x, y, z;
-            if (grp == 0) { x = 2 * U * i + U; y = 2 * U * j; z = 2 * U * k; }
-            else if (grp == 1) { x = 2 * U * k; y = 2 * U * i + U; z = 2 * U * j; }
-            else { x = 2 * U * j; y = 2 * U * k; z = 2 * U * i + U; }
-            visit<T, U, K, M>(t, x, y, z, inclusive);
-        }
+        const int grp = p / L, m = p % L;
+        const int i = m / Q, j = (m % Q) / (N + 1), k = (m % Q) % (N + 1);
+        if (grp == 0) { x = 2 * U * i + U; y = 2 * U * j; z = 2 * U * k; }
+        else if (grp == 1) { x = 2 * U * k; y = 2 * U * i + U; z = 2 * U * j; }
+        else { x = 2 * U * j; y = 2 * U * k; z = 2 * U * i + U; }
     } else if constexpr (K == FACE) {
         constexpr int L = N * N * (N + 1), Q = N * N;
-        for (int p = tid; p < 3 * L; p += kThreads) {
-            const int grp = p / L, m = p % L;
-            const int i = m / Q, j = (m % Q) / N, k = (m % Q) % N;
-            int x, y, z;
-            if (grp == 0) { x = 2 * U * i; y = 2 * U * j + U; z = 2 * U * k + U; }
-            else if (grp == 1) { x = 2 * U * k + U; y = 2 * U * i; z = 2 * U * j + U; }
-            else { x = 2 * U * j + U; y = 2 * U * k + U; z = 2 * U * i; }
-            visit<T, U, K, M>(t, x, y, z, inclusive);
-        }
+        const int grp = p / L, m = p % L;
+        const int i = m / Q, j = (m % Q) / N, k = (m % Q) % N;
+        if (grp == 0) { x = 2 * U * i; y = 2 * U * j + U; z = 2 * U * k + U; }
+        else if (grp == 1) { x = 2 * U * k + U; y = 2 * U * i; z = 2 * U * j + U; }
+        else { x = 2 * U * j + U; y = 2 * U * k + U; z = 2 * U * i; }
     } else {
         constexpr int Q = N * N;
-        for (int p = tid; p < N * N * N; p += kThreads) {
-            const int i = p / Q, j = (p % Q) / N, k = (p % Q) % N;
-            visit<T, U, K, M>(t, 2 * U * i + U, 2 * U * j + U, 2 * U * k + U, inclusive);
-        }
+        const int i = p / Q, j = (p % Q) / N, k = (p % Q) % N;
+        x = 2 * U * i + U;
+        y = 2 * U * j + U;
+        z = 2 * U * k + U;
     }
-    __syncthreads();
 }
 
-// The three stages of one level; the last stage of level 0 uses the
-// exclusive border outside tuning (InterpolationPredictor.cu:1940-2025).
+// Whether point (x, y, z) of the tile is predicted in this sweep: inside the
+// tile (the +1 halo included for inclusive stages; the last stage of the
+// tile row keeps its far plane) and inside the field.
+template <typename T>
+__device__ __forceinline__ bool visited(const Ctx<T>& t, int x, int y, int z, bool inclusive) {
+    const bool in = inclusive ? (x <= 16 && y <= 16 && z <= 16)
+                              : (x < 16 && y < 16 && z < 16 + (t.c1z ? 0 : 1));
+    return in && t.ox + x < t.g.nx && t.oy + y < t.g.ny && t.oz + z < t.g.nz;
+}
+
+template <typename T, int U, int K>
+__device__ __forceinline__ T predict(const Ctx<T>& t, int x, int y, int z) {
+    if constexpr (K == BLUE || K == BLUE_R) {
+        return pred_1d<T, U, 2>(t, x, y, z);
+    } else if constexpr (K == YELLOW || K == YELLOW_R) {
+        return pred_1d<T, U, 1>(t, x, y, z);
+    } else if constexpr (K == HOLLOW || K == HOLLOW_R) {
+        return pred_1d<T, U, 0>(t, x, y, z);
+    } else if constexpr (K == LINE) {
+        if ((z % (2 * U)) > 0) return pred_1d<T, U, 2>(t, x, y, z);
+        if ((y % (2 * U)) > 0) return pred_1d<T, U, 1>(t, x, y, z);
+        return pred_1d<T, U, 0>(t, x, y, z);
+    } else if constexpr (K == FACE) {
+        return pred_face<T, U>(t, x, y, z);
+    } else {
+        return pred_cube<T, U>(t, x, y, z);
+    }
+}
+
+template <int U>
+constexpr int level_of() {
+    return U == 1 ? 0 : (U == 2 ? 1 : (U == 4 ? 2 : 3));
+}
+
+// The lane id through an opaque move: index arithmetic derived from it in
+// one phase of the kernel is not merged with that of another phase and kept
+// live in registers between them.
+__device__ __forceinline__ int lane_id() {
+    int v;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "v"((int)threadIdx.x));
+    return v;
+}
+
+// Barrier for LDS producer/consumer hand-off between stages: waits for this
+// wave's LDS operations only, so outstanding global loads (prefetched codes)
+// and stores (emitted codes) stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// Quantise v against pred, write the reconstruction back and return the
+// code (InterpolationPredictor.cu:1327-1341).
+template <typename T>
+__device__ __forceinline__ int32_t quantize(const Ctx<T>& t, T& v, T pred) {
+    const T err = v - pred;
+    T code = (err < (T)0 ? -err : err) * t.eb_r + (T)1;
+    code = err < (T)0 ? -code : code;
+    const int qv = (int)(code / (T)2);
+    code = (T)qv;
+    v = pred + code * t.ebx2;
+    return qv;
+}
+
+// Decompression first fills the LDS slot of every point with the bit
+// pattern of its own code (anchors hold their value, points of skipped
+// levels 0); a stage then reads the slot of the point it reconstructs and
+// overwrites it with pred + code * ebx2.  Predictions read only points of
+// earlier stages, which already hold values.
+__device__ __forceinline__ float code_slot(int32_t q, float) { return __int_as_float(q); }
+__device__ __forceinline__ double code_slot(int32_t q, double) {
+    return __longlong_as_double((long long)(uint32_t)q);
+}
+__device__ __forceinline__ int32_t slot_code(float v) { return __float_as_int(v); }
+__device__ __forceinline__ int32_t slot_code(double v) { return (int32_t)(uint32_t)__double_as_longlong(v); }
+
+// ----------------------------------------------------------- point form --
+// Lane l visits points p = l + 256 * it of the stage (tuner, and the
+// multi-dimensional stages).
+template <typename T, int U, int K, int M>
+__device__ __forceinline__ void stage(Ctx<T>& t, bool inclusive) {
+    constexpr int TOTAL = stage_total<U, K>();
+    constexpr int LEVEL = level_of<U>();
+    const int lane = lane_id();
+#pragma unroll
+    for (int it = 0; it < stage_iters<U, K>(); ++it) {
+        const int p = lane + it * kThreads;
+        if (TOTAL % kThreads != 0 && p >= TOTAL) break;
+        int x, y, z;
+        stage_point<U, K>(p, x, y, z);
+        if (!visited(t, x, y, z, inclusive)) continue;
+        const T pred = predict<T, U, K>(t, x, y, z);
+        T& v = t.s[sidx(x, y, z)];
+        if constexpr (M == TUNE) {
+            const T e = v - pred;
+            t.acc += (double)(e < (T)0 ? -e : e);
+        } else if constexpr (M == COMPRESS) {
+            const int32_t qv = quantize(t, v, pred);
+            if (x < 16 && y < 16 && z < 16) t.qc_out[lm_index<LEVEL>(t.g, t.ox + x, t.oy + y, t.oz + z)] = qv;
+        } else {
+            v = pred + (T)slot_code(v) * t.ebx2;
+        }
+    }
+    lds_barrier();
+}
+
+// ------------------------------------------------------------ line form --
+// 1-D stages along y or z run line by line: lane l owns one line along the
+// prediction axis A (consecutive lanes are consecutive lines along x, so
+// their codes are neighbours in memory), reads the N + 1 known values of the
+// line once and predicts its N points with compile-time stencils.  Which
+// stencil a point uses depends only on its position along A and on the
+// tile's place in the field, so the choice is the same for every lane.
+// The values produced are those of the per-point rule above (same formula,
+// same operation order); only the lane assignment differs, and points of one
+// stage never read each other.
+enum Stencil { S_KEEP, S_LIN, S_QUAD_R, S_QUAD_L, S_CUBIC };
+
+template <int U, int K, bool EXCL>
+struct LineShape {
+    static constexpr int N = 16 / (2 * U);
+    static constexpr int A = (K == BLUE || K == BLUE_R) ? 2 : ((K == YELLOW || K == YELLOW_R) ? 1 : 0);
+    // P: first other axis (x, or y when A = x); Q: second (z, or y when A = z)
+    static constexpr bool FULL_P = (K == HOLLOW || K == YELLOW_R || K == BLUE_R);
+    static constexpr bool FULL_Q = (K == YELLOW || K == HOLLOW || K == BLUE_R);
+    static constexpr int SP = FULL_P ? U : 2 * U;
+    static constexpr int SQ = FULL_Q ? U : 2 * U;
+    // the exclusive last stage of level 0 visits x < 16, y < 16, z < 16 (its
+    // z = 16 plane would lie outside the field: only the last tile along z
+    // may visit it, and there oz + 16 >= nz)
+    static constexpr int CP = EXCL ? 16 : (FULL_P ? 2 * N + 1 : N + 1);
+    static constexpr int CQ = EXCL ? 16 : (FULL_Q ? 2 * N + 1 : N + 1);
+    static constexpr int LINES = CP * CQ;
+    static constexpr int ITERS = (LINES + kThreads - 1) / kThreads;
+    static constexpr int SLOTS = ITERS * N;
+    static constexpr int STA = A == 0 ? 1 : (A == 1 ? kTS : kTS * kTS);
+};
+
+// Stencil of point j of a line (position c = (2j+1)U along the axis), the
+// rule of InterpolationPredictor.cu:1269-1326 with c2 = (j >= 1) and
+// c3 = (j <= N - 2).
+template <int U>
+__device__ __forceinline__ int stencil_of(int j, int N, bool c1, int o, int nax) {
+    const int gc = o + (2 * j + 1) * U;
+    const bool c2 = j >= 1, c3 = j <= N - 2, c4 = gc + 3 * U < nax, c5 = gc + U < nax;
+    if ((c1 && !c2 && !c3) || (!c1 && !c2 && !(c3 && c4) && c5)) return S_LIN;
+    if ((c1 && !c2 && c3) || (!c1 && !c2 && c3 && c4)) return S_QUAD_R;
+    if ((c1 && c2 && !c3) || (!c1 && c2 && !(c3 && c4) && c5)) return S_QUAD_L;
+    if ((c1 && c2 && c3) || (!c1 && c2 && c3 && c4)) return S_CUBIC;
+    return S_KEEP;
+}
+
+template <typename T>
+__device__ __forceinline__ T apply_stencil(int k, T t0, T t1, T t2, T t3) {
+    switch (k) {
+        case S_LIN: return lin(t1, t2);
+        case S_QUAD_R: return quad_r(t1, t2, t3);
+        case S_QUAD_L: return quad_l(t0, t1, t2);
+        case S_CUBIC: return cubic(t0, t1, t2, t3, false);
+        default: return t1;
+    }
+}
+
+struct LineLane {
+    int base;      // LDS index of the line's c = 0 point
+    bool on;       // line inside the tile's sweep and the field
+    bool own;      // line inside the 16^3 tile proper (codes are stored)
+    int64_t idx0;  // level-major index of point j = 0
+    int64_t st;    // level-major stride between consecutive points
+};
+
+template <typename T, int U, int K, bool EXCL>
+__device__ __forceinline__ LineLane line_lane(const Ctx<T>& t, int it) {
+    using S = LineShape<U, K, EXCL>;
+    LineLane r;
+    const int l = lane_id() + it * kThreads;
+    const int ip = l % S::CP, iq = l / S::CP;
+    const int pc = S::SP * ip, qc = S::SQ * iq;
+    int x, y, z;  // point j = 0
+    if constexpr (S::A == 0) { x = U; y = pc; z = qc; }
+    else if constexpr (S::A == 1) { x = pc; y = U; z = qc; }
+    else { x = pc; y = qc; z = U; }
+    const int gx = t.ox + x, gy = t.oy + y, gz = t.oz + z;
+    bool on = (S::LINES % kThreads == 0 || l < S::LINES);
+    if constexpr (S::A != 0) on = on && gx < t.g.nx;
+    if constexpr (S::A != 1) on = on && gy < t.g.ny;
+    if constexpr (S::A != 2) on = on && gz < t.g.nz;
+    r.on = on;
+    r.own = x < 16 && y < 16 && z < 16;
+    r.base = sidx(x, y, z) - U * S::STA;
+    constexpr int LEVEL = level_of<U>();
+    r.idx0 = lm_index<LEVEL>(t.g, gx, gy, gz);
+    r.st = 0;
+    if constexpr (S::N >= 2) {
+        // the index is affine along the line: the varying coordinate keeps
+        // its parity and its halves grow by one per point
+        const int64_t i1 = S::A == 0 ? lm_index<LEVEL>(t.g, gx + 2 * U, gy, gz)
+                                     : (S::A == 1 ? lm_index<LEVEL>(t.g, gx, gy + 2 * U, gz)
+                                                  : lm_index<LEVEL>(t.g, gx, gy, gz + 2 * U));
+        r.st = i1 - r.idx0;
+    }
+    return r;
+}
+
+template <typename T, int U, int K, bool EXCL, int M>
+__device__ __forceinline__ void line_stage(Ctx<T>& t) {
+    using S = LineShape<U, K, EXCL>;
+    constexpr int N = S::N;
+    const int o = S::A == 0 ? t.ox : (S::A == 1 ? t.oy : t.oz);
+    const int nax = S::A == 0 ? t.g.nx : (S::A == 1 ? t.g.ny : t.g.nz);
+    const bool c1 = S::A == 0 ? t.c1x : (S::A == 1 ? t.c1y : t.c1z);
+#pragma unroll
+    for (int it = 0; it < S::ITERS; ++it) {
+        const LineLane ln = line_lane<T, U, K, EXCL>(t, it);
+        if (!ln.on) continue;
+        T k[N + 1];
+#pragma unroll
+        for (int j = 0; j <= N; ++j) k[j] = t.s[ln.base + 2 * j * U * S::STA];
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            if (o + (2 * j + 1) * U >= nax) break;  // beyond the field; so are later points
+            const T pred = apply_stencil(stencil_of<U>(j, N, c1, o, nax), k[j > 0 ? j - 1 : 0], k[j], k[j + 1],
+                                         k[j + 2 <= N ? j + 2 : 0]);
+            T& v = t.s[ln.base + (2 * j + 1) * U * S::STA];
+            if constexpr (M == COMPRESS) {
+                const int32_t qv = quantize(t, v, pred);
+                if (ln.own) t.qc_out[ln.idx0 + j * ln.st] = qv;
+            } else {
+                v = pred + (T)slot_code(v) * t.ebx2;
+            }
+        }
+    }
+    lds_barrier();
+}
+
+// ---------------------------------------------------------- column form --
+// The last stage of level 0 in z-y-x order predicts along x at x = 1, 3,
+// .., 15 for y, z in 0..15.  Lane l takes x = 2(l mod 8) + 1,
+// y = (l / 8) mod 16 and z = l / 128 + 2 it: eight lanes cover the odd
+// points of a row, whose codes are adjacent, and each lane keeps one
+// stencil and one code stride.
+constexpr int kHollow0Iters = 8;
+
+struct ColumnLane {
+    int x, y, z0;
+    bool on;
+    int64_t idx0, dz;
+};
+
+template <typename T>
+__device__ __forceinline__ ColumnLane column_lane(const Ctx<T>& t) {
+    ColumnLane c;
+    const int l = lane_id();
+    c.x = 2 * (l & 7) + 1;
+    c.y = (l >> 3) & 15;
+    c.z0 = l >> 7;
+    c.on = t.ox + c.x < t.g.nx && t.oy + c.y < t.g.ny;
+    c.idx0 = lm_index<0>(t.g, t.ox + c.x, t.oy + c.y, t.oz + c.z0);
+    c.dz = 2 * t.g.lxy[0] - t.g.lxy[1];  // z -> z + 2 keeps the parity of z
+    return c;
+}
+
+template <typename T, int M>
+__device__ __forceinline__ void hollow0_stage(Ctx<T>& t) {
+    const ColumnLane c = column_lane(t);
+    if (c.on) {
+        const int j = c.x >> 1;
+        const int k = stencil_of<1>(j, 8, t.c1x, t.ox, t.g.nx);
+        const int i0 = j > 0 ? c.x - 3 : 0, i3 = c.x + 3 <= 16 ? c.x + 3 : 0;
+#pragma unroll
+        for (int it = 0; it < kHollow0Iters; ++it) {
+            const int z = c.z0 + 2 * it;
+            if (t.oz + z >= t.g.nz) break;
+            const int base = sidx(0, c.y, z);
+            const T pred = apply_stencil(k, t.s[base + i0], t.s[base + c.x - 1], t.s[base + c.x + 1],
+                                         t.s[base + i3]);
+            T& v = t.s[base + c.x];
+            if constexpr (M == COMPRESS) {
+                const int32_t qv = quantize(t, v, pred);
+                t.qc_out[c.idx0 + it * c.dz] = qv;
+            } else {
+                v = pred + (T)slot_code(v) * t.ebx2;
+            }
+        }
+    }
+    lds_barrier();
+}
+
+// ---------------------------------------------------------------- levels --
+// The three stages of one level; outside tuning the last stage of level 0
+// uses the exclusive border (InterpolationPredictor.cu:1940-2025).
 template <typename T, int U, int M>
 __device__ __forceinline__ void level(Ctx<T>& t, bool md, bool rev) {
-    const bool last_incl = !(M != TUNE && U == 1);
+    constexpr bool EX = M != TUNE && U == 1;
     if (md) {
         stage<T, U, LINE, M>(t, true);
         stage<T, U, FACE, M>(t, true);
-        stage<T, U, CUBE, M>(t, last_incl);
+        stage<T, U, CUBE, M>(t, !EX);
+    } else if constexpr (M == TUNE) {
+        if (rev) {
+            stage<T, U, HOLLOW_R, M>(t, true);
+            stage<T, U, YELLOW_R, M>(t, true);
+            stage<T, U, BLUE_R, M>(t, true);
+        } else {
+            stage<T, U, BLUE, M>(t, true);
+            stage<T, U, YELLOW, M>(t, true);
+            stage<T, U, HOLLOW, M>(t, true);
+        }
     } else if (rev) {
-        stage<T, U, HOLLOW_R, M>(t, true);
-        stage<T, U, YELLOW_R, M>(t, true);
-        stage<T, U, BLUE_R, M>(t, last_incl);
+        line_stage<T, U, HOLLOW_R, false, M>(t);
+        line_stage<T, U, YELLOW_R, false, M>(t);
+        line_stage<T, U, BLUE_R, EX, M>(t);
     } else {
-        stage<T, U, BLUE, M>(t, true);
-        stage<T, U, YELLOW, M>(t, true);
-        stage<T, U, HOLLOW, M>(t, last_incl);
+        line_stage<T, U, BLUE, false, M>(t);
+        line_stage<T, U, YELLOW, false, M>(t);
+        if constexpr (EX) hollow0_stage<T, M>(t);
+        else line_stage<T, U, HOLLOW, false, M>(t);
+    }
+}
+
+// Decompression start: the slot of every tile point (17^3, halo included)
+// gets its code, its anchor or 0.  Level L's points are the multiples of
+// 2^L in the tile that are not all multiples of 2^(L+1); lanes run along x
+// so each load instruction reads consecutive codes.  All loads are issued
+// before the first LDS write.
+template <int L>
+struct FillShape {
+    static constexpr int U = 1 << L, E = 16 / U + 1, TOTAL = E * E * E;
+    static constexpr int ITERS = (TOTAL + kThreads - 1) / kThreads;
+};
+
+template <int L>
+__device__ __forceinline__ int fill_slot(int lane, int it) {  // LDS slot of the lane's it-th point, or -1
+    using F = FillShape<L>;
+    const int p = lane + it * kThreads;
+    const int i = p % F::E, j = (p / F::E) % F::E, k = p / (F::E * F::E);
+    const bool mine = (F::TOTAL % kThreads == 0 || p < F::TOTAL) && ((i | j | k) & 1);
+    return mine ? sidx(i * F::U, j * F::U, k * F::U) : -1;
+}
+
+template <typename T, int L>
+__device__ __forceinline__ void fill_load(const Ctx<T>& t, bool load, int32_t (&v)[FillShape<L>::ITERS]) {
+    using F = FillShape<L>;
+    const int lane = lane_id();
+#pragma unroll
+    for (int it = 0; it < F::ITERS; ++it) {
+        const int p = lane + it * kThreads;
+        const int i = p % F::E, j = (p / F::E) % F::E, k = p / (F::E * F::E);
+        const bool mine = (F::TOTAL % kThreads == 0 || p < F::TOTAL) && ((i | j | k) & 1);
+        const int gx = t.ox + i * F::U, gy = t.oy + j * F::U, gz = t.oz + k * F::U;
+        v[it] = 0;
+        if (load && mine && gx < t.g.nx && gy < t.g.ny && gz < t.g.nz) v[it] = t.qc_in[lm_index<L>(t.g, gx, gy, gz)];
+        // keep each iteration's address arithmetic next to its load: only
+        // the loaded codes stay live across the batch
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+template <typename T, int L>
+__device__ __forceinline__ void fill_store(T* s, const int32_t (&v)[FillShape<L>::ITERS]) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int it = 0; it < FillShape<L>::ITERS; ++it) {
+        const int slot = fill_slot<L>(lane, it);
+        if (slot >= 0) s[slot] = code_slot(v[it], T());
     }
 }
 
@@ -335,15 +650,26 @@ struct LevelEb {
     T eb_r[4], ebx2[4];
 };
 
+// Tile plus halo into LDS: all global loads of a lane are issued before the
+// first LDS write so the tile costs one memory latency.
 template <typename T>
 __device__ __forceinline__ void load_tile(T* s, const T* data, const Geom& g, int ox, int oy, int oz) {
-    for (int i = threadIdx.x; i < kTileVals; i += kThreads) {
+    constexpr int IT = (kTileVals + kThreads - 1) / kThreads;
+    T v[IT];
+    const int lane = lane_id();
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int i = lane + it * kThreads;
         const int x = i % kTS, y = (i / kTS) % kTS, z = i / (kTS * kTS);
         const int gx = ox + x, gy = oy + y, gz = oz + z;
-        T v = (T)0;
-        if (gx < g.nx && gy < g.ny && gz < g.nz)
-            v = data[(int64_t)gx + (int64_t)gy * g.nx + (int64_t)gz * g.nx * g.ny];
-        s[i] = v;
+        v[it] = (T)0;
+        if (i < kTileVals && gx < g.nx && gy < g.ny && gz < g.nz)
+            v[it] = data[(int64_t)gx + (int64_t)gy * g.nx + (int64_t)gz * g.nx * g.ny];
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int i = lane + it * kThreads;
+        if (i < kTileVals) s[i] = v[it];
     }
 }
 
@@ -351,7 +677,7 @@ __device__ __forceinline__ void load_tile(T* s, const T* data, const Geom& g, in
 // `interpolation_progressive_reverse` (decompress: anchors, or zero anchors
 // plus old for the delta form).
 template <typename T, int M>
-__global__ __launch_bounds__(kThreads) void spline_kernel(const T* __restrict__ data, Geom g,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void spline_kernel(const T* __restrict__ data, Geom g,
                                                           LevelEb<T> le, const uint8_t* __restrict__ flags8,
                                                           const int32_t* __restrict__ d_maxlevel,
                                                           int32_t* __restrict__ qc_out,
@@ -379,21 +705,49 @@ __global__ __launch_bounds__(kThreads) void spline_kernel(const T* __restrict__ 
     for (int i = 0; i < 8; ++i) fl[i] = flags8[i];
     const int maxlevel = d_maxlevel ? *d_maxlevel : 4;
 
+    // decompression: the previous reconstruction (delta form) and every
+    // code of the tile are requested before any is needed
+    constexpr int OIT = 4096 / kThreads;
+    T prev[OIT];
+    if constexpr (M == DECOMPRESS) {
+        if (old) {
+            const int lane = lane_id();
+#pragma unroll
+            for (int it = 0; it < OIT; ++it) {
+                const int i = lane + it * kThreads;
+                const int gx = t.ox + (i & 15), gy = t.oy + ((i >> 4) & 15), gz = t.oz + (i >> 8);
+                prev[it] = (T)0;
+                if (gx < g.nx && gy < g.ny && gz < g.nz)
+                    prev[it] = old[(int64_t)gx + (int64_t)gy * g.nx + (int64_t)gz * g.nx * g.ny];
+            }
+        }
+    }
+
     if constexpr (M == COMPRESS) {
         load_tile(tile, data, g, t.ox, t.oy, t.oz);
         __syncthreads();
         if (threadIdx.x == 0)
             anchors[bx + (int64_t)by * g.gx + (int64_t)bz * g.gx * g.gy] = tile[0];
     } else {
-        for (int i = threadIdx.x; i < kTileVals; i += kThreads) tile[i] = (T)0;
-        __syncthreads();
-        if (anchors_in && threadIdx.x < 8) {  // x_reset_scratch_data 1051-1061
-            const int ax = bx + (threadIdx.x & 1), ay = by + ((threadIdx.x >> 1) & 1),
-                      az = bz + (threadIdx.x >> 2);
-            if (ax < g.gx && ay < g.gy && az < g.gz)
-                tile[sidx(16 * (threadIdx.x & 1), 16 * ((threadIdx.x >> 1) & 1), 16 * (threadIdx.x >> 2))] =
-                    anchors_in[ax + (int64_t)ay * g.gx + (int64_t)az * g.gx * g.gy];
+        int32_t v3[FillShape<3>::ITERS], v2[FillShape<2>::ITERS], v1[FillShape<1>::ITERS], v0[FillShape<0>::ITERS];
+        fill_load<T, 3>(t, 3 <= maxlevel, v3);
+        fill_load<T, 2>(t, 2 <= maxlevel, v2);
+        fill_load<T, 1>(t, 1 <= maxlevel, v1);
+        fill_load<T, 0>(t, 0 <= maxlevel, v0);
+        T anc = (T)0;
+        int aslot = -1;
+        if (threadIdx.x < 8) {  // x_reset_scratch_data 1051-1061: the eight corner anchors
+            const int cx = threadIdx.x & 1, cy = (threadIdx.x >> 1) & 1, cz = threadIdx.x >> 2;
+            const int ax = bx + cx, ay = by + cy, az = bz + cz;
+            aslot = sidx(16 * cx, 16 * cy, 16 * cz);
+            if (anchors_in && ax < g.gx && ay < g.gy && az < g.gz)
+                anc = anchors_in[ax + (int64_t)ay * g.gx + (int64_t)az * g.gx * g.gy];
         }
+        fill_store<T, 3>(tile, v3);
+        fill_store<T, 2>(tile, v2);
+        fill_store<T, 1>(tile, v1);
+        fill_store<T, 0>(tile, v0);
+        if (aslot >= 0) tile[aslot] = anc;
         __syncthreads();
     }
 
@@ -403,13 +757,15 @@ __global__ __launch_bounds__(kThreads) void spline_kernel(const T* __restrict__ 
     if (0 <= maxlevel) { t.eb_r = le.eb_r[0]; t.ebx2 = le.ebx2[0]; level<T, 1, M>(t, fl[0], fl[4]); }
 
     if constexpr (M == DECOMPRESS) {
-        for (int i = threadIdx.x; i < 4096; i += kThreads) {
+        const int lane = lane_id();
+#pragma unroll
+        for (int it = 0; it < OIT; ++it) {
+            const int i = lane + it * kThreads;
             const int x = i & 15, y = (i >> 4) & 15, z = i >> 8;
             const int gx = t.ox + x, gy = t.oy + y, gz = t.oz + z;
             if (gx < g.nx && gy < g.ny && gz < g.nz) {
-                const int64_t gi = (int64_t)gx + (int64_t)gy * g.nx + (int64_t)gz * g.nx * g.ny;
                 const T v = tile[sidx(x, y, z)];
-                out[gi] = old ? v + old[gi] : v;
+                out[(int64_t)gx + (int64_t)gy * g.nx + (int64_t)gz * g.nx * g.ny] = old ? v + prev[it] : v;
             }
         }
     }

@@ -97,16 +97,22 @@ def test_predictor_parity(pa, oracle, dtype, shape):
             _lib.check(_lib.lib().pa_spline_reconstruct(P(qc), P(an), dt, C.byref(L), eb, rel, P(fl), None,
                                                          P(out), S()))
             assert np.array_equal(host(out).view(np.uint8), rec_o.view(np.uint8))
-            # delta form with a partial level range
-            ml = torch.tensor([2], dtype=torch.int32, device="cuda")
+            # partial level ranges: anchored form and delta form
             half = qc_o // 3
             old = rec_o * dtype(0.5)
-            dl_o = oracle.spline_delta_reconstruct(half, old, eb, rel, flags, maxlevel=2)
-            out2 = torch.empty(shape, dtype=d.dtype, device="cuda")
-            half_d, old_d = dev(half), dev(old)  # keep alive until the kernel has run
-            _lib.check(_lib.lib().pa_spline_delta_reconstruct(P(half_d), dt, C.byref(L), eb, rel, P(fl), P(ml),
-                                                               P(old_d), P(out2), S()))
-            assert np.array_equal(host(out2).view(np.uint8), dl_o.view(np.uint8))
+            half_d, old_d = dev(half), dev(old)  # keep alive until the kernels have run
+            for maxlevel in (0, 1, 2, 3):
+                ml = torch.tensor([maxlevel], dtype=torch.int32, device="cuda")
+                part_o = oracle.spline_reconstruct(qc_o, an_o, shape, eb, rel, flags, maxlevel=maxlevel)
+                out1 = torch.empty(shape, dtype=d.dtype, device="cuda")
+                _lib.check(_lib.lib().pa_spline_reconstruct(P(qc), P(an), dt, C.byref(L), eb, rel, P(fl), P(ml),
+                                                             P(out1), S()))
+                assert np.array_equal(host(out1).view(np.uint8), part_o.view(np.uint8))
+                dl_o = oracle.spline_delta_reconstruct(half, old, eb, rel, flags, maxlevel=maxlevel)
+                out2 = torch.empty(shape, dtype=d.dtype, device="cuda")
+                _lib.check(_lib.lib().pa_spline_delta_reconstruct(P(half_d), dt, C.byref(L), eb, rel, P(fl),
+                                                                   P(ml), P(old_d), P(out2), S()))
+                assert np.array_equal(host(out2).view(np.uint8), dl_o.view(np.uint8))
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
