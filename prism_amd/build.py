@@ -25,6 +25,13 @@ ARCH = os.environ.get("PRISM_AMD_ARCH", "gfx950")
 SOURCES = ["layout.cpp", "extrema.hip", "predictor.hip", "bitplane.hip", "codec.hip", "strategy.hip"]
 HEADERS = ["common.hpp"]
 
+# C++ API (prism::Compressor & co.) and the `prism` command line, host code
+# over the C ABI
+CPP = os.path.join(PKG, "cpp")
+CPP_SOURCES = ["parameters.cpp", "compressor.cpp", "prism.cpp"]
+CPP_LIB = os.path.join(OUT, "libprism_cpp.so")
+CLI = os.path.join(OUT, "prism")
+
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
           "-I", os.path.join(ROOT, "include"), "-I", SRC]
 DEVICE = [f"--offload-arch={ARCH}", "-fhip-fp32-correctly-rounded-divide-sqrt",
@@ -58,19 +65,50 @@ def _compile(src: str, force: bool) -> str:
     return obj
 
 
+def _cpp_headers():
+    inc = os.path.join(CPP, "include", "prism")
+    return [os.path.join(inc, f) for f in sorted(os.listdir(inc))] + [os.path.join(ROOT, "include", "prism_amd.h")]
+
+
+def _compile_cpp(src: str, force: bool) -> str:
+    path = os.path.join(CPP, "src", src)
+    obj = os.path.join(OUT, "cpp_" + os.path.splitext(src)[0] + ".o")
+    if not force and not _stale(obj, [path] + _cpp_headers()):
+        return obj
+    cmd = [HIPCC, "-O2", "-std=c++17", "-fPIC", "-Wall", "-I", os.path.join(ROOT, "include"), "-I",
+           os.path.join(CPP, "include"), "-c", path, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return obj
+
+
+def _link(target: str, inputs, extra, force: bool):
+    if not force and not _stale(target, inputs):
+        return
+    cmd = [HIPCC, "-o", target + ".tmp"] + extra + list(inputs)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed for {target}:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(target + ".tmp", target)
+
+
 def build(force: bool = False, verbose: bool = False) -> str:
+    """Builds libprism_amd.so (kernels + C ABI), libprism_cpp.so (C++ API)
+    and the `prism` executable into prism_amd/_build; returns the first."""
     os.makedirs(OUT, exist_ok=True)
-    jobs = min(len(SOURCES), int(os.environ.get("MAX_JOBS", "8")))
+    jobs = min(len(SOURCES) + len(CPP_SOURCES), int(os.environ.get("MAX_JOBS", "8")))
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(lambda s: _compile(s, force), SOURCES))
-    if force or _stale(LIB, objs):
-        cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        if r.returncode != 0:
-            raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
-        os.replace(LIB + ".tmp", LIB)
+        cpp_objs = list(ex.map(lambda s: _compile_cpp(s, force), CPP_SOURCES + ["main.cpp"]))
+    _link(LIB, objs, [f"--offload-arch={ARCH}", "-shared", "-fPIC"], force)
+    rpath = "-Wl,-rpath,$ORIGIN"
+    _link(CPP_LIB, cpp_objs[:-1] + [LIB], ["-shared", "-fPIC", rpath], force)
+    _link(CLI, cpp_objs[-1:] + [CPP_LIB, LIB], [rpath], force)
     if verbose:
         print(LIB)
+        print(CPP_LIB)
+        print(CLI)
     return LIB
 
 

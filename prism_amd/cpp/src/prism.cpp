@@ -1,0 +1,135 @@
+// prism.cpp -- the PRISM driver: compress a raw field file into a stream
+// file, decompress it, or retrieve it progressively at a list of bounds
+// (the flows of the reference's src/prism.cu:9-190).
+//
+// One stream and one Compressor serve all rounds of a progressive run: the
+// compressed file is read once and the decoded bit-planes, codes and
+// previous reconstruction stay resident in HBM between rounds (the
+// reference reloads the file and rebuilds its Compressor each round,
+// prism.cu:131-135, with the same results).
+#include "prism/prism.hpp"
+
+#include <cstdio>
+#include <memory>
+
+#include "prism/analyze.hpp"
+#include "prism/compressor.hpp"
+
+namespace {
+
+using prism::check_hip;
+
+template <typename T>
+std::unique_ptr<prism::StatBuffer<T>> field(const prism_context* c) {
+    return std::make_unique<prism::StatBuffer<T>>(c->dtype, 0, c->x, c->y, c->z);
+}
+
+template <typename T>
+void prism_compress(prism_context* c, void* stream) {
+    auto input = field<T>(c);
+    input->template load_fromfile<ori_File>(c->oriFilePath);
+    input->H2D();
+    if (c->error_mode == REL) c->eb = c->rel_eb * input->findMinMax();
+    if (c->report_time) {  // warm-up pass
+        prism::Compressor<T, i4> warm;
+        warm.init(c);
+        warm.compress_pipeline(c, input.get(), stream);
+    }
+    prism::Compressor<T, i4> cmp;
+    cmp.init(c);
+    cmp.compress_pipeline(c, input.get(), stream);
+    cmp.compressed_data->unload_tofile(c->cmpFilePath, (long long)cmp.total_compressed_size,
+                                       prism::typetofile::deviceTofile);
+    if (c->report_time)
+        prism::print_result<T, 0, 0>(c->size, cmp.total_compressed_size, 0.0, cmp.time_pred, cmp.time_bitplane,
+                                     cmp.time_encode);
+}
+
+template <typename T>
+void load_stream(prism::Compressor<T, i4>& d, const prism_context* c) {
+    d.compressed_data->template load_fromfile<cmp_File>(c->cmpFilePath);
+    d.compressed_data->H2D();
+}
+
+template <typename T>
+void report_quality(const prism_context* c, prism::StatBuffer<T>* out) {
+    auto ori = field<T>(c);
+    ori->template load_fromfile<ori_File>(c->oriFilePath);
+    prism::statistic<T>(static_cast<const T*>(ori->h), static_cast<const T*>(out->h), c->size);
+}
+
+template <typename T>
+void prism_decompress(prism_context* c, void* stream) {
+    if (c->report_time) {  // warm-up pass
+        auto tmp = field<T>(c);
+        prism::Compressor<T, i4> warm;
+        warm.init(c);
+        load_stream(warm, c);
+        warm.decompress_pipeline(c, tmp.get(), stream);
+    }
+    prism::Compressor<T, i4> d;
+    d.init(c);
+    load_stream(d, c);
+    auto out = field<T>(c);
+    d.decompress_pipeline(c, out.get(), stream);
+    out->D2H();
+    out->unload_tofile(c->decFilePath, prism::typetofile::hostTofile);
+    if (c->report_time)
+        prism::print_result<T, 1, 0>(c->size, d.total_compressed_size, d.itime_enum, d.itime_pred, d.itime_bitplane,
+                                     d.itime_decode);
+    if (c->report_cr) report_quality<T>(c, out.get());
+}
+
+template <typename T>
+void prism_progressive_decompress(prism_context* c, void* stream) {
+    if (c->report_time) {  // warm-up round, then the retrieval state is reset
+        auto tmp = field<T>(c);
+        prism::Compressor<T, i4> warm;
+        warm.init(c);
+        load_stream(warm, c);
+        warm.decompress_progressive_pipeline(c, nullptr, tmp.get(), c->target_ebs[0], 0, stream);
+        check_hip(hipMemset(c->begin, 0, 4 * sizeof(int)), "hipMemset");
+        check_hip(hipMemset(c->end, 0, 4 * sizeof(int)), "hipMemset");
+    }
+    prism::Compressor<T, i4> d;
+    d.init(c);
+    load_stream(d, c);
+    auto cur = field<T>(c), prev = field<T>(c);
+    size_t loaded = d.ap->bytes;
+    const size_t rounds = c->target_ebs.size();
+    for (size_t i = 0; i < rounds; ++i) {
+        std::swap(cur, prev);  // prev holds the last reconstruction
+        d.decompress_progressive_pipeline(c, i ? prev.get() : nullptr, cur.get(), c->target_ebs[i],
+                                          i ? c->target_ebs[i - 1] : 0.0, stream);
+        cur->D2H();
+        if (c->report_time)
+            prism::print_result<T, 1, 1>(c->size, d.total_compressed_size, d.itime_enum, d.itime_pred,
+                                         d.itime_bitplane, d.itime_decode);
+        if (c->report_cr) {
+            report_quality<T>(c, cur.get());
+            prism::calculate_accumulate_size<T>(loaded, c->size, c->begin, c->end, d.compressedSize_bp_d,
+                                                d.bytes_read);
+        }
+    }
+    cur->unload_tofile(c->decFilePath, prism::typetofile::hostTofile);
+}
+
+template <typename T>
+void run(prism_context* c, void* stream) {
+    if (c->isComp) prism_compress<T>(c, stream);
+    if (c->isDecomp) {
+        if (c->compMode && !c->target_ebs.empty()) prism_progressive_decompress<T>(c, stream);
+        else prism_decompress<T>(c, stream);
+    }
+}
+
+}  // namespace
+
+void initialize(prism_context* config) {
+    hipStream_t stream;
+    check_hip(hipStreamCreate(&stream), "hipStreamCreate");
+    if (config->dtype == PRISM_TYPE_FLOAT) run<f4>(config, stream);
+    else if (config->dtype == PRISM_TYPE_DOUBLE) run<f8>(config, stream);
+    check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    check_hip(hipStreamDestroy(stream), "hipStreamDestroy");
+}
