@@ -56,6 +56,8 @@ extern "C" int pa_layout_init(pa_layout* L, uint32_t nx, uint32_t ny, uint32_t n
     L->dev_len[0] = (count[0] + n4 + 7) / 8 * 8;
     L->dev_align = (L->prefix[0] - L->cnt[0]) + (L->dev_len[0] - count[0] - n4);
     L->code_len = n4 + n + L->dev_align;
+    // the kernels index the code array with 32-bit arithmetic
+    if (L->code_len >= ((int64_t)1 << 31)) return PA_ERR_ARG;
 
     for (int lv = 0; lv < 4; ++lv) L->dev_stride[lv] = device_plane_bytes(L->dev_len[lv]);
     int64_t base = 0;

@@ -19,6 +19,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kTileVals = kTS * kTS * kTS;
+constexpr int kXcds = 8;  // MI355X: 8 XCDs, 32 CUs each
 
 enum Kind { BLUE, YELLOW, HOLLOW, HOLLOW_R, YELLOW_R, BLUE_R, LINE, FACE, CUBE };
 enum Mode { COMPRESS, DECOMPRESS, TUNE };
@@ -599,49 +600,128 @@ __device__ __forceinline__ void level(Ctx<T>& t, bool md, bool rev) {
 
 // Decompression start: the slot of every tile point (17^3, halo included)
 // gets its code, its anchor or 0.  Level L's points are the multiples of
-// 2^L in the tile that are not all multiples of 2^(L+1); lanes run along x
-// so each load instruction reads consecutive codes.  All loads are issued
-// before the first LDS write.
+// U = 2^L in the tile that are not all multiples of 2U.  Along a row (y, z)
+// of level L the level-major index is affine in x (the compaction formula,
+// InterpolationPredictor.cu:2070-2083): with i = x / U,
+//   rows with y/U or z/U odd hold every i:  idx = base_L(y, z) + ox/U + i,
+//   other rows hold the odd i only:          idx = base_L(y, z) + ox/(2U) + i/2.
+// The 404 row bases of a tile (17^2 + 9^2 + 5^2 + 3^2) are computed once
+// into LDS; then every code load is one table read and an add, and all
+// loads are issued before the first slot is written.
+constexpr int kRows0 = 289, kRows1 = 81, kRows2 = 25, kRows3 = 9;
+constexpr int kRowTab = kRows0 + kRows1 + kRows2 + kRows3;  // 404
+template <int L>
+constexpr int row_off() {
+    return L == 0 ? 0 : (L == 1 ? kRows0 : (L == 2 ? kRows0 + kRows1 : kRows0 + kRows1 + kRows2));
+}
+
+template <int L>
+__device__ __forceinline__ int32_t row_base(const Geom& g, int gy, int gz) {
+    const int64_t b = gy >> L, c = gz >> L;
+    int64_t r = b * g.lx[L] + c * g.lxy[L] + g.prefix[L] - ((c + 1) >> 1) * g.lxy[L + 1];
+    if ((c & 1) == 0) r -= ((b + 1) >> 1) * g.lx[L + 1];
+    return (int32_t)r;  // < code_len < 2^31
+}
+
+template <typename T, int L>
+__device__ __forceinline__ void row_entry(const Ctx<T>& t, int32_t* tab, int row) {
+    constexpr int U = 1 << L, E = 16 / U + 1;
+    const int gy = t.oy + U * (row % E), gz = t.oz + U * (row / E);
+    tab[row_off<L>() + row] = (gy < t.g.ny && gz < t.g.nz) ? row_base<L>(t.g, gy, gz) : 0;
+}
+
+template <typename T>
+__device__ __forceinline__ void fill_rows(const Ctx<T>& t, int32_t* tab) {
+    const int lane = lane_id();
+    row_entry<T, 0>(t, tab, lane);
+    if (lane + kThreads < kRows0) row_entry<T, 0>(t, tab, lane + kThreads);
+    if (lane < kRows1) row_entry<T, 1>(t, tab, lane);
+    else if (lane < kRows1 + kRows2) row_entry<T, 2>(t, tab, lane - kRows1);
+    else if (lane < kRows1 + kRows2 + kRows3) row_entry<T, 3>(t, tab, lane - kRows1 - kRows2);
+}
+
+// Level 0: lane l keeps x = l mod 16 and takes rows r = l / 16 + 16 it; the
+// x = 16 column is a second, short pass.  Levels 1-3: lane l takes lattice
+// point p = l + 256 it of the level's E^3 lattice (x fastest).
+constexpr int kFill0Iters = (kRows0 + 15) / 16;                   // 19
+constexpr int kFill0EdgeIters = (kRows0 + kThreads - 1) / kThreads;  // 2
 template <int L>
 struct FillShape {
     static constexpr int U = 1 << L, E = 16 / U + 1, TOTAL = E * E * E;
     static constexpr int ITERS = (TOTAL + kThreads - 1) / kThreads;
 };
 
-template <int L>
-__device__ __forceinline__ int fill_slot(int lane, int it) {  // LDS slot of the lane's it-th point, or -1
-    using F = FillShape<L>;
-    const int p = lane + it * kThreads;
-    const int i = p % F::E, j = (p / F::E) % F::E, k = p / (F::E * F::E);
-    const bool mine = (F::TOTAL % kThreads == 0 || p < F::TOTAL) && ((i | j | k) & 1);
-    return mine ? sidx(i * F::U, j * F::U, k * F::U) : -1;
+template <typename T>
+__device__ __forceinline__ void fill0_load(const Ctx<T>& t, const int32_t* tab, bool load,
+                                           int32_t (&v)[kFill0Iters], int32_t (&e)[kFill0EdgeIters]) {
+    const int lane = lane_id();
+    const int x = lane & 15;
+    const bool xin = t.ox + x < t.g.nx;
+#pragma unroll
+    for (int it = 0; it < kFill0Iters; ++it) {
+        const int r = (lane >> 4) + 16 * it;
+        const int y = r % kTS, z = r / kTS;
+        const bool full = ((y | z) & 1) != 0;
+        v[it] = 0;
+        if (load && r < kRows0 && xin && (full || (x & 1)) && t.oy + y < t.g.ny && t.oz + z < t.g.nz)
+            v[it] = t.qc_in[(uint32_t)(tab[r] + (full ? t.ox + x : (t.ox >> 1) + (x >> 1)))];
+    }
+#pragma unroll
+    for (int it = 0; it < kFill0EdgeIters; ++it) {  // x = 16: level 0 where y or z is odd
+        const int r = lane + kThreads * it;
+        const int y = r % kTS, z = r / kTS;
+        e[it] = 0;
+        if (load && r < kRows0 && ((y | z) & 1) && t.ox + 16 < t.g.nx && t.oy + y < t.g.ny && t.oz + z < t.g.nz)
+            e[it] = t.qc_in[(uint32_t)(tab[r] + t.ox + 16)];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void fill0_store(T* s, const int32_t (&v)[kFill0Iters],
+                                            const int32_t (&e)[kFill0EdgeIters]) {
+    const int lane = lane_id();
+    const int x = lane & 15;
+#pragma unroll
+    for (int it = 0; it < kFill0Iters; ++it) {
+        const int r = (lane >> 4) + 16 * it;
+        const int y = r % kTS, z = r / kTS;
+        if (r < kRows0 && (((y | z) & 1) || (x & 1))) s[r * kTS + x] = code_slot(v[it], T());
+    }
+#pragma unroll
+    for (int it = 0; it < kFill0EdgeIters; ++it) {
+        const int r = lane + kThreads * it;
+        const int y = r % kTS, z = r / kTS;
+        if (r < kRows0 && ((y | z) & 1)) s[r * kTS + 16] = code_slot(e[it], T());
+    }
 }
 
 template <typename T, int L>
-__device__ __forceinline__ void fill_load(const Ctx<T>& t, bool load, int32_t (&v)[FillShape<L>::ITERS]) {
+__device__ __forceinline__ void fill_load(const Ctx<T>& t, const int32_t* tab, bool load,
+                                          int32_t (&v)[FillShape<L>::ITERS]) {
     using F = FillShape<L>;
     const int lane = lane_id();
 #pragma unroll
     for (int it = 0; it < F::ITERS; ++it) {
         const int p = lane + it * kThreads;
-        const int i = p % F::E, j = (p / F::E) % F::E, k = p / (F::E * F::E);
-        const bool mine = (F::TOTAL % kThreads == 0 || p < F::TOTAL) && ((i | j | k) & 1);
-        const int gx = t.ox + i * F::U, gy = t.oy + j * F::U, gz = t.oz + k * F::U;
+        const int i = p % F::E, row = p / F::E, j = row % F::E, k = row / F::E;
+        const bool full = ((j | k) & 1) != 0;
+        const bool mine = (F::TOTAL % kThreads == 0 || p < F::TOTAL) && (full || (i & 1));
         v[it] = 0;
-        if (load && mine && gx < t.g.nx && gy < t.g.ny && gz < t.g.nz) v[it] = t.qc_in[lm_index<L>(t.g, gx, gy, gz)];
-        // keep each iteration's address arithmetic next to its load: only
-        // the loaded codes stay live across the batch
-        __builtin_amdgcn_sched_barrier(0);
+        if (load && mine && t.ox + i * F::U < t.g.nx && t.oy + j * F::U < t.g.ny && t.oz + k * F::U < t.g.nz)
+            v[it] = t.qc_in[(uint32_t)(tab[row_off<L>() + row] + (full ? (t.ox >> L) + i : (t.ox >> (L + 1)) + (i >> 1)))];
     }
 }
 
 template <typename T, int L>
 __device__ __forceinline__ void fill_store(T* s, const int32_t (&v)[FillShape<L>::ITERS]) {
+    using F = FillShape<L>;
     const int lane = lane_id();
 #pragma unroll
-    for (int it = 0; it < FillShape<L>::ITERS; ++it) {
-        const int slot = fill_slot<L>(lane, it);
-        if (slot >= 0) s[slot] = code_slot(v[it], T());
+    for (int it = 0; it < F::ITERS; ++it) {
+        const int p = lane + it * kThreads;
+        const int i = p % F::E, row = p / F::E, j = row % F::E, k = row / F::E;
+        if ((F::TOTAL % kThreads == 0 || p < F::TOTAL) && ((i | j | k) & 1))
+            s[sidx(i * F::U, j * F::U, k * F::U)] = code_slot(v[it], T());
     }
 }
 
@@ -686,7 +766,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                                                           const T* __restrict__ anchors_in,
                                                           const T* __restrict__ old, T* __restrict__ out) {
     __shared__ T tile[kTileVals];
-    const int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+    // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs,
+    // so workgroup b runs on XCD b mod 8; each XCD takes a contiguous range
+    // of tiles (x fastest) and the cache lines that neighbouring tiles share
+    // (halo rows and columns, level-major code runs) meet in one L2.
+    const int64_t ntiles = (int64_t)g.gx * g.gy * g.gz;
+    const int64_t b = blockIdx.x, xcd = b % kXcds, per = ntiles / kXcds, extra = ntiles % kXcds;
+    const int64_t tix = xcd * per + (xcd < extra ? xcd : extra) + b / kXcds;
+    const int bx = (int)(tix % g.gx), by = (int)((tix / g.gx) % g.gy), bz = (int)(tix / ((int64_t)g.gx * g.gy));
     Ctx<T> t;
     t.s = tile;
     t.ox = bx * 16;
@@ -729,11 +816,15 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         if (threadIdx.x == 0)
             anchors[bx + (int64_t)by * g.gx + (int64_t)bz * g.gx * g.gy] = tile[0];
     } else {
-        int32_t v3[FillShape<3>::ITERS], v2[FillShape<2>::ITERS], v1[FillShape<1>::ITERS], v0[FillShape<0>::ITERS];
-        fill_load<T, 3>(t, 3 <= maxlevel, v3);
-        fill_load<T, 2>(t, 2 <= maxlevel, v2);
-        fill_load<T, 1>(t, 1 <= maxlevel, v1);
-        fill_load<T, 0>(t, 0 <= maxlevel, v0);
+        __shared__ int32_t rowtab[kRowTab];
+        fill_rows(t, rowtab);
+        lds_barrier();
+        int32_t v3[FillShape<3>::ITERS], v2[FillShape<2>::ITERS], v1[FillShape<1>::ITERS], v0[kFill0Iters],
+            e0[kFill0EdgeIters];
+        fill_load<T, 3>(t, rowtab, 3 <= maxlevel, v3);
+        fill_load<T, 2>(t, rowtab, 2 <= maxlevel, v2);
+        fill_load<T, 1>(t, rowtab, 1 <= maxlevel, v1);
+        fill0_load<T>(t, rowtab, 0 <= maxlevel, v0, e0);
         T anc = (T)0;
         int aslot = -1;
         if (threadIdx.x < 8) {  // x_reset_scratch_data 1051-1061: the eight corner anchors
@@ -746,7 +837,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         fill_store<T, 3>(tile, v3);
         fill_store<T, 2>(tile, v2);
         fill_store<T, 1>(tile, v1);
-        fill_store<T, 0>(tile, v0);
+        fill0_store<T>(tile, v0, e0);
         if (aslot >= 0) tile[aslot] = anc;
         __syncthreads();
     }
@@ -912,8 +1003,9 @@ int launch_spline(const pa_layout* L, const T* data, double eb, double rel_eb, c
                   const int32_t* d_maxlevel, int32_t* qc_out, const int32_t* qc_in, T* anchors,
                   const T* anchors_in, const T* old, T* out, hipStream_t st) {
     Geom g = make_geom(L);
-    if (g.gy > 65535 || g.gz > 65535) return PA_ERR_ARG;
-    dim3 grid(g.gx, g.gy, g.gz);
+    const int64_t ntiles = (int64_t)g.gx * g.gy * g.gz;
+    if (ntiles > 0x7fffffff) return PA_ERR_ARG;
+    dim3 grid((unsigned)ntiles);
     LevelEb<T> le = level_ebs<T>(eb, rel_eb);
     hipLaunchKernelGGL((spline_kernel<T, M>), grid, dim3(kThreads), 0, st, data, g, le, flags8, d_maxlevel,
                        qc_out, qc_in, anchors, anchors_in, old, out);
