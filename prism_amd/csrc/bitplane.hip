@@ -86,9 +86,18 @@ __global__ __launch_bounds__(kThreads) void shuffle_kernel(const int32_t* __rest
     const int64_t c0 = 8 * g0;  // first code of the block within the level
     const int64_t real = v.real_cnt[lv];
     const int32_t* src = qc + v.prefix[lv];
-    for (int i = threadIdx.x; i < kCodesPerBlock; i += kThreads) {
-        const int64_t r = c0 + i;
-        stage[(i >> 5) * kRow + (i & 31)] = r < real ? src[r] : 0;
+    // all 32 loads of a lane are in flight before the first LDS write
+    constexpr int kPer = kCodesPerBlock / kThreads;
+    int32_t c[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int64_t r = c0 + threadIdx.x + k * kThreads;
+        c[k] = r < real ? src[r] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int i = threadIdx.x + k * kThreads;
+        stage[(i >> 5) * kRow + (i & 31)] = c[k];
     }
     __syncthreads();
     const int64_t g = g0 + 4 * threadIdx.x;

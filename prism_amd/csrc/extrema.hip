@@ -34,7 +34,25 @@ __global__ __launch_bounds__(256) void extrema_kernel(const T* __restrict__ in, 
     const int64_t nv = n / V;
     const Vec* vin = reinterpret_cast<const Vec*>(in);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv; i += stride) {
+    // kBatch independent 16-byte loads per lane are in flight before any is
+    // reduced: one load per wave at a time would cap the pass at a few TB/s
+    constexpr int kBatch = 8;
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    for (; i + (kBatch - 1) * stride < nv; i += kBatch * stride) {
+        Vec v[kBatch];
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) v[k] = vin[i + k * stride];
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) {
+            const T* e = reinterpret_cast<const T*>(&v[k]);
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                mn = fmin(mn, (double)e[j]);
+                mx = fmax(mx, (double)e[j]);
+            }
+        }
+    }
+    for (; i < nv; i += stride) {
         const Vec v = vin[i];
         const T* e = reinterpret_cast<const T*>(&v);
 #pragma unroll

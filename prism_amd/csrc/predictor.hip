@@ -26,7 +26,10 @@ enum Mode { COMPRESS, DECOMPRESS, TUNE };
 
 __device__ __forceinline__ int sidx(int x, int y, int z) { return (z * kTS + y) * kTS + x; }
 
-template <typename T>
+// Per-tile context.  IN = true for a tile that is not the last one along
+// any axis: all 17^3 points lie inside the field and every boundary test of
+// the stencils is decided at compile time.
+template <typename T, bool IN = false>
 struct Ctx {
     T* s;             // LDS tile
     int ox, oy, oz;   // tile origin
@@ -37,6 +40,16 @@ struct Ctx {
     const int32_t* qc_in;
     bool natural;     // tuner candidates 9..11
     double acc;       // tuner partial sum of this lane
+    // the tile is not the last one along axis A (0 = x, 1 = y, 2 = z)
+    template <int A>
+    __device__ __forceinline__ bool c1() const {
+        return IN || (A == 0 ? c1x : (A == 1 ? c1y : c1z));
+    }
+    // global coordinate v along axis A lies inside the field
+    template <int A>
+    __device__ __forceinline__ bool inf(int v) const {
+        return IN || v < (A == 0 ? g.nx : (A == 1 ? g.ny : g.nz));
+    }
 };
 
 template <typename T>
@@ -66,12 +79,12 @@ __device__ __forceinline__ int interp_order(int c, int g, int N, bool c1) {
 }
 
 // 1-D rule along axis A (0 = x, 1 = y, 2 = z), InterpolationPredictor.cu:1269-1326
-template <typename T, int U, int A>
-__device__ __forceinline__ T pred_1d(const Ctx<T>& t, int x, int y, int z) {
+template <typename T, int U, int A, bool IN>
+__device__ __forceinline__ T pred_1d(const Ctx<T, IN>& t, int x, int y, int z) {
     const int c = A == 0 ? x : (A == 1 ? y : z);
     const int o = A == 0 ? t.ox : (A == 1 ? t.oy : t.oz);
     const int N = A == 0 ? t.g.nx : (A == 1 ? t.g.ny : t.g.nz);
-    const bool c1 = A == 0 ? t.c1x : (A == 1 ? t.c1y : t.c1z);
+    const bool c1 = t.template c1<A>();
     const int gc = o + c;
     const int i0 = c - 3 * U >= 0 ? c - 3 * U : 0;
     const int i1 = c - U >= 0 ? c - U : 0;
@@ -103,8 +116,8 @@ __device__ __forceinline__ void taps(int c, int (&id)[4]) {
 }
 
 // FACE rule, InterpolationPredictor.cu:1546-1669
-template <typename T, int U>
-__device__ __forceinline__ T pred_face(const Ctx<T>& t, int x, int y, int z) {
+template <typename T, int U, bool IN>
+__device__ __forceinline__ T pred_face(const Ctx<T, IN>& t, int x, int y, int z) {
     int idz[4], idy[4], idx[4];
     taps<U>(z, idz);
     taps<U>(y, idy);
@@ -117,24 +130,24 @@ __device__ __forceinline__ T pred_face(const Ctx<T>& t, int x, int y, int z) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) s2[i] = t.s[sidx(x, idy[i], z)];
         pred = t.s[sidx(x, idy[1], idz[1])];
-        i1 = interp_order<U>(z, t.oz + z, t.g.nz, t.c1z);
-        i2 = interp_order<U>(y, t.oy + y, t.g.ny, t.c1y);
+        i1 = interp_order<U>(z, t.oz + z, t.g.nz, t.template c1<2>());
+        i2 = interp_order<U>(y, t.oy + y, t.g.ny, t.template c1<1>());
     } else if ((y % (2 * U)) == 0) {  // directions z (1) and x (2)
 #pragma unroll
         for (int i = 0; i < 4; ++i) s1[i] = t.s[sidx(x, y, idz[i])];
 #pragma unroll
         for (int i = 0; i < 4; ++i) s2[i] = t.s[sidx(idx[i], y, z)];
         pred = t.s[sidx(idx[1], y, idz[1])];
-        i1 = interp_order<U>(z, t.oz + z, t.g.nz, t.c1z);
-        i2 = interp_order<U>(x, t.ox + x, t.g.nx, t.c1x);
+        i1 = interp_order<U>(z, t.oz + z, t.g.nz, t.template c1<2>());
+        i2 = interp_order<U>(x, t.ox + x, t.g.nx, t.template c1<0>());
     } else {  // directions y (1) and x (2)
 #pragma unroll
         for (int i = 0; i < 4; ++i) s1[i] = t.s[sidx(x, idy[i], z)];
 #pragma unroll
         for (int i = 0; i < 4; ++i) s2[i] = t.s[sidx(idx[i], y, z)];
         pred = t.s[sidx(idx[1], idy[1], z)];
-        i1 = interp_order<U>(y, t.oy + y, t.g.ny, t.c1y);
-        i2 = interp_order<U>(x, t.ox + x, t.g.nx, t.c1x);
+        i1 = interp_order<U>(y, t.oy + y, t.g.ny, t.template c1<1>());
+        i2 = interp_order<U>(x, t.ox + x, t.g.nx, t.template c1<0>());
     }
     const bool nat = t.natural;
     if (i1 == 4 && i2 == 4) {
@@ -182,15 +195,15 @@ __device__ __forceinline__ T pred_face(const Ctx<T>& t, int x, int y, int z) {
 }
 
 // CUBE rule, InterpolationPredictor.cu:1671-1723
-template <typename T, int U>
-__device__ __forceinline__ T pred_cube(const Ctx<T>& t, int x, int y, int z) {
+template <typename T, int U, bool IN>
+__device__ __forceinline__ T pred_cube(const Ctx<T, IN>& t, int x, int y, int z) {
     int idz[4], idy[4], idx[4];
     taps<U>(z, idz);
     taps<U>(y, idy);
     taps<U>(x, idx);
-    const int iz = interp_order<U>(z, t.oz + z, t.g.nz, t.c1z);
-    const int iy = interp_order<U>(y, t.oy + y, t.g.ny, t.c1y);
-    const int ix = interp_order<U>(x, t.ox + x, t.g.nx, t.c1x);
+    const int iz = interp_order<U>(z, t.oz + z, t.g.nz, t.template c1<2>());
+    const int iy = interp_order<U>(y, t.oy + y, t.g.ny, t.template c1<1>());
+    const int ix = interp_order<U>(x, t.ox + x, t.g.nx, t.template c1<0>());
     T tx[4], ty[4], tz[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) tx[i] = t.s[sidx(idx[i], y, z)];
@@ -282,15 +295,15 @@ __device__ __forceinline__ void stage_point(int p, int& x, int& y, int& z) {
 // Whether point (x, y, z) of the tile is predicted in this sweep: inside the
 // tile (the +1 halo included for inclusive stages; the last stage of the
 // tile row keeps its far plane) and inside the field.
-template <typename T>
-__device__ __forceinline__ bool visited(const Ctx<T>& t, int x, int y, int z, bool inclusive) {
+template <typename T, bool IN>
+__device__ __forceinline__ bool visited(const Ctx<T, IN>& t, int x, int y, int z, bool inclusive) {
     const bool in = inclusive ? (x <= 16 && y <= 16 && z <= 16)
-                              : (x < 16 && y < 16 && z < 16 + (t.c1z ? 0 : 1));
-    return in && t.ox + x < t.g.nx && t.oy + y < t.g.ny && t.oz + z < t.g.nz;
+                              : (x < 16 && y < 16 && z < 16 + (t.template c1<2>() ? 0 : 1));
+    return in && t.template inf<0>(t.ox + x) && t.template inf<1>(t.oy + y) && t.template inf<2>(t.oz + z);
 }
 
-template <typename T, int U, int K>
-__device__ __forceinline__ T predict(const Ctx<T>& t, int x, int y, int z) {
+template <typename T, int U, int K, bool IN>
+__device__ __forceinline__ T predict(const Ctx<T, IN>& t, int x, int y, int z) {
     if constexpr (K == BLUE || K == BLUE_R) {
         return pred_1d<T, U, 2>(t, x, y, z);
     } else if constexpr (K == YELLOW || K == YELLOW_R) {
@@ -331,8 +344,8 @@ __device__ __forceinline__ void lds_barrier() {
 
 // Quantise v against pred, write the reconstruction back and return the
 // code (InterpolationPredictor.cu:1327-1341).
-template <typename T>
-__device__ __forceinline__ int32_t quantize(const Ctx<T>& t, T& v, T pred) {
+template <typename T, bool IN>
+__device__ __forceinline__ int32_t quantize(const Ctx<T, IN>& t, T& v, T pred) {
     const T err = v - pred;
     T code = (err < (T)0 ? -err : err) * t.eb_r + (T)1;
     code = err < (T)0 ? -code : code;
@@ -357,8 +370,8 @@ __device__ __forceinline__ int32_t slot_code(double v) { return (int32_t)(uint32
 // ----------------------------------------------------------- point form --
 // Lane l visits points p = l + 256 * it of the stage (tuner, and the
 // multi-dimensional stages).
-template <typename T, int U, int K, int M>
-__device__ __forceinline__ void stage(Ctx<T>& t, bool inclusive) {
+template <typename T, int U, int K, int M, bool IN>
+__device__ __forceinline__ void stage(Ctx<T, IN>& t, bool inclusive) {
     constexpr int TOTAL = stage_total<U, K>();
     constexpr int LEVEL = level_of<U>();
     const int lane = lane_id();
@@ -449,8 +462,8 @@ struct LineLane {
     int64_t st;    // level-major stride between consecutive points
 };
 
-template <typename T, int U, int K, bool EXCL>
-__device__ __forceinline__ LineLane line_lane(const Ctx<T>& t, int it) {
+template <typename T, int U, int K, bool EXCL, bool IN>
+__device__ __forceinline__ LineLane line_lane(const Ctx<T, IN>& t, int it) {
     using S = LineShape<U, K, EXCL>;
     LineLane r;
     const int l = lane_id() + it * kThreads;
@@ -462,9 +475,9 @@ __device__ __forceinline__ LineLane line_lane(const Ctx<T>& t, int it) {
     else { x = pc; y = qc; z = U; }
     const int gx = t.ox + x, gy = t.oy + y, gz = t.oz + z;
     bool on = (S::LINES % kThreads == 0 || l < S::LINES);
-    if constexpr (S::A != 0) on = on && gx < t.g.nx;
-    if constexpr (S::A != 1) on = on && gy < t.g.ny;
-    if constexpr (S::A != 2) on = on && gz < t.g.nz;
+    if constexpr (S::A != 0) on = on && t.template inf<0>(gx);
+    if constexpr (S::A != 1) on = on && t.template inf<1>(gy);
+    if constexpr (S::A != 2) on = on && t.template inf<2>(gz);
     r.on = on;
     r.own = x < 16 && y < 16 && z < 16;
     r.base = sidx(x, y, z) - U * S::STA;
@@ -482,13 +495,13 @@ __device__ __forceinline__ LineLane line_lane(const Ctx<T>& t, int it) {
     return r;
 }
 
-template <typename T, int U, int K, bool EXCL, int M>
-__device__ __forceinline__ void line_stage(Ctx<T>& t) {
+template <typename T, int U, int K, bool EXCL, int M, bool IN>
+__device__ __forceinline__ void line_stage(Ctx<T, IN>& t) {
     using S = LineShape<U, K, EXCL>;
     constexpr int N = S::N;
     const int o = S::A == 0 ? t.ox : (S::A == 1 ? t.oy : t.oz);
     const int nax = S::A == 0 ? t.g.nx : (S::A == 1 ? t.g.ny : t.g.nz);
-    const bool c1 = S::A == 0 ? t.c1x : (S::A == 1 ? t.c1y : t.c1z);
+    const bool c1 = t.template c1<S::A>();
 #pragma unroll
     for (int it = 0; it < S::ITERS; ++it) {
         const LineLane ln = line_lane<T, U, K, EXCL>(t, it);
@@ -498,7 +511,7 @@ __device__ __forceinline__ void line_stage(Ctx<T>& t) {
         for (int j = 0; j <= N; ++j) k[j] = t.s[ln.base + 2 * j * U * S::STA];
 #pragma unroll
         for (int j = 0; j < N; ++j) {
-            if (o + (2 * j + 1) * U >= nax) break;  // beyond the field; so are later points
+            if (!t.template inf<S::A>(o + (2 * j + 1) * U)) break;  // beyond the field; so are later points
             const T pred = apply_stencil(stencil_of<U>(j, N, c1, o, nax), k[j > 0 ? j - 1 : 0], k[j], k[j + 1],
                                          k[j + 2 <= N ? j + 2 : 0]);
             T& v = t.s[ln.base + (2 * j + 1) * U * S::STA];
@@ -527,30 +540,30 @@ struct ColumnLane {
     int64_t idx0, dz;
 };
 
-template <typename T>
-__device__ __forceinline__ ColumnLane column_lane(const Ctx<T>& t) {
+template <typename T, bool IN>
+__device__ __forceinline__ ColumnLane column_lane(const Ctx<T, IN>& t) {
     ColumnLane c;
     const int l = lane_id();
     c.x = 2 * (l & 7) + 1;
     c.y = (l >> 3) & 15;
     c.z0 = l >> 7;
-    c.on = t.ox + c.x < t.g.nx && t.oy + c.y < t.g.ny;
+    c.on = t.template inf<0>(t.ox + c.x) && t.template inf<1>(t.oy + c.y);
     c.idx0 = lm_index<0>(t.g, t.ox + c.x, t.oy + c.y, t.oz + c.z0);
     c.dz = 2 * t.g.lxy[0] - t.g.lxy[1];  // z -> z + 2 keeps the parity of z
     return c;
 }
 
-template <typename T, int M>
-__device__ __forceinline__ void hollow0_stage(Ctx<T>& t) {
+template <typename T, int M, bool IN>
+__device__ __forceinline__ void hollow0_stage(Ctx<T, IN>& t) {
     const ColumnLane c = column_lane(t);
     if (c.on) {
         const int j = c.x >> 1;
-        const int k = stencil_of<1>(j, 8, t.c1x, t.ox, t.g.nx);
+        const int k = stencil_of<1>(j, 8, t.template c1<0>(), t.ox, t.g.nx);
         const int i0 = j > 0 ? c.x - 3 : 0, i3 = c.x + 3 <= 16 ? c.x + 3 : 0;
 #pragma unroll
         for (int it = 0; it < kHollow0Iters; ++it) {
             const int z = c.z0 + 2 * it;
-            if (t.oz + z >= t.g.nz) break;
+            if (!t.template inf<2>(t.oz + z)) break;
             const int base = sidx(0, c.y, z);
             const T pred = apply_stencil(k, t.s[base + i0], t.s[base + c.x - 1], t.s[base + c.x + 1],
                                          t.s[base + i3]);
@@ -569,8 +582,8 @@ __device__ __forceinline__ void hollow0_stage(Ctx<T>& t) {
 // ---------------------------------------------------------------- levels --
 // The three stages of one level; outside tuning the last stage of level 0
 // uses the exclusive border (InterpolationPredictor.cu:1940-2025).
-template <typename T, int U, int M>
-__device__ __forceinline__ void level(Ctx<T>& t, bool md, bool rev) {
+template <typename T, int U, int M, bool IN>
+__device__ __forceinline__ void level(Ctx<T, IN>& t, bool md, bool rev) {
     constexpr bool EX = M != TUNE && U == 1;
     if (md) {
         stage<T, U, LINE, M>(t, true);
@@ -623,15 +636,15 @@ __device__ __forceinline__ int32_t row_base(const Geom& g, int gy, int gz) {
     return (int32_t)r;  // < code_len < 2^31
 }
 
-template <typename T, int L>
-__device__ __forceinline__ void row_entry(const Ctx<T>& t, int32_t* tab, int row) {
+template <typename T, int L, bool IN>
+__device__ __forceinline__ void row_entry(const Ctx<T, IN>& t, int32_t* tab, int row) {
     constexpr int U = 1 << L, E = 16 / U + 1;
     const int gy = t.oy + U * (row % E), gz = t.oz + U * (row / E);
-    tab[row_off<L>() + row] = (gy < t.g.ny && gz < t.g.nz) ? row_base<L>(t.g, gy, gz) : 0;
+    tab[row_off<L>() + row] = (t.template inf<1>(gy) && t.template inf<2>(gz)) ? row_base<L>(t.g, gy, gz) : 0;
 }
 
-template <typename T>
-__device__ __forceinline__ void fill_rows(const Ctx<T>& t, int32_t* tab) {
+template <typename T, bool IN>
+__device__ __forceinline__ void fill_rows(const Ctx<T, IN>& t, int32_t* tab) {
     const int lane = lane_id();
     row_entry<T, 0>(t, tab, lane);
     if (lane + kThreads < kRows0) row_entry<T, 0>(t, tab, lane + kThreads);
@@ -651,19 +664,19 @@ struct FillShape {
     static constexpr int ITERS = (TOTAL + kThreads - 1) / kThreads;
 };
 
-template <typename T>
-__device__ __forceinline__ void fill0_load(const Ctx<T>& t, const int32_t* tab, bool load,
+template <typename T, bool IN>
+__device__ __forceinline__ void fill0_load(const Ctx<T, IN>& t, const int32_t* tab, bool load,
                                            int32_t (&v)[kFill0Iters], int32_t (&e)[kFill0EdgeIters]) {
     const int lane = lane_id();
     const int x = lane & 15;
-    const bool xin = t.ox + x < t.g.nx;
+    const bool xin = t.template inf<0>(t.ox + x);
 #pragma unroll
     for (int it = 0; it < kFill0Iters; ++it) {
         const int r = (lane >> 4) + 16 * it;
         const int y = r % kTS, z = r / kTS;
         const bool full = ((y | z) & 1) != 0;
         v[it] = 0;
-        if (load && r < kRows0 && xin && (full || (x & 1)) && t.oy + y < t.g.ny && t.oz + z < t.g.nz)
+        if (load && r < kRows0 && xin && (full || (x & 1)) && t.template inf<1>(t.oy + y) && t.template inf<2>(t.oz + z))
             v[it] = t.qc_in[(uint32_t)(tab[r] + (full ? t.ox + x : (t.ox >> 1) + (x >> 1)))];
     }
 #pragma unroll
@@ -671,7 +684,8 @@ __device__ __forceinline__ void fill0_load(const Ctx<T>& t, const int32_t* tab, 
         const int r = lane + kThreads * it;
         const int y = r % kTS, z = r / kTS;
         e[it] = 0;
-        if (load && r < kRows0 && ((y | z) & 1) && t.ox + 16 < t.g.nx && t.oy + y < t.g.ny && t.oz + z < t.g.nz)
+        if (load && r < kRows0 && ((y | z) & 1) && t.template inf<0>(t.ox + 16) && t.template inf<1>(t.oy + y) &&
+            t.template inf<2>(t.oz + z))
             e[it] = t.qc_in[(uint32_t)(tab[r] + t.ox + 16)];
     }
 }
@@ -695,8 +709,8 @@ __device__ __forceinline__ void fill0_store(T* s, const int32_t (&v)[kFill0Iters
     }
 }
 
-template <typename T, int L>
-__device__ __forceinline__ void fill_load(const Ctx<T>& t, const int32_t* tab, bool load,
+template <typename T, int L, bool IN>
+__device__ __forceinline__ void fill_load(const Ctx<T, IN>& t, const int32_t* tab, bool load,
                                           int32_t (&v)[FillShape<L>::ITERS]) {
     using F = FillShape<L>;
     const int lane = lane_id();
@@ -707,7 +721,8 @@ __device__ __forceinline__ void fill_load(const Ctx<T>& t, const int32_t* tab, b
         const bool full = ((j | k) & 1) != 0;
         const bool mine = (F::TOTAL % kThreads == 0 || p < F::TOTAL) && (full || (i & 1));
         v[it] = 0;
-        if (load && mine && t.ox + i * F::U < t.g.nx && t.oy + j * F::U < t.g.ny && t.oz + k * F::U < t.g.nz)
+        if (load && mine && t.template inf<0>(t.ox + i * F::U) && t.template inf<1>(t.oy + j * F::U) &&
+            t.template inf<2>(t.oz + k * F::U))
             v[it] = t.qc_in[(uint32_t)(tab[row_off<L>() + row] + (full ? (t.ox >> L) + i : (t.ox >> (L + 1)) + (i >> 1)))];
     }
 }
@@ -732,8 +747,8 @@ struct LevelEb {
 
 // Tile plus halo into LDS: all global loads of a lane are issued before the
 // first LDS write so the tile costs one memory latency.
-template <typename T>
-__device__ __forceinline__ void load_tile(T* s, const T* data, const Geom& g, int ox, int oy, int oz) {
+template <typename T, bool IN>
+__device__ __forceinline__ void load_tile(const Ctx<T, IN>& t, const T* data) {
     constexpr int IT = (kTileVals + kThreads - 1) / kThreads;
     T v[IT];
     const int lane = lane_id();
@@ -741,40 +756,39 @@ __device__ __forceinline__ void load_tile(T* s, const T* data, const Geom& g, in
     for (int it = 0; it < IT; ++it) {
         const int i = lane + it * kThreads;
         const int x = i % kTS, y = (i / kTS) % kTS, z = i / (kTS * kTS);
-        const int gx = ox + x, gy = oy + y, gz = oz + z;
+        const int gx = t.ox + x, gy = t.oy + y, gz = t.oz + z;
         v[it] = (T)0;
-        if (i < kTileVals && gx < g.nx && gy < g.ny && gz < g.nz)
-            v[it] = data[(int64_t)gx + (int64_t)gy * g.nx + (int64_t)gz * g.nx * g.ny];
+        if (i < kTileVals && t.template inf<0>(gx) && t.template inf<1>(gy) && t.template inf<2>(gz))
+            v[it] = data[(int64_t)gx + (int64_t)gy * t.g.nx + (int64_t)gz * t.g.nx * t.g.ny];
     }
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const int i = lane + it * kThreads;
-        if (i < kTileVals) s[i] = v[it];
+        if (i < kTileVals) t.s[i] = v[it];
     }
 }
 
-// kernel `interpolation` (compress) and `interpolation_reverse` /
-// `interpolation_progressive_reverse` (decompress: anchors, or zero anchors
-// plus old for the delta form).
-template <typename T, int M>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void spline_kernel(const T* __restrict__ data, Geom g,
-                                                          LevelEb<T> le, const uint8_t* __restrict__ flags8,
-                                                          const int32_t* __restrict__ d_maxlevel,
-                                                          int32_t* __restrict__ qc_out,
-                                                          const int32_t* __restrict__ qc_in,
-                                                          T* __restrict__ anchors,
-                                                          const T* __restrict__ anchors_in,
-                                                          const T* __restrict__ old, T* __restrict__ out) {
-    __shared__ T tile[kTileVals];
-    // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs,
-    // so workgroup b runs on XCD b mod 8; each XCD takes a contiguous range
-    // of tiles (x fastest) and the cache lines that neighbouring tiles share
-    // (halo rows and columns, level-major code runs) meet in one L2.
-    const int64_t ntiles = (int64_t)g.gx * g.gy * g.gz;
-    const int64_t b = blockIdx.x, xcd = b % kXcds, per = ntiles / kXcds, extra = ntiles % kXcds;
-    const int64_t tix = xcd * per + (xcd < extra ? xcd : extra) + b / kXcds;
-    const int bx = (int)(tix % g.gx), by = (int)((tix / g.gx) % g.gy), bz = (int)(tix / ((int64_t)g.gx * g.gy));
-    Ctx<T> t;
+template <typename T>
+struct SplineArgs {
+    const T* data;
+    Geom g;
+    LevelEb<T> le;
+    const uint8_t* flags8;
+    const int32_t* d_maxlevel;
+    int32_t* qc_out;
+    const int32_t* qc_in;
+    T* anchors;
+    const T* anchors_in;
+    const T* old;
+    T* out;
+};
+
+// One tile: compress (load, predict, quantise, store codes) or decompress
+// (fill the code slots, reconstruct, add old, store).
+template <typename T, int M, bool IN>
+__device__ __forceinline__ void tile_run(const SplineArgs<T>& a, T* tile, int32_t* rowtab, int bx, int by, int bz) {
+    const Geom& g = a.g;
+    Ctx<T, IN> t;
     t.s = tile;
     t.ox = bx * 16;
     t.oy = by * 16;
@@ -783,40 +797,38 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     t.c1y = t.oy + 16 < g.ny;
     t.c1z = t.oz + 16 < g.nz;
     t.g = g;
-    t.qc_out = qc_out;
-    t.qc_in = qc_in;
+    t.qc_out = a.qc_out;
+    t.qc_in = a.qc_in;
     t.natural = false;
     t.acc = 0.0;
     uint8_t fl[8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) fl[i] = flags8[i];
-    const int maxlevel = d_maxlevel ? *d_maxlevel : 4;
+    for (int i = 0; i < 8; ++i) fl[i] = a.flags8[i];
+    const int maxlevel = a.d_maxlevel ? *a.d_maxlevel : 4;
 
     // decompression: the previous reconstruction (delta form) and every
     // code of the tile are requested before any is needed
     constexpr int OIT = 4096 / kThreads;
     T prev[OIT];
     if constexpr (M == DECOMPRESS) {
-        if (old) {
+        if (a.old) {
             const int lane = lane_id();
 #pragma unroll
             for (int it = 0; it < OIT; ++it) {
                 const int i = lane + it * kThreads;
                 const int gx = t.ox + (i & 15), gy = t.oy + ((i >> 4) & 15), gz = t.oz + (i >> 8);
                 prev[it] = (T)0;
-                if (gx < g.nx && gy < g.ny && gz < g.nz)
-                    prev[it] = old[(int64_t)gx + (int64_t)gy * g.nx + (int64_t)gz * g.nx * g.ny];
+                if (t.template inf<0>(gx) && t.template inf<1>(gy) && t.template inf<2>(gz))
+                    prev[it] = a.old[(int64_t)gx + (int64_t)gy * g.nx + (int64_t)gz * g.nx * g.ny];
             }
         }
     }
 
     if constexpr (M == COMPRESS) {
-        load_tile(tile, data, g, t.ox, t.oy, t.oz);
+        load_tile(t, a.data);
         __syncthreads();
-        if (threadIdx.x == 0)
-            anchors[bx + (int64_t)by * g.gx + (int64_t)bz * g.gx * g.gy] = tile[0];
+        if (threadIdx.x == 0) a.anchors[bx + (int64_t)by * g.gx + (int64_t)bz * g.gx * g.gy] = tile[0];
     } else {
-        __shared__ int32_t rowtab[kRowTab];
         fill_rows(t, rowtab);
         lds_barrier();
         int32_t v3[FillShape<3>::ITERS], v2[FillShape<2>::ITERS], v1[FillShape<1>::ITERS], v0[kFill0Iters],
@@ -831,8 +843,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             const int cx = threadIdx.x & 1, cy = (threadIdx.x >> 1) & 1, cz = threadIdx.x >> 2;
             const int ax = bx + cx, ay = by + cy, az = bz + cz;
             aslot = sidx(16 * cx, 16 * cy, 16 * cz);
-            if (anchors_in && ax < g.gx && ay < g.gy && az < g.gz)
-                anc = anchors_in[ax + (int64_t)ay * g.gx + (int64_t)az * g.gx * g.gy];
+            if (a.anchors_in && ax < g.gx && ay < g.gy && az < g.gz)
+                anc = a.anchors_in[ax + (int64_t)ay * g.gx + (int64_t)az * g.gx * g.gy];
         }
         fill_store<T, 3>(tile, v3);
         fill_store<T, 2>(tile, v2);
@@ -842,6 +854,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         __syncthreads();
     }
 
+    const LevelEb<T>& le = a.le;
     if (3 <= maxlevel) { t.eb_r = le.eb_r[3]; t.ebx2 = le.ebx2[3]; level<T, 8, M>(t, fl[3], fl[7]); }
     if (2 <= maxlevel) { t.eb_r = le.eb_r[2]; t.ebx2 = le.ebx2[2]; level<T, 4, M>(t, fl[2], fl[6]); }
     if (1 <= maxlevel) { t.eb_r = le.eb_r[1]; t.ebx2 = le.ebx2[1]; level<T, 2, M>(t, fl[1], fl[5]); }
@@ -854,12 +867,35 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             const int i = lane + it * kThreads;
             const int x = i & 15, y = (i >> 4) & 15, z = i >> 8;
             const int gx = t.ox + x, gy = t.oy + y, gz = t.oz + z;
-            if (gx < g.nx && gy < g.ny && gz < g.nz) {
+            if (t.template inf<0>(gx) && t.template inf<1>(gy) && t.template inf<2>(gz)) {
                 const T v = tile[sidx(x, y, z)];
-                out[(int64_t)gx + (int64_t)gy * g.nx + (int64_t)gz * g.nx * g.ny] = old ? v + prev[it] : v;
+                a.out[(int64_t)gx + (int64_t)gy * g.nx + (int64_t)gz * g.nx * g.ny] = a.old ? v + prev[it] : v;
             }
         }
     }
+}
+
+// kernel `interpolation` (compress) and `interpolation_reverse` /
+// `interpolation_progressive_reverse` (decompress: anchors, or zero anchors
+// plus old for the delta form).  Tiles away from the far faces of the field
+// (all 17^3 points inside) run a variant whose boundary tests are resolved
+// at compile time.
+template <typename T, int M>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void spline_kernel(SplineArgs<T> a) {
+    __shared__ T tile[kTileVals];
+    __shared__ int32_t rowtab[M == DECOMPRESS ? kRowTab : 1];
+    // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs,
+    // so workgroup b runs on XCD b mod 8; each XCD takes a contiguous range
+    // of tiles (x fastest) and the cache lines that neighbouring tiles share
+    // (halo rows and columns, level-major code runs) meet in one L2.
+    const Geom& g = a.g;
+    const int64_t ntiles = (int64_t)g.gx * g.gy * g.gz;
+    const int64_t b = blockIdx.x, xcd = b % kXcds, per = ntiles / kXcds, extra = ntiles % kXcds;
+    const int64_t tix = xcd * per + (xcd < extra ? xcd : extra) + b / kXcds;
+    const int bx = (int)(tix % g.gx), by = (int)((tix / g.gx) % g.gy), bz = (int)(tix / ((int64_t)g.gx * g.gy));
+    const bool interior = bx * 16 + 16 < g.nx && by * 16 + 16 < g.ny && bz * 16 + 16 < g.nz;
+    if (interior) tile_run<T, M, true>(a, tile, rowtab, bx, by, bz);
+    else tile_run<T, M, false>(a, tile, rowtab, bx, by, bz);
 }
 
 // Zero the code-array entries the predictor does not write: the anchor
@@ -897,7 +933,7 @@ __global__ __launch_bounds__(kThreads) void tune_kernel(const T* __restrict__ da
     t.qc_out = nullptr;
     t.qc_in = nullptr;
     t.acc = 0.0;
-    load_tile(tile, data, g, t.ox, t.oy, t.oz);
+    load_tile(t, data);
     __syncthreads();
     // candidate table of auto_tuning_interpolation (pre_compute_att 110-141, 898-962)
     t.natural = cand >= 9 && cand <= 11;
@@ -1007,8 +1043,8 @@ int launch_spline(const pa_layout* L, const T* data, double eb, double rel_eb, c
     if (ntiles > 0x7fffffff) return PA_ERR_ARG;
     dim3 grid((unsigned)ntiles);
     LevelEb<T> le = level_ebs<T>(eb, rel_eb);
-    hipLaunchKernelGGL((spline_kernel<T, M>), grid, dim3(kThreads), 0, st, data, g, le, flags8, d_maxlevel,
-                       qc_out, qc_in, anchors, anchors_in, old, out);
+    const SplineArgs<T> args{data, g, le, flags8, d_maxlevel, qc_out, qc_in, anchors, anchors_in, old, out};
+    hipLaunchKernelGGL((spline_kernel<T, M>), grid, dim3(kThreads), 0, st, args);
     PA_LAUNCH_CHECK();
     return PA_OK;
 }
