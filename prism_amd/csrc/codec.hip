@@ -1,6 +1,9 @@
 // codec.hip -- lossless bit-plane stream on gfx950: zero elimination plus
-// three rounds of repetition elimination per 16 KiB chunk, chunk offsets by
-// a single-pass decoupled look-back, the parallel decoder, and the trailer.
+// three rounds of repetition elimination per 16 KiB chunk (one chunk per
+// wave64, encoded into a staging slot), chunk offsets by a scan of the size
+// table, a gather into the stream, the parallel decoder, and the trailer.
+// No pass takes an atomic on a shared address: on MI355X those serialise
+// across workgroups at about 12 ns each.
 //
 // Stream (zbpc.cu:155-263, RZE.hpp:10-167): u64 insize | u16 csize[C] |
 // payloads in chunk order | zero pad to 8 | trailer.  Chunk order is level
@@ -93,52 +96,6 @@ __device__ __forceinline__ int block_scan(int v, int* lds, int* total) {
     return before + x - v;
 }
 
-constexpr uint64_t kEmpty = 0, kAgg = 1, kPrefix = 2;
-
-// Status words carry their own value, so relaxed agent-scope atomics are
-// enough: nothing else is published through them.
-__device__ __forceinline__ void publish(uint64_t* status, int64_t id, uint64_t flag, uint64_t value) {
-    __hip_atomic_store(status + id, (flag << 62) | value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Decoupled look-back by one wave64: exclusive payload offset of chunk id.
-// Chunk ids are handed out in arrival order, so every predecessor belongs to
-// a wave that is already running and publishes its size without waiting.
-__device__ int64_t look_back(uint64_t* status, int64_t id) {
-    const int lane = threadIdx.x & 63;
-    int64_t excl = 0, top = id - 1;
-    while (true) {
-        const int64_t idx = top - lane;
-        const uint64_t st = idx >= 0 ? __hip_atomic_load(status + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                     : (kPrefix << 62);
-        const uint64_t flag = st >> 62;
-        const uint64_t val = st & ((1ull << 62) - 1);
-        const uint64_t pmask = __ballot(flag == kPrefix);
-        const uint64_t emask = __ballot(flag == kEmpty);
-        if (pmask) {
-            const int p = __ffsll((unsigned long long)pmask) - 1;
-            const uint64_t need = p == 63 ? ~0ull : ((2ull << p) - 1);
-            if (emask & need) {
-                __builtin_amdgcn_s_sleep(1);
-                continue;
-            }
-            int64_t part = lane <= p ? (int64_t)val : 0;
-#pragma unroll
-            for (int d = 32; d >= 1; d >>= 1) part += __shfl_xor(part, d, 64);
-            return excl + part;
-        }
-        if (emask) {
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-        }
-        int64_t part = (int64_t)val;
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) part += __shfl_xor(part, d, 64);
-        excl += part;
-        top -= 64;
-    }
-}
-
 __device__ __forceinline__ int wave_incl_scan(int v) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
@@ -158,10 +115,9 @@ __device__ __forceinline__ uint32_t nz_bytes(uint32_t w) {
 // keep the ordering of LDS accesses of one wave across phases
 __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); __builtin_amdgcn_wave_barrier(); }
 
-__global__ void encode_init_kernel(uint64_t* status, int64_t n, uint32_t* counter) {
+__global__ void zero_words_kernel(uint64_t* p, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        status[i] = 0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0;
+        p[i] = 0;
 }
 
 constexpr int kWaves = 4;  // chunks per workgroup, one per wave64
@@ -177,39 +133,36 @@ static_assert(sizeof(WaveSlot) % 16 == 0, "slots must keep 16-byte alignment");
 // the three repetition-elimination rounds work on registers and shuffles.
 __global__ __launch_bounds__(64 * kWaves) void encode_kernel(const uint8_t* __restrict__ bp, ChunkGeom g,
                                                              uint8_t* __restrict__ dst_stream,
-                                                             uint64_t* __restrict__ cum,
-                                                             uint64_t* __restrict__ outsize,
-                                                             uint64_t* __restrict__ status,
-                                                             uint32_t* __restrict__ counter) {
+                                                             uint8_t* __restrict__ staging) {
     __shared__ WaveSlot slots[kWaves];
     const int lane = threadIdx.x & 63;
     WaveSlot& W = slots[threadIdx.x >> 6];
     uint8_t* out = reinterpret_cast<uint8_t*>(W.out);
-    uint32_t got = 0;
-    if (lane == 0) got = atomicAdd(counter, 1u);
-    const int64_t id = (int64_t)__shfl(got, 0, 64);
+    const int64_t id = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
     if (id >= g.nchunks) return;
     const ChunkPos cp = chunk_pos(g, id);
     const int osize = cp.osize;
     const uint8_t* src = bp + cp.base;
 
-    // zero elimination
-    int Z = 0;
-#pragma unroll 1
-    for (int k = 0; k < kCS / 1024; ++k) {
-        const int seg = 64 * k + lane;
-        const int b = 16 * seg;
-        uint32_t w[4] = {0u, 0u, 0u, 0u};
+    // zero elimination; the whole chunk is requested before the first step
+    // (16 KiB per wave, 64 registers per lane) so it costs one memory latency
+    constexpr int kSteps = kCS / 1024;
+    uint2 lo8[kSteps], hi8[kSteps];
+#pragma unroll
+    for (int k = 0; k < kSteps; ++k) {
+        const int b = 16 * (64 * k + lane);
+        lo8[k] = make_uint2(0u, 0u);
+        hi8[k] = make_uint2(0u, 0u);
         if (b < osize) {
-            const uint2 lo = *reinterpret_cast<const uint2*>(src + b);
-            w[0] = lo.x;
-            w[1] = lo.y;
-            if (b + 8 < osize) {
-                const uint2 hi = *reinterpret_cast<const uint2*>(src + b + 8);
-                w[2] = hi.x;
-                w[3] = hi.y;
-            }
+            lo8[k] = *reinterpret_cast<const uint2*>(src + b);
+            if (b + 8 < osize) hi8[k] = *reinterpret_cast<const uint2*>(src + b + 8);
         }
+    }
+    int Z = 0;
+#pragma unroll
+    for (int k = 0; k < kSteps; ++k) {
+        const int seg = 64 * k + lane;
+        const uint32_t w[4] = {lo8[k].x, lo8[k].y, hi8[k].x, hi8[k].y};
         const uint32_t m = nz_bytes(w[0]) | (nz_bytes(w[1]) << 4) | (nz_bytes(w[2]) << 8) | (nz_bytes(w[3]) << 12);
         const int cnt = __popc(m);
         const int incl = wave_incl_scan(cnt);
@@ -290,35 +243,79 @@ __global__ __launch_bounds__(64 * kWaves) void encode_kernel(const uint8_t* __re
         out[csize - 1] = (uint8_t)((osize >> 8) & 0xFF);
     }
 
-    // chunk offset by decoupled look-back
-    if (lane == 0) publish(status, id, id == 0 ? kPrefix : kAgg, (uint64_t)csize);
-    const int64_t excl = id == 0 ? 0 : look_back(status, id);
-    if (lane == 0 && id != 0) publish(status, id, kPrefix, (uint64_t)(excl + csize));
-    const int64_t incl_off = excl + csize;
+    // the encoded chunk goes to its 16 KiB staging slot (16-byte aligned);
+    // a raw chunk is copied later straight from the bit-planes
     wave_sync();
+    if (!raw) {
+        const uint4* sv = reinterpret_cast<const uint4*>(W.out);
+        uint4* dv = reinterpret_cast<uint4*>(staging + id * kCS);
+        for (int w = lane; w < (csize + 15) / 16; w += 64) dv[w] = sv[w];
+    }
+    if (lane == 0) reinterpret_cast<uint16_t*>(dst_stream + 8)[id] = (uint16_t)csize;
+}
 
-    // store at an arbitrary byte offset: head bytes, funnel-shifted words, tail bytes
-    uint8_t* d = dst_stream + 8 + 2 * g.nchunks + excl;
+// Payload offsets by one workgroup scan of the size table, then the stream
+// header, the stream length and the cumulative size table by (level, plane)
+// (zbpc.cu:176-181, 231-262).
+__global__ __launch_bounds__(1024) void stream_scan_kernel(const uint16_t* __restrict__ sizes, ChunkGeom g,
+                                                           int64_t* __restrict__ offsets, uint8_t* __restrict__ dst_stream,
+                                                           uint64_t* __restrict__ cum, uint64_t* __restrict__ outsize) {
+    __shared__ int64_t s_w[16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t n = g.nchunks, per = (n + 1023) / 1024;
+    const int64_t lo = tid * per, hi = lo + per < n ? lo + per : n;
+    int64_t sum = 0;
+    for (int64_t i = lo; i < hi; ++i) sum += sizes[i];
+    int64_t x = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int64_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) s_w[wave] = x;
+    __syncthreads();
+    int64_t run = x - sum;
+    for (int w = 0; w < wave; ++w) run += s_w[w];
+    for (int64_t i = lo; i < hi; ++i) {
+        offsets[i] = run;
+        run += sizes[i];
+        const ChunkPos cp = chunk_pos(g, i);
+        if (cp.last_of_plane) cum[(3 - cp.lv) * 32 + cp.plane] = (uint64_t)run;
+        if (i == n - 1) {
+            *reinterpret_cast<uint64_t*>(dst_stream) = (uint64_t)g.host_size;
+            *outsize = (uint64_t)(8 + 2 * n + run);
+        }
+    }
+}
+
+// One chunk per wave64: csize bytes from staging (or from the bit-planes for
+// a raw chunk) to their offset in the stream: head bytes, funnel-shifted
+// words, tail bytes.  No source word past the chunk's own bytes is read.
+__global__ __launch_bounds__(256) void gather_kernel(const uint8_t* __restrict__ bp, const uint8_t* __restrict__ staging,
+                                                     ChunkGeom g, const int64_t* __restrict__ offsets,
+                                                     uint8_t* __restrict__ dst_stream) {
+    const int lane = threadIdx.x & 63;
+    const int64_t id = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (id >= g.nchunks) return;
+    const int csize = reinterpret_cast<const uint16_t*>(dst_stream + 8)[id];
+    const ChunkPos cp = chunk_pos(g, id);
+    const bool raw = csize == cp.osize;
+    const uint8_t* sb = raw ? bp + cp.base : staging + id * kCS;  // both 4-byte aligned
+    const uint32_t* sw = reinterpret_cast<const uint32_t*>(sb);
+    const int swords = (csize + 3) >> 2;  // source words that hold chunk bytes
+    uint8_t* d = dst_stream + 8 + 2 * g.nchunks + offsets[id];
     int head = (int)((4 - ((uintptr_t)d & 3)) & 3);
     if (head > csize) head = csize;
-    const uint32_t* sw = raw ? reinterpret_cast<const uint32_t*>(src) : W.out;
-    const uint8_t* sb = raw ? src : out;
     if (lane < head) d[lane] = sb[lane];
     const int nw = (csize - head) >> 2;
     uint32_t* dw = reinterpret_cast<uint32_t*>(d + head);
-    for (int w = lane; w < nw; w += 64)
-        dw[w] = head == 0 ? sw[w] : (sw[w] >> (8 * head)) | (sw[w + 1] << (32 - 8 * head));
+    for (int w = lane; w < nw; w += 64) {
+        const uint32_t lo = sw[w];
+        const uint32_t hi = w + 1 < swords ? sw[w + 1] : 0u;
+        dw[w] = head == 0 ? lo : (lo >> (8 * head)) | (hi << (32 - 8 * head));
+    }
     const int tail0 = head + 4 * nw;
     if (lane < csize - tail0) d[tail0 + lane] = sb[tail0 + lane];
-
-    if (lane == 0) {
-        reinterpret_cast<uint16_t*>(dst_stream + 8)[id] = (uint16_t)csize;
-        if (cp.last_of_plane) cum[(3 - cp.lv) * 32 + cp.plane] = (uint64_t)incl_off;
-        if (id == g.nchunks - 1) {
-            *reinterpret_cast<uint64_t*>(dst_stream) = (uint64_t)g.host_size;
-            *outsize = (uint64_t)(8 + 2 * g.nchunks + incl_off);
-        }
-    }
 }
 
 // compress_merge: zero pad to 8 bytes, then flags, range, size table, anchors.
@@ -371,40 +368,89 @@ __global__ __launch_bounds__(1024) void size_scan_kernel(const uint16_t* __restr
     }
 }
 
+// Whether chunk cp holds a byte of a requested device bit-plane: planes
+// [begin[lr], end[lr]) of level lv span device bytes
+// [A_l + begin * s_l, A_l + end * s_l).
+__device__ __forceinline__ bool chunk_selected(const ChunkGeom& g, const ChunkPos& cp, const int32_t* begin,
+                                               const int32_t* end) {
+    bool sel = false;
+#pragma unroll
+    for (int lv = 0; lv < 4; ++lv) {
+        const int lr = 3 - lv;
+        const int b = begin[lr], e = end[lr];
+        if (b < e) {
+            const int64_t lo = g.dbase[lv] + (int64_t)b * g.dstride[lv];
+            const int64_t hi = g.dbase[lv] + (int64_t)e * g.dstride[lv];
+            if (cp.base < hi && cp.base + cp.osize > lo) sel = true;
+        }
+    }
+    return sel;
+}
+
+// One workgroup: the ids of the selected chunks in chunk order, their count,
+// and the payload bytes they hold (one atomic per call).
+__global__ __launch_bounds__(1024) void window_select_kernel(const uint16_t* __restrict__ sizes, ChunkGeom g,
+                                                             const int32_t* __restrict__ begin,
+                                                             const int32_t* __restrict__ end,
+                                                             int32_t* __restrict__ list, int32_t* __restrict__ count,
+                                                             uint64_t* __restrict__ bytes_read) {
+    __shared__ int s_w[16];
+    __shared__ unsigned long long s_b[16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t n = g.nchunks, per = (n + 1023) / 1024;
+    const int64_t lo = tid * per, hi = lo + per < n ? lo + per : n;
+    int cnt = 0;
+    unsigned long long bytes = 0;
+    for (int64_t i = lo; i < hi; ++i)
+        if (chunk_selected(g, chunk_pos(g, i), begin, end)) {
+            ++cnt;
+            bytes += sizes[i];
+        }
+    int x = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    unsigned long long bsum = bytes;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) bsum += __shfl_xor(bsum, d, 64);
+    if (lane == 63) s_w[wave] = x;
+    if (lane == 0) s_b[wave] = bsum;
+    __syncthreads();
+    int pos = x - cnt;
+    for (int w = 0; w < wave; ++w) pos += s_w[w];
+    for (int64_t i = lo; i < hi; ++i)
+        if (chunk_selected(g, chunk_pos(g, i), begin, end)) list[pos++] = (int32_t)i;
+    if (tid == 1023) *count = pos;
+    if (tid == 0 && bytes_read) {
+        unsigned long long t = 0;
+        for (int w = 0; w < 16; ++w) t += s_b[w];
+        atomicAdd((unsigned long long*)bytes_read, t);
+    }
+}
+
 // Decoder: one workgroup per chunk (chunks are independent once the
-// offsets are known).  WINDOW keeps only chunks holding a byte of a
-// requested device bit-plane.
-template <bool WINDOW>
-__global__ __launch_bounds__(kThreads) void decode_kernel(const uint8_t* __restrict__ in, ChunkGeom g,
-                                                          const int64_t* __restrict__ offsets,
-                                                          uint8_t* __restrict__ bp,
-                                                          const int32_t* __restrict__ begin,
-                                                          const int32_t* __restrict__ end,
-                                                          uint64_t* __restrict__ bytes_read,
-                                                          int32_t* __restrict__ err, int64_t in_len) {
-    __shared__ uint32_t s_c[kCS / 4 + 2];
-    __shared__ uint64_t s_b0[256];
-    __shared__ uint8_t s_b1[256];
-    __shared__ uint8_t s_b2[32];
-    __shared__ int s_scan[8];
+// offsets are known).  The windowed form walks the list of selected chunks.
+struct DecodeLds {
+    uint32_t c[kCS / 4 + 2];
+    uint64_t b0[256];
+    uint8_t b1[256];
+    uint8_t b2[32];
+    int scan[8];
+};
+
+__device__ void decode_chunk(const uint8_t* __restrict__ in, const ChunkGeom& g, const int64_t* __restrict__ offsets,
+                             uint8_t* __restrict__ bp, int32_t* __restrict__ err, int64_t in_len, int64_t id,
+                             DecodeLds& S) {
+    uint32_t* s_c = S.c;
+    uint64_t* s_b0 = S.b0;
+    uint8_t* s_b1 = S.b1;
+    uint8_t* s_b2 = S.b2;
+    int* s_scan = S.scan;
     const int tid = threadIdx.x;
-    const int64_t id = blockIdx.x;
     const ChunkPos cp = chunk_pos(g, id);
     const int osize = cp.osize;
-    if (WINDOW) {
-        bool sel = false;
-#pragma unroll
-        for (int lv = 0; lv < 4; ++lv) {
-            const int lr = 3 - lv;
-            const int b = begin[lr], e = end[lr];
-            if (b < e) {
-                const int64_t lo = g.dbase[lv] + (int64_t)b * g.dstride[lv];
-                const int64_t hi = g.dbase[lv] + (int64_t)e * g.dstride[lv];
-                if (cp.base < hi && cp.base + osize > lo) sel = true;
-            }
-        }
-        if (!sel) return;
-    }
     const uint16_t* sizes = reinterpret_cast<const uint16_t*>(in + 8);
     const int csize = sizes[id];
     const int64_t start = 8 + 2 * g.nchunks + offsets[id];
@@ -414,7 +460,6 @@ __global__ __launch_bounds__(kThreads) void decode_kernel(const uint8_t* __restr
         for (int i = tid; i < (osize >> 3); i += kThreads) dst0[i] = 0ull;
         return;
     }
-    if (WINDOW && bytes_read && tid == 0) atomicAdd((unsigned long long*)bytes_read, (unsigned long long)csize);
     const uint8_t* cbg = in + start;
     uint64_t* dst = reinterpret_cast<uint64_t*>(bp + cp.base);
     const int nwords = osize >> 3;
@@ -426,7 +471,20 @@ __global__ __launch_bounds__(kThreads) void decode_kernel(const uint8_t* __restr
     const int shift = (int)((uintptr_t)cbg & 3);
     const uint32_t* aw = reinterpret_cast<const uint32_t*>(cbg - shift);
     const int cw = (shift + csize + 3) >> 2;
-    for (int i = tid; i < cw; i += kThreads) s_c[i] = aw[i];
+    {  // all loads of a lane in flight before the first LDS write
+        constexpr int kPer = (kCS / 4 + 2 + kThreads - 1) / kThreads;
+        uint32_t t[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int i = tid + k * kThreads;
+            t[k] = i < cw ? aw[i] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int i = tid + k * kThreads;
+            if (i < cw) s_c[i] = t[k];
+        }
+    }
     __syncthreads();
     const uint8_t* cb = reinterpret_cast<const uint8_t*>(s_c) + shift;
     if (csize == osize) {  // stored raw
@@ -506,14 +564,35 @@ __global__ __launch_bounds__(kThreads) void decode_kernel(const uint8_t* __restr
     }
 }
 
+template <bool WINDOW>
+__global__ __launch_bounds__(kThreads) void decode_kernel(const uint8_t* __restrict__ in, ChunkGeom g,
+                                                          const int64_t* __restrict__ offsets,
+                                                          uint8_t* __restrict__ bp,
+                                                          const int32_t* __restrict__ list,
+                                                          const int32_t* __restrict__ count,
+                                                          int32_t* __restrict__ err, int64_t in_len) {
+    __shared__ DecodeLds S;
+    if (!WINDOW) {
+        decode_chunk(in, g, offsets, bp, err, in_len, blockIdx.x, S);
+        return;
+    }
+    const int n = *count;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        decode_chunk(in, g, offsets, bp, err, in_len, list[i], S);
+        __syncthreads();  // the next chunk reuses the LDS
+    }
+}
+
 }  // namespace
 }  // namespace pa
 
 using namespace pa;
 
+// offsets (8 B per chunk), then a 16 KiB staging slot per chunk (+16 B of
+// slack for the rounded-up copy of the last slot)
 extern "C" size_t pa_encode_workspace(const pa_layout* L) {
     if (!L) return 0;
-    return (size_t)L->nchunks * 8 + 256;
+    return ((size_t)L->nchunks * 8 + 255) / 256 * 256 + (size_t)L->nchunks * kCS + 256;
 }
 
 extern "C" int pa_lossless_encode(const uint8_t* d_bp, const pa_layout* L, uint8_t* d_out, uint64_t* d_cum,
@@ -522,15 +601,16 @@ extern "C" int pa_lossless_encode(const uint8_t* d_bp, const pa_layout* L, uint8
     if (!d_ws || ws_bytes < pa_encode_workspace(L)) return PA_ERR_WORKSPACE;
     hipStream_t st = as_stream(stream);
     ChunkGeom g = make_chunk_geom(L);
-    uint64_t* status = static_cast<uint64_t*>(d_ws);
-    uint32_t* counter = reinterpret_cast<uint32_t*>(status + L->nchunks);
-    int blocks = (int)((L->nchunks + 255) / 256);
-    if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(encode_init_kernel, dim3(blocks), dim3(256), 0, st, status, L->nchunks, counter);
-    PA_LAUNCH_CHECK();
+    int64_t* offsets = static_cast<int64_t*>(d_ws);
+    uint8_t* staging = static_cast<uint8_t*>(d_ws) + ((size_t)L->nchunks * 8 + 255) / 256 * 256;
     const unsigned eblocks = (unsigned)((L->nchunks + kWaves - 1) / kWaves);
-    hipLaunchKernelGGL(encode_kernel, dim3(eblocks), dim3(64 * kWaves), 0, st, d_bp, g, d_out, d_cum, d_outsize,
-                       status, counter);
+    hipLaunchKernelGGL(encode_kernel, dim3(eblocks), dim3(64 * kWaves), 0, st, d_bp, g, d_out, staging);
+    PA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(stream_scan_kernel, dim3(1), dim3(1024), 0, st, reinterpret_cast<const uint16_t*>(d_out + 8),
+                       g, offsets, d_out, d_cum, d_outsize);
+    PA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(gather_kernel, dim3((unsigned)((L->nchunks + 3) / 4)), dim3(256), 0, st, d_bp, staging, g,
+                       offsets, d_out);
     PA_LAUNCH_CHECK();
     return PA_OK;
 }
@@ -553,7 +633,7 @@ extern "C" int pa_compress_merge(uint8_t* d_out, const pa_layout* L, int dtype, 
 // workspace: int32 error word (+pad) | int64 offsets[nchunks]
 extern "C" size_t pa_decode_workspace(const pa_layout* L) {
     if (!L) return 0;
-    return 64 + (size_t)L->nchunks * 8;
+    return 64 + (size_t)L->nchunks * 8 + ((size_t)L->nchunks * 4 + 63) / 64 * 64;
 }
 
 static int decode_common(const uint8_t* d_in, int64_t in_len, const pa_layout* L, const int32_t* d_begin, const int32_t* d_end,
@@ -566,19 +646,25 @@ static int decode_common(const uint8_t* d_in, int64_t in_len, const pa_layout* L
     int32_t* err = static_cast<int32_t*>(d_ws);
     int64_t* offsets = reinterpret_cast<int64_t*>(static_cast<uint8_t*>(d_ws) + 64);
     if (!offsets_ready) {
-        hipLaunchKernelGGL(encode_init_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<uint64_t*>(d_ws), 8,
-                           reinterpret_cast<uint32_t*>(d_ws));
+        hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<uint64_t*>(d_ws), 8);
         PA_LAUNCH_CHECK();
         hipLaunchKernelGGL(size_scan_kernel, dim3(1), dim3(1024), 0, st,
                            reinterpret_cast<const uint16_t*>(d_in + 8), L->nchunks, offsets);
         PA_LAUNCH_CHECK();
     }
-    if (window)
-        hipLaunchKernelGGL(decode_kernel<true>, dim3((unsigned)L->nchunks), dim3(kThreads), 0, st, d_in, g, offsets,
-                           d_bp, d_begin, d_end, d_bytes_read, err, in_len);
-    else
-        hipLaunchKernelGGL(decode_kernel<false>, dim3((unsigned)L->nchunks), dim3(kThreads), 0, st, d_in, g,
-                           offsets, d_bp, nullptr, nullptr, nullptr, err, in_len);
+    if (window) {
+        int32_t* list = reinterpret_cast<int32_t*>(offsets + L->nchunks);
+        int32_t* count = err + 1;
+        hipLaunchKernelGGL(window_select_kernel, dim3(1), dim3(1024), 0, st,
+                           reinterpret_cast<const uint16_t*>(d_in + 8), g, d_begin, d_end, list, count, d_bytes_read);
+        PA_LAUNCH_CHECK();
+        const unsigned grid = (unsigned)(L->nchunks < 2048 ? L->nchunks : 2048);
+        hipLaunchKernelGGL(decode_kernel<true>, dim3(grid), dim3(kThreads), 0, st, d_in, g, offsets, d_bp, list, count,
+                           err, in_len);
+    } else {
+        hipLaunchKernelGGL(decode_kernel<false>, dim3((unsigned)L->nchunks), dim3(kThreads), 0, st, d_in, g, offsets,
+                           d_bp, nullptr, nullptr, err, in_len);
+    }
     PA_LAUNCH_CHECK();
     return PA_OK;
 }

@@ -34,14 +34,19 @@ __global__ __launch_bounds__(256) void extrema_kernel(const T* __restrict__ in, 
     const int64_t nv = n / V;
     const Vec* vin = reinterpret_cast<const Vec*>(in);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    // kBatch independent 16-byte loads per lane are in flight before any is
-    // reduced: one load per wave at a time would cap the pass at a few TB/s
+    // Each workgroup streams one contiguous chunk of whole 32 KiB batches;
+    // a batch puts kBatch independent 16-byte loads per lane in flight
+    // before any is reduced.
     constexpr int kBatch = 8;
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    for (; i + (kBatch - 1) * stride < nv; i += kBatch * stride) {
+    constexpr int64_t kSpan = (int64_t)kBatch * 256;  // vectors per batch
+    const int64_t batches = nv / kSpan;
+    const int64_t per = (batches + gridDim.x - 1) / gridDim.x;
+    const int64_t b0 = blockIdx.x * per, b1 = b0 + per < batches ? b0 + per : batches;
+    for (int64_t bt = b0; bt < b1; ++bt) {
+        const Vec* p = vin + bt * kSpan + threadIdx.x;
         Vec v[kBatch];
 #pragma unroll
-        for (int k = 0; k < kBatch; ++k) v[k] = vin[i + k * stride];
+        for (int k = 0; k < kBatch; ++k) v[k] = p[k * 256];
 #pragma unroll
         for (int k = 0; k < kBatch; ++k) {
             const T* e = reinterpret_cast<const T*>(&v[k]);
@@ -52,7 +57,7 @@ __global__ __launch_bounds__(256) void extrema_kernel(const T* __restrict__ in, 
             }
         }
     }
-    for (; i < nv; i += stride) {
+    for (int64_t i = batches * kSpan + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv; i += stride) {
         const Vec v = vin[i];
         const T* e = reinterpret_cast<const T*>(&v);
 #pragma unroll
@@ -81,8 +86,12 @@ __global__ __launch_bounds__(256) void extrema_kernel(const T* __restrict__ in, 
             mn = fmin(mn, s_min[w]);
             mx = fmax(mx, s_max[w]);
         }
-        atomicMin(keys, order_key(mn));
-        atomicMax(keys + 1, order_key(mx));
+        // atomics on one address from every workgroup serialise (about 12 ns
+        // each on MI355X); a workgroup that cannot improve the running value
+        // skips its atomic, and the running values settle after a few
+        const unsigned long long kmn = order_key(mn), kmx = order_key(mx);
+        if (kmn < __hip_atomic_load(keys, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(keys, kmn);
+        if (kmx > __hip_atomic_load(keys + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(keys + 1, kmx);
     }
 }
 

@@ -606,8 +606,11 @@ __device__ __forceinline__ void level(Ctx<T, IN>& t, bool md, bool rev) {
     } else {
         line_stage<T, U, BLUE, false, M>(t);
         line_stage<T, U, YELLOW, false, M>(t);
-        if constexpr (EX) hollow0_stage<T, M>(t);
-        else line_stage<T, U, HOLLOW, false, M>(t);
+        // compression stores codes, which coalesce in the column form;
+        // decompression reads them from LDS, where x-lines with one stencil
+        // per point position are cheaper
+        if constexpr (EX && M == COMPRESS) hollow0_stage<T, M>(t);
+        else line_stage<T, U, HOLLOW, EX, M>(t);
     }
 }
 

@@ -7,8 +7,8 @@
 // returns the tied index with the smallest bit-reversed value, so the same
 // winner is the maximum of one 64-bit key
 //     value << 22 | (63 - rev6(b0)) << 16 | (1023 - rev10(tid)) << 6 | (63 - k)
-// (pos = tid + 1024k), which any reduction order finds.  No grid barrier and
-// no co-residency assumption: waves fold their maxima with a 64-bit atomicMax.
+// (pos = tid + 1024k), which any reduction order finds: each workgroup
+// writes the maximum key of its candidates and one workgroup reduces them.
 #include "common.hpp"
 
 namespace pa {
@@ -26,52 +26,9 @@ __device__ __forceinline__ uint64_t plane_cost(int lr, int b) {
     return (num + d - 1) / d;
 }
 
-__global__ void strategy_reset_kernel(unsigned long long* key) { *key = 0ull; }
-
-__global__ __launch_bounds__(256) void strategy_search_kernel(const uint64_t* __restrict__ cum,
-                                                              uint64_t target_cost,
-                                                              unsigned long long* __restrict__ best) {
-    __shared__ uint64_t s_val[4][33], s_cost[4][33];
-    const int tid = threadIdx.x;
-    if (tid < 4 * 33) {
-        const int lr = tid / 33, b = tid % 33;
-        uint64_t v;
-        if (b == 0) v = cum[lr * 32 + 31];  // cumulative quirk, dataloader.cu:47
-        else if (b == 32) v = 0;
-        else v = cum[lr * 32 + 31] - cum[lr * 32 + b - 1];
-        s_val[lr][b] = v;
-        s_cost[lr][b] = plane_cost(lr, b);
-    }
-    __syncthreads();
-    const int b0 = blockIdx.x;
-    const uint64_t c0 = s_cost[0][b0], v0 = s_val[0][b0];
-    const uint64_t tie0 = (uint64_t)(63u - (__brev((uint32_t)b0) >> 26)) << 16;
-    unsigned long long best_key = 0ull;
-    for (int pos = tid; pos < 33 * 33 * 33; pos += 256) {
-        const int b1 = pos / 1089, b2 = (pos / 33) % 33, b3 = pos % 33;
-        const uint64_t tc = c0 + s_cost[1][b1] + s_cost[2][b2] + s_cost[3][b3];
-        if (tc > target_cost) continue;
-        const uint64_t v = v0 + s_val[1][b1] + s_val[2][b2] + s_val[3][b3];
-        if (v == 0) continue;
-        const uint64_t key = (v << 22) | tie0 |
-                             ((uint64_t)(1023u - (__brev((uint32_t)(pos & 1023)) >> 22)) << 6) |
-                             (uint64_t)(63 - (pos >> 10));
-        if (key > best_key) best_key = key;
-    }
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        const unsigned long long o = __shfl_xor(best_key, d, 64);
-        best_key = o > best_key ? o : best_key;
-    }
-    if ((tid & 63) == 0 && best_key) atomicMax(best, best_key);
-}
-
 // begin <- end; end <- max(end, choice); maxlevel = largest level that
 // advanced (dataloader.cu:135-163), 4 when none did
-__global__ void strategy_apply_kernel(const unsigned long long* __restrict__ best, int32_t* begin, int32_t* end,
-                                      int32_t* maxlevel) {
-    if (threadIdx.x != 0) return;
-    const unsigned long long key = *best;
+__device__ void strategy_apply(unsigned long long key, int32_t* begin, int32_t* end, int32_t* maxlevel) {
     int ch[4] = {32, 32, 32, 32};
     if (key) {
         const int k = 63 - (int)(key & 63);
@@ -96,6 +53,78 @@ __global__ void strategy_apply_kernel(const unsigned long long* __restrict__ bes
     if (maxlevel) *maxlevel = ml;
 }
 
+constexpr int kSearchBlocks = 33 * 33;
+
+// Grid (b0, b1) = 33 x 33 workgroups; lane p of a workgroup scores b2 =
+// p / 33, b3 = p mod 33 (p < 1089) and the workgroup writes its best key to
+// its own slot (no shared atomics: they serialise across workgroups).
+__global__ __launch_bounds__(256) void strategy_search_kernel(const uint64_t* __restrict__ cum,
+                                                              uint64_t target_cost,
+                                                              unsigned long long* __restrict__ keys) {
+    __shared__ uint64_t s_val[4][33], s_cost[4][33];
+    __shared__ unsigned long long s_best[4];
+    const int tid = threadIdx.x;
+    if (tid < 4 * 33) {
+        const int lr = tid / 33, b = tid % 33;
+        uint64_t v;
+        if (b == 0) v = cum[lr * 32 + 31];  // cumulative quirk, dataloader.cu:47
+        else if (b == 32) v = 0;
+        else v = cum[lr * 32 + 31] - cum[lr * 32 + b - 1];
+        s_val[lr][b] = v;
+        s_cost[lr][b] = plane_cost(lr, b);
+    }
+    __syncthreads();
+    const int b0 = blockIdx.x, b1 = blockIdx.y;
+    const uint64_t c01 = s_cost[0][b0] + s_cost[1][b1], v01 = s_val[0][b0] + s_val[1][b1];
+    const uint64_t tie0 = (uint64_t)(63u - (__brev((uint32_t)b0) >> 26)) << 16;
+    unsigned long long best_key = 0ull;
+    if (c01 <= target_cost) {
+        for (int p = tid; p < 33 * 33; p += 256) {
+            const int b2 = p / 33, b3 = p % 33;
+            const uint64_t tc = c01 + s_cost[2][b2] + s_cost[3][b3];
+            if (tc > target_cost) continue;
+            const uint64_t v = v01 + s_val[2][b2] + s_val[3][b3];
+            if (v == 0) continue;
+            const int pos = b1 * 1089 + p;  // the reference's thread + 1024 k position
+            const uint64_t key = (v << 22) | tie0 |
+                                 ((uint64_t)(1023u - (__brev((uint32_t)(pos & 1023)) >> 22)) << 6) |
+                                 (uint64_t)(63 - (pos >> 10));
+            if (key > best_key) best_key = key;
+        }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const unsigned long long o = __shfl_xor(best_key, d, 64);
+        best_key = o > best_key ? o : best_key;
+    }
+    if ((tid & 63) == 0) s_best[tid >> 6] = best_key;
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long k = s_best[0];
+        for (int w = 1; w < 4; ++w) k = s_best[w] > k ? s_best[w] : k;
+        keys[blockIdx.y * gridDim.x + blockIdx.x] = k;
+    }
+}
+
+// maximum of the workgroup keys, then the update of begin / end / maxlevel
+__global__ __launch_bounds__(256) void strategy_apply_kernel(const unsigned long long* __restrict__ keys,
+                                                             int32_t* begin, int32_t* end, int32_t* maxlevel) {
+    __shared__ unsigned long long s_best[4];
+    unsigned long long k = 0ull;
+    for (int i = threadIdx.x; i < kSearchBlocks; i += 256) k = keys[i] > k ? keys[i] : k;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const unsigned long long o = __shfl_xor(k, d, 64);
+        k = o > k ? o : k;
+    }
+    if ((threadIdx.x & 63) == 0) s_best[threadIdx.x >> 6] = k;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) k = s_best[w] > k ? s_best[w] : k;
+        strategy_apply(k, begin, end, maxlevel);
+    }
+}
+
 // loadall (dataloader.cu:21-28)
 __global__ void load_all_kernel(int32_t* begin, int32_t* end, int32_t* maxlevel) {
     if (threadIdx.x != 0) return;
@@ -110,7 +139,7 @@ __global__ void load_all_kernel(int32_t* begin, int32_t* end, int32_t* maxlevel)
 
 using namespace pa;
 
-extern "C" size_t pa_strategy_workspace(void) { return 64; }
+extern "C" size_t pa_strategy_workspace(void) { return sizeof(unsigned long long) * kSearchBlocks; }
 
 extern "C" int pa_find_strategy(const uint64_t* d_cum, int32_t* d_begin, int32_t* d_end, int32_t* d_maxlevel,
                                 double eb, double target_error, void* d_ws, size_t ws_bytes, void* stream) {
@@ -121,12 +150,10 @@ extern "C" int pa_find_strategy(const uint64_t* d_cum, int32_t* d_begin, int32_t
         const double q = target_error * 0.954 / eb;
         if (!(q < 1.8e19)) return PA_ERR_ARG;
         const uint64_t target_cost = (uint64_t)__builtin_floor(q) - 1;
-        unsigned long long* key = static_cast<unsigned long long*>(d_ws);
-        hipLaunchKernelGGL(strategy_reset_kernel, dim3(1), dim3(1), 0, st, key);
+        unsigned long long* keys = static_cast<unsigned long long*>(d_ws);
+        hipLaunchKernelGGL(strategy_search_kernel, dim3(33, 33), dim3(256), 0, st, d_cum, target_cost, keys);
         PA_LAUNCH_CHECK();
-        hipLaunchKernelGGL(strategy_search_kernel, dim3(33), dim3(256), 0, st, d_cum, target_cost, key);
-        PA_LAUNCH_CHECK();
-        hipLaunchKernelGGL(strategy_apply_kernel, dim3(1), dim3(64), 0, st, key, d_begin, d_end, d_maxlevel);
+        hipLaunchKernelGGL(strategy_apply_kernel, dim3(1), dim3(256), 0, st, keys, d_begin, d_end, d_maxlevel);
         PA_LAUNCH_CHECK();
     } else {
         hipLaunchKernelGGL(load_all_kernel, dim3(1), dim3(64), 0, st, d_begin, d_end, d_maxlevel);
