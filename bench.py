@@ -32,7 +32,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
-    ap.add_argument("--cpu-sample-planes", type=int, default=64,
+    ap.add_argument("--cpu-sample-planes", type=int, default=256,
                     help="z-planes of the field timed on the host cores for the CPU baseline")
     ap.add_argument("--shape", type=int, nargs=3, default=[384, 384, 256], metavar=("NX", "NY", "NZ"))
     ap.add_argument("--rel-eb", type=float, default=1e-6)
@@ -174,6 +174,7 @@ def main():
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": d["gbps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": (d["gbps"] / HBM_PEAK_GBPS) if d["gbps"] else None, "traffic": None,
                 "avg_launch_ms": d["avg_ms"], "algorithmic_bytes_per_launch": d["avg_bytes"]}
+    roofline.update(measured_traffic(dominant, nx, ny, nz, s))
 
     cpu = None
     if args.cpu_baseline == "on" and int(os.environ.get("RANK", "0")) == 0:
@@ -190,6 +191,23 @@ def main():
         "levels": levels, "stages": stats, "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line))
+
+
+def measured_traffic(stage, nx, ny, nz, s):
+    """HBM bytes per launch of a stage's kernel from the committed rocprofv3
+    counter summary of this workload (FETCH_SIZE and WRITE_SIZE passes, each
+    scaled by the same counter on a copy kernel of known size); empty when no
+    summary covers this shape and dtype."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"traffic_density_{nx}x{ny}x{nz}_f{8 * s}.json")
+    try:
+        with open(path) as f:
+            prof = json.load(f)
+        st = prof["stages"][stage]
+    except (OSError, KeyError, ValueError):
+        return {}
+    return {"traffic": st["traffic_bytes"], "traffic_fetch": st["fetch_bytes"], "traffic_write": st["write_bytes"],
+            "traffic_source": os.path.relpath(path, os.path.dirname(os.path.abspath(__file__)))}
 
 
 def cpu_baseline(field, args):

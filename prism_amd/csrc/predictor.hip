@@ -332,6 +332,7 @@ constexpr int level_of() {
 __device__ __forceinline__ int lane_id() {
     int v;
     asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "v"((int)threadIdx.x));
+    __builtin_assume(v >= 0 && v < kThreads);
     return v;
 }
 
@@ -814,15 +815,16 @@ __device__ __forceinline__ void tile_run(const SplineArgs<T>& a, T* tile, int32_
     constexpr int OIT = 4096 / kThreads;
     T prev[OIT];
     if constexpr (M == DECOMPRESS) {
-        if (a.old) {
+        if (a.old) {  // lane: (x, y) = (l mod 16, l / 16); iteration: z
             const int lane = lane_id();
+            const int gx = t.ox + (lane & 15), gy = t.oy + (lane >> 4);
+            const bool in_xy = t.template inf<0>(gx) && t.template inf<1>(gy);
+            const int64_t nxy = (int64_t)g.nx * g.ny;
+            const T* src = a.old + (int64_t)gx + (int64_t)gy * g.nx + (int64_t)t.oz * nxy;
 #pragma unroll
-            for (int it = 0; it < OIT; ++it) {
-                const int i = lane + it * kThreads;
-                const int gx = t.ox + (i & 15), gy = t.oy + ((i >> 4) & 15), gz = t.oz + (i >> 8);
-                prev[it] = (T)0;
-                if (t.template inf<0>(gx) && t.template inf<1>(gy) && t.template inf<2>(gz))
-                    prev[it] = a.old[(int64_t)gx + (int64_t)gy * g.nx + (int64_t)gz * g.nx * g.ny];
+            for (int z = 0; z < OIT; ++z) {
+                prev[z] = (T)0;
+                if (in_xy && t.template inf<2>(t.oz + z)) prev[z] = src[z * nxy];
             }
         }
     }
@@ -863,16 +865,18 @@ __device__ __forceinline__ void tile_run(const SplineArgs<T>& a, T* tile, int32_
     if (1 <= maxlevel) { t.eb_r = le.eb_r[1]; t.ebx2 = le.ebx2[1]; level<T, 2, M>(t, fl[1], fl[5]); }
     if (0 <= maxlevel) { t.eb_r = le.eb_r[0]; t.ebx2 = le.ebx2[0]; level<T, 1, M>(t, fl[0], fl[4]); }
 
-    if constexpr (M == DECOMPRESS) {
+    if constexpr (M == DECOMPRESS) {  // lane: (x, y); iteration: z
         const int lane = lane_id();
+        const int x = lane & 15, y = lane >> 4;
+        const int gx = t.ox + x, gy = t.oy + y;
+        const bool in_xy = t.template inf<0>(gx) && t.template inf<1>(gy);
+        const int64_t nxy = (int64_t)g.nx * g.ny;
+        T* dst = a.out + (int64_t)gx + (int64_t)gy * g.nx + (int64_t)t.oz * nxy;
 #pragma unroll
-        for (int it = 0; it < OIT; ++it) {
-            const int i = lane + it * kThreads;
-            const int x = i & 15, y = (i >> 4) & 15, z = i >> 8;
-            const int gx = t.ox + x, gy = t.oy + y, gz = t.oz + z;
-            if (t.template inf<0>(gx) && t.template inf<1>(gy) && t.template inf<2>(gz)) {
+        for (int z = 0; z < OIT; ++z) {
+            if (in_xy && t.template inf<2>(t.oz + z)) {
                 const T v = tile[sidx(x, y, z)];
-                a.out[(int64_t)gx + (int64_t)gy * g.nx + (int64_t)gz * g.nx * g.ny] = a.old ? v + prev[it] : v;
+                dst[z * nxy] = a.old ? v + prev[z] : v;
             }
         }
     }
