@@ -150,6 +150,14 @@ int pa_find_strategy(const uint64_t* d_cum, int32_t* d_begin, int32_t* d_end,
                      int32_t* d_maxlevel, double eb, double target_error, void* d_ws,
                      size_t ws_bytes, void* stream);
 
+/* statistic (include/utils/analyze.hpp:5-70) on the device, in one pass:
+ * d_out (double[9]) = { L-inf error, its first index, sum of squared errors,
+ * min and max of d_ori, MSE, PSNR (dB), NRMSE, Max_RE }.  Deterministic
+ * (fixed reduction order). */
+size_t pa_quality_workspace(int64_t n);
+int pa_quality(const void* d_ori, const void* d_rec, int dtype, int64_t n, double* d_out, void* d_ws,
+               size_t ws_bytes, void* stream);
+
 /* Library / device information (no reference counterpart). */
 const char* pa_version(void);
 int pa_device_count(void);

@@ -82,6 +82,8 @@ SIGNATURES = {
     "pa_lossless_decode_window": (I32, [P, I64, LP, P, P, P, P, I32, P, SZ, P]),
     "pa_strategy_workspace": (SZ, []),
     "pa_find_strategy": (I32, [P, P, P, P, D, D, P, SZ, P]),
+    "pa_quality_workspace": (SZ, [I64]),
+    "pa_quality": (I32, [P, P, I32, I64, P, P, SZ, P]),
     "pa_version": (C.c_char_p, []),
     "pa_device_count": (I32, []),
 }

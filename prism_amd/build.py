@@ -22,7 +22,7 @@ LIB = os.path.join(OUT, "libprism_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PRISM_AMD_ARCH", "gfx950")
 
-SOURCES = ["layout.cpp", "extrema.hip", "predictor.hip", "bitplane.hip", "codec.hip", "strategy.hip"]
+SOURCES = ["layout.cpp", "extrema.hip", "predictor.hip", "bitplane.hip", "codec.hip", "strategy.hip", "quality.hip"]
 HEADERS = ["common.hpp"]
 
 # C++ API (prism::Compressor & co.) and the `prism` command line, host code

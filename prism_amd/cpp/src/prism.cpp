@@ -51,11 +51,27 @@ void load_stream(prism::Compressor<T, i4>& d, const prism_context* c) {
     d.compressed_data->H2D();
 }
 
+// statistic (analyze.hpp:5-70) on the device: the original is uploaded once
+// and compared with the reconstruction where it lies, in HBM
 template <typename T>
 void report_quality(const prism_context* c, prism::StatBuffer<T>* out) {
     auto ori = field<T>(c);
     ori->template load_fromfile<ori_File>(c->oriFilePath);
-    prism::statistic<T>(static_cast<const T*>(ori->h), static_cast<const T*>(out->h), c->size);
+    ori->H2D();
+    const int64_t n = (int64_t)c->size;
+    const size_t wsb = pa_quality_workspace(n);
+    void* ws = nullptr;
+    double* d_q = nullptr;
+    check_hip(hipMalloc(&ws, wsb), "hipMalloc");
+    check_hip(hipMalloc((void**)&d_q, 9 * sizeof(double)), "hipMalloc");
+    prism::check_pa(pa_quality(ori->d, out->d, sizeof(T) == 4 ? PA_F32 : PA_F64, n, d_q, ws, wsb, nullptr), "pa_quality");
+    double q[9];
+    check_hip(hipMemcpy(q, d_q, sizeof(q), hipMemcpyDeviceToHost), "D2H");
+    check_hip(hipFree(ws), "hipFree");
+    check_hip(hipFree(d_q), "hipFree");
+    std::printf("  Max_E  = %.15lf, idx = %d\n", q[0], (int)q[1]);
+    std::printf("  Max_RE = %G\n", q[8]);
+    std::printf("  PSNR = %.7lf, NRMSE = %.10G\n", q[6], q[7]);
 }
 
 template <typename T>

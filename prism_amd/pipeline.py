@@ -119,6 +119,25 @@ def extrema(data: torch.Tensor):
     return mm
 
 
+QUALITY_FIELDS = ("linf", "linf_index", "sse", "min", "max", "mse", "psnr_db", "nrmse", "max_rel_err")
+
+
+def quality(original: torch.Tensor, reconstruction: torch.Tensor) -> dict:
+    """Error statistics of a reconstruction (statistic, analyze.hpp:5-70),
+    computed on the device in one pass; one synchronisation to read them."""
+    if original.shape != reconstruction.shape or original.dtype != reconstruction.dtype:
+        raise _lib.PrismError("quality: original and reconstruction differ in shape or dtype")
+    n = original.numel()
+    out = torch.empty(9, dtype=torch.float64, device=original.device)
+    ws = _empty(lib().pa_quality_workspace(n))
+    check(lib().pa_quality(_p(original), _p(reconstruction), DTYPES[original.dtype], n, _p(out), _p(ws), ws.numel(),
+                           _stream()), "pa_quality")
+    vals = out.cpu().tolist()
+    res = dict(zip(QUALITY_FIELDS, vals))
+    res["linf_index"] = int(res["linf_index"])
+    return res
+
+
 def compress(data: torch.Tensor, eb: float, rel: bool = True, progressive: bool = True,
              return_parts: bool = False, timer=None):
     """Compresses a contiguous device field (T[nz][ny][nx])."""

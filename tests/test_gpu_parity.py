@@ -271,3 +271,29 @@ def test_end_to_end_stream_parity(pa, oracle, shape, dtype, rel, progressive):
             r_o, ml_o, _ = po.retrieve(t)
             r_g = host(rv.retrieve(t))
             assert np.array_equal(r_g.view(np.uint8), r_o.view(np.uint8)), t
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_quality_statistics(pa, dtype):
+    """Device statistics against numpy: L-inf and its first index exact,
+    min / max exact, sums within rounding; ties resolve to the first index."""
+    _lib, pipe = pa
+    rng = np.random.default_rng(3)
+    for n in (1, 257, (1 << 20) + 3):
+        announce("quality", dtype.__name__, n)
+        a = rng.normal(0, 1, n).astype(dtype)
+        b = (a + rng.normal(0, 1e-3, n)).astype(dtype)
+        if n > 300:  # two equal largest errors: the first index wins
+            b[100] = a[100] + dtype(0.5)
+            b[200] = a[200] + dtype(0.5)
+        q = pipe.quality(dev(a), dev(b))
+        e = np.abs(b.astype(np.float64) - a.astype(np.float64))
+        assert q["linf"] == e.max() and q["linf_index"] == int(np.argmax(e))
+        assert q["min"] == float(a.min()) and q["max"] == float(a.max())
+        sse = float((e * e).sum())
+        assert abs(q["sse"] - sse) <= 1e-12 * max(sse, 1e-300)
+        rg = float(a.max()) - float(a.min())
+        if rg > 0 and sse > 0:
+            mse = sse / n
+            assert abs(q["psnr_db"] - (20 * np.log10(rg) - 10 * np.log10(mse))) < 1e-9
+            assert abs(q["nrmse"] - np.sqrt(mse) / rg) <= 1e-12 * np.sqrt(mse) / rg
