@@ -110,7 +110,8 @@ int pa_inverse_bitshuffle_window(const uint8_t* d_bp, const pa_layout* L, int mo
 /* lossless_encode (zbpc.cu:476-520, kernel zbpc_encode 155-263): writes
  * header, size table and payloads to d_out (capacity >= pa_compress_bound),
  * the 4x32 cumulative size table to d_cum and the unpadded stream length to
- * d_outsize.  Single pass with a decoupled look-back for chunk offsets. */
+ * d_outsize.  Three passes: chunks encoded into staging slots, payload
+ * offsets by a scan of the size table, a gather into the stream. */
 size_t pa_encode_workspace(const pa_layout* L);
 int pa_lossless_encode(const uint8_t* d_bp, const pa_layout* L, uint8_t* d_out, uint64_t* d_cum,
                        uint64_t* d_outsize, void* d_ws, size_t ws_bytes, void* stream);
@@ -157,6 +158,44 @@ int pa_find_strategy(const uint64_t* d_cum, int32_t* d_begin, int32_t* d_end,
 size_t pa_quality_workspace(int64_t n);
 int pa_quality(const void* d_ori, const void* d_rec, int dtype, int64_t n, double* d_out, void* d_ws,
                size_t ws_bytes, void* stream);
+
+/* ---- Batched compression (no reference counterpart) ----
+ * Many fields of mixed shapes and dtypes compressed by one fixed sequence of
+ * launches: one per stage (extrema, bound, tuner, code gaps, spline,
+ * shuffle, encode, scan, gather, trailer; at most 15 kernels) whatever the
+ * number of fields, optionally captured once in a HIP graph and replayed.
+ * Each field's stream is byte-identical to the one the plain per-field
+ * calls (pa_extrema .. pa_compress_merge) produce for the same input. */
+typedef struct pa_batch_field {
+    const void* data; /* device T[nz][ny][nx], 16-byte aligned */
+    int32_t dtype;    /* PA_F32 / PA_F64 */
+    int32_t mode;     /* PA_MODE_NB (progressive) / PA_MODE_SM */
+    int32_t rel;      /* 1: eb is relative to the field's value range */
+    int32_t reserved;
+    double eb;
+    uint32_t nx, ny, nz, reserved2;
+    uint8_t* out; /* device, >= pa_compress_bound(layout, dtype) bytes */
+} pa_batch_field;
+
+typedef struct pa_batch_plan pa_batch_plan;
+
+/* device workspace of a batch: per-field scratch plus the descriptor tables */
+size_t pa_batch_workspace(const pa_batch_field* fields, int nf);
+/* Plans a batch (host only; uploads the descriptor tables synchronously).
+ * d_result: device uint64[4 * nf]; after each run, entry 4i.. of field i =
+ * { stream length, status (0 or PA_ERR_ARG for a non-positive bound, e.g. a
+ * constant field in relative mode), value range and absolute bound as f64
+ * bit patterns }.  use_graph != 0 captures the launch sequence once in a HIP
+ * graph that every run replays. */
+int pa_batch_plan_create(const pa_batch_field* fields, int nf, void* d_ws, size_t ws_bytes,
+                         uint64_t* d_result, int use_graph, pa_batch_plan** plan);
+/* Points the plan at new inputs / outputs of the same shapes, dtypes, modes
+ * and bound kinds (bounds may change); synchronous upload. */
+int pa_batch_plan_rebind(pa_batch_plan* plan, const pa_batch_field* fields, int nf);
+int pa_batch_run(pa_batch_plan* plan, void* stream);
+/* kernel launches one run issues (graph nodes when captured) */
+int pa_batch_plan_launches(const pa_batch_plan* plan);
+int pa_batch_plan_destroy(pa_batch_plan* plan);
 
 /* Library / device information (no reference counterpart). */
 const char* pa_version(void);

@@ -53,6 +53,15 @@ class Layout(C.Structure):
         return out
 
 
+class BatchFieldDesc(C.Structure):
+    """pa_batch_field"""
+    _fields_ = [
+        ("data", C.c_void_p), ("dtype", C.c_int32), ("mode", C.c_int32), ("rel", C.c_int32),
+        ("reserved", C.c_int32), ("eb", C.c_double), ("nx", C.c_uint32), ("ny", C.c_uint32),
+        ("nz", C.c_uint32), ("reserved2", C.c_uint32), ("out", C.c_void_p),
+    ]
+
+
 P = C.c_void_p
 I32 = C.c_int
 I64 = C.c_int64
@@ -84,6 +93,12 @@ SIGNATURES = {
     "pa_find_strategy": (I32, [P, P, P, P, D, D, P, SZ, P]),
     "pa_quality_workspace": (SZ, [I64]),
     "pa_quality": (I32, [P, P, I32, I64, P, P, SZ, P]),
+    "pa_batch_workspace": (SZ, [P, I32]),
+    "pa_batch_plan_create": (I32, [P, I32, P, SZ, P, I32, C.POINTER(C.c_void_p)]),
+    "pa_batch_plan_rebind": (I32, [P, P, I32]),
+    "pa_batch_run": (I32, [P, P]),
+    "pa_batch_plan_launches": (I32, [P]),
+    "pa_batch_plan_destroy": (I32, [P]),
     "pa_version": (C.c_char_p, []),
     "pa_device_count": (I32, []),
 }

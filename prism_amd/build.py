@@ -22,8 +22,9 @@ LIB = os.path.join(OUT, "libprism_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PRISM_AMD_ARCH", "gfx950")
 
-SOURCES = ["layout.cpp", "extrema.hip", "predictor.hip", "bitplane.hip", "codec.hip", "strategy.hip", "quality.hip"]
-HEADERS = ["common.hpp"]
+SOURCES = ["layout.cpp", "extrema.hip", "predictor.hip", "bitplane.hip", "codec.hip", "strategy.hip", "quality.hip",
+           "batch.hip"]
+HEADERS = ["common.hpp", "batch.hpp"]
 
 # C++ API (prism::Compressor & co.) and the `prism` command line, host code
 # over the C ABI

@@ -1,0 +1,304 @@
+// batch.hip -- planning and running a batch of fields (host side).
+//
+// A plan lays out every field's scratch in one caller-provided workspace,
+// builds the per-stage prefix tables that let one launch cover all fields,
+// uploads the descriptors once and (optionally) captures the fixed launch
+// sequence in a HIP graph.  Fields are ordered float32 first so that each
+// dtype-specialised kernel covers one contiguous range of fields; results
+// are reported in the caller's order.
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "batch.hpp"
+#include "common.hpp"
+
+struct pa_batch_plan {
+    int nf = 0, nf32 = 0;
+    std::vector<int> order;  // order[k] = caller index of descriptor k
+    std::vector<pa::BatchField> host;
+    pa::BatchDev dev{};
+    uint8_t* table = nullptr;  // descriptors and prefixes inside the workspace
+    int launches = 0;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+using pa::BatchField;
+
+constexpr size_t kAlign = 256;
+size_t up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+// per-field scratch, in the order the plan lays it out
+struct FieldSizes {
+    size_t minmax, ebinfo, flags, scores, tune, qc, anchors, bp, offsets, staging, cum, outsize;
+    size_t total() const {
+        return minmax + ebinfo + flags + scores + tune + qc + anchors + bp + offsets + staging + cum + outsize;
+    }
+};
+
+FieldSizes field_sizes(const BatchField& f) {
+    const pa_layout& L = f.L;
+    FieldSizes z;
+    z.minmax = up(4 * sizeof(double));
+    z.ebinfo = up(10 * sizeof(double) + 8 * sizeof(float));
+    z.flags = up(8);
+    z.scores = up(15 * sizeof(double));
+    z.tune = up((size_t)f.tnblocks * 15 * sizeof(double));
+    z.qc = up((size_t)L.code_len * 4);
+    z.anchors = up((size_t)L.n4 * f.dtype);
+    z.bp = up((size_t)L.host_size);
+    z.offsets = up((size_t)L.nchunks * 8);
+    z.staging = up((size_t)L.nchunks * pa::kCS + 16);
+    z.cum = up(128 * sizeof(uint64_t));
+    z.outsize = up(sizeof(uint64_t));
+    return z;
+}
+
+constexpr int kPrefixes = 7;  // ext, tune, gap, tile, shuf, chunk, merge
+
+size_t table_bytes(int nf) { return up((size_t)nf * sizeof(BatchField)) + up((size_t)kPrefixes * (nf + 1) * 8); }
+
+// descriptor of one caller field: layout and host-side set-up (no pointers)
+int describe(const pa_batch_field& in, BatchField& f) {
+    if (!in.data || !in.out || (in.dtype != PA_F32 && in.dtype != PA_F64) ||
+        (in.mode != PA_MODE_NB && in.mode != PA_MODE_SM) || !(in.eb > 0))
+        return PA_ERR_ARG;
+    if (((uintptr_t)in.data & 15) != 0) return PA_ERR_ARG;  // 16-byte loads in the extrema pass
+    f = BatchField{};
+    f.data = in.data;
+    f.dtype = in.dtype;
+    f.mode = in.mode;
+    f.rel = in.rel ? 1 : 0;
+    f.eb = in.eb;
+    f.out = in.out;
+    const int st = pa_layout_init(&f.L, in.nx, in.ny, in.nz);
+    if (st != PA_OK) return st;
+    pa::batch_predict_setup(f);
+    return PA_OK;
+}
+
+// field order: float32 first, each dtype in caller order
+int plan_order(const pa_batch_field* fields, int nf, std::vector<int>& order) {
+    order.clear();
+    for (int pass = 0; pass < 2; ++pass)
+        for (int i = 0; i < nf; ++i)
+            if ((fields[i].dtype == PA_F32) == (pass == 0)) order.push_back(i);
+    int nf32 = 0;
+    for (int i = 0; i < nf; ++i) nf32 += fields[i].dtype == PA_F32;
+    return nf32;
+}
+
+int enqueue(const pa_batch_plan* p, hipStream_t st) {
+    int e;
+    if ((e = pa::batch_extrema(p->dev, st)) != PA_OK) return e;
+    if ((e = pa::batch_predict(p->dev, st)) != PA_OK) return e;
+    if ((e = pa::batch_shuffle(p->dev, st)) != PA_OK) return e;
+    return pa::batch_encode(p->dev, st);
+}
+
+int count_launches(const pa::BatchDev& b) {
+    return 2 + (b.n_ext > 0)                                                      // extrema
+           + 2 + (b.n_tune32 > 0) + (b.n_tune > b.n_tune32) + (b.n_gap > 0)       // bound, tuner, pick, gaps
+           + (b.n_tile32 > 0) + (b.n_tile > b.n_tile32)                           // spline
+           + (b.n_shuf > 0)                                                       // shuffle
+           + 2 + 2 * (b.n_chunk > 0);                                             // encode, scan, gather, trailer
+}
+
+// descriptors (with device pointers) of every field, in plan order; fails
+// when a field differs in shape, dtype, mode or bound kind from `ref`
+int build_descriptors(const pa_batch_field* fields, int nf, const std::vector<int>& order, uint8_t* ws,
+                      uint64_t* d_result, std::vector<BatchField>& out, const std::vector<BatchField>* ref) {
+    out.resize(nf);
+    size_t off = 0;
+    for (int k = 0; k < nf; ++k) {
+        const int i = order[k];
+        BatchField& f = out[k];
+        const int st = describe(fields[i], f);
+        if (st != PA_OK) return st;
+        if (ref) {
+            const BatchField& r = (*ref)[k];
+            if (r.dtype != f.dtype || r.mode != f.mode || r.rel != f.rel || r.L.nx != f.L.nx || r.L.ny != f.L.ny ||
+                r.L.nz != f.L.nz)
+                return PA_ERR_ARG;
+        }
+        const FieldSizes z = field_sizes(f);
+        uint8_t* b = ws + off;
+        f.minmax = reinterpret_cast<double*>(b);
+        b += z.minmax;
+        f.ebinfo = reinterpret_cast<double*>(b);
+        b += z.ebinfo;
+        f.flags = b;
+        b += z.flags;
+        f.scores = reinterpret_cast<double*>(b);
+        b += z.scores;
+        f.tune_part = reinterpret_cast<double*>(b);
+        b += z.tune;
+        f.qc = reinterpret_cast<int32_t*>(b);
+        b += z.qc;
+        f.anchors = b;
+        b += z.anchors;
+        f.bp = b;
+        b += z.bp;
+        f.offsets = reinterpret_cast<int64_t*>(b);
+        b += z.offsets;
+        f.staging = b;
+        b += z.staging;
+        f.cum = reinterpret_cast<uint64_t*>(b);
+        b += z.cum;
+        f.outsize = reinterpret_cast<uint64_t*>(b);
+        off += z.total();
+        f.result = d_result + 4 * (size_t)i;
+    }
+    return PA_OK;
+}
+
+size_t scratch_bytes(const std::vector<BatchField>& fs) {
+    size_t s = 0;
+    for (const BatchField& f : fs) s += field_sizes(f).total();
+    return s;
+}
+
+int upload(pa_batch_plan* p) {
+    const int nf = p->nf;
+    std::vector<int64_t> pre((size_t)kPrefixes * (nf + 1), 0);
+    int64_t* P[kPrefixes];
+    for (int s = 0; s < kPrefixes; ++s) P[s] = pre.data() + (size_t)s * (nf + 1);
+    for (int k = 0; k < nf; ++k) {
+        const BatchField& f = p->host[k];
+        const int64_t cnt[kPrefixes] = {f.rel ? pa::batch_ext_blocks(&f.L, f.dtype) : 0,
+                                        f.tnblocks,
+                                        pa::batch_gap_blocks(f),
+                                        (int64_t)f.L.cnt[3],  // tiles = anchor-grid points
+                                        pa::batch_shuf_blocks(&f.L),
+                                        f.L.nchunks,
+                                        pa::batch_merge_blocks(&f.L, f.dtype)};
+        for (int s = 0; s < kPrefixes; ++s) P[s][k + 1] = P[s][k] + cnt[s];
+    }
+    pa::BatchDev& d = p->dev;
+    const size_t desc = up((size_t)nf * sizeof(BatchField));
+    d.fields = reinterpret_cast<BatchField*>(p->table);
+    d.nf = nf;
+    d.nf32 = p->nf32;
+    const int64_t* dp = reinterpret_cast<const int64_t*>(p->table + desc);
+    d.p_ext = dp;
+    d.p_tune = dp + (nf + 1);
+    d.p_gap = dp + 2 * (nf + 1);
+    d.p_tile = dp + 3 * (nf + 1);
+    d.p_shuf = dp + 4 * (nf + 1);
+    d.p_chunk = dp + 5 * (nf + 1);
+    d.p_merge = dp + 6 * (nf + 1);
+    d.n_ext = P[0][nf];
+    d.n_tune = P[1][nf];
+    d.n_gap = P[2][nf];
+    d.n_tile = P[3][nf];
+    d.n_shuf = P[4][nf];
+    d.n_chunk = P[5][nf];
+    d.n_merge = P[6][nf];
+    d.n_tune32 = P[1][p->nf32];
+    d.n_tile32 = P[3][p->nf32];
+    const int64_t lim = 0x7fffffff;
+    if (d.n_ext > lim || d.n_tune > lim || d.n_gap > lim || d.n_tile > lim || d.n_shuf > lim ||
+        (d.n_chunk + 3) / 4 > lim || d.n_merge > lim)
+        return PA_ERR_ARG;
+    if (hipMemcpy(p->table, p->host.data(), (size_t)nf * sizeof(BatchField), hipMemcpyHostToDevice) != hipSuccess)
+        return PA_ERR_HIP;
+    if (hipMemcpy(p->table + desc, pre.data(), pre.size() * 8, hipMemcpyHostToDevice) != hipSuccess) return PA_ERR_HIP;
+    p->launches = count_launches(d);
+    return PA_OK;
+}
+
+int capture(pa_batch_plan* p) {
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return PA_ERR_HIP;
+    int st = PA_OK;
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed) != hipSuccess) {
+        st = PA_ERR_HIP;
+    } else {
+        st = enqueue(p, s);
+        const hipError_t ec = hipStreamEndCapture(s, &g);
+        if (st == PA_OK && ec != hipSuccess) st = PA_ERR_HIP;
+    }
+    if (st == PA_OK && hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        p->exec = nullptr;
+        st = PA_ERR_HIP;
+    }
+    if (g) hipGraphDestroy(g);
+    hipStreamDestroy(s);
+    return st;
+}
+
+}  // namespace
+
+extern "C" size_t pa_batch_workspace(const pa_batch_field* fields, int nf) {
+    if (!fields || nf <= 0) return 0;
+    size_t s = table_bytes(nf);
+    for (int i = 0; i < nf; ++i) {
+        BatchField f;
+        if (describe(fields[i], f) != PA_OK) return 0;
+        s += field_sizes(f).total();
+    }
+    return s;
+}
+
+extern "C" int pa_batch_plan_create(const pa_batch_field* fields, int nf, void* d_ws, size_t ws_bytes,
+                                    uint64_t* d_result, int use_graph, pa_batch_plan** plan) {
+    if (!fields || nf <= 0 || !d_result || !plan) return PA_ERR_ARG;
+    *plan = nullptr;
+    const size_t need = pa_batch_workspace(fields, nf);
+    if (need == 0) return PA_ERR_ARG;
+    if (!d_ws || ws_bytes < need) return PA_ERR_WORKSPACE;
+    pa_batch_plan* p = new (std::nothrow) pa_batch_plan;
+    if (!p) return PA_ERR_ARG;
+    p->nf = nf;
+    p->nf32 = plan_order(fields, nf, p->order);
+    uint8_t* ws = static_cast<uint8_t*>(d_ws);
+    int st = build_descriptors(fields, nf, p->order, ws + table_bytes(nf), d_result, p->host, nullptr);
+    if (st == PA_OK) {
+        p->table = ws;
+        st = upload(p);
+    }
+    if (st == PA_OK && use_graph) st = capture(p);
+    if (st != PA_OK) {
+        delete p;
+        return st;
+    }
+    *plan = p;
+    return PA_OK;
+}
+
+extern "C" int pa_batch_plan_rebind(pa_batch_plan* p, const pa_batch_field* fields, int nf) {
+    if (!p || !fields || nf != p->nf) return PA_ERR_ARG;
+    std::vector<int> order;
+    plan_order(fields, nf, order);
+    if (order != p->order) return PA_ERR_ARG;
+    // the descriptors keep their scratch: same shapes give the same offsets
+    std::vector<BatchField> fs;
+    uint8_t* scratch = reinterpret_cast<uint8_t*>(p->host[0].minmax);
+    uint64_t* d_result = p->host[0].result - 4 * (size_t)p->order[0];
+    const int st = build_descriptors(fields, nf, order, scratch, d_result, fs, &p->host);
+    if (st != PA_OK) return st;
+    p->host.swap(fs);
+    if (hipMemcpy(p->table, p->host.data(), (size_t)nf * sizeof(BatchField), hipMemcpyHostToDevice) != hipSuccess)
+        return PA_ERR_HIP;
+    return PA_OK;
+}
+
+extern "C" int pa_batch_run(pa_batch_plan* p, void* stream) {
+    if (!p) return PA_ERR_ARG;
+    hipStream_t st = pa::as_stream(stream);
+    if (p->exec) return hipGraphLaunch(p->exec, st) == hipSuccess ? PA_OK : PA_ERR_HIP;
+    return enqueue(p, st);
+}
+
+extern "C" int pa_batch_plan_launches(const pa_batch_plan* p) { return p ? p->launches : 0; }
+
+extern "C" int pa_batch_plan_destroy(pa_batch_plan* p) {
+    if (!p) return PA_OK;
+    int st = PA_OK;
+    if (p->exec && hipGraphExecDestroy(p->exec) != hipSuccess) st = PA_ERR_HIP;
+    delete p;
+    return st;
+}

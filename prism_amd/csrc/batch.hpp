@@ -1,0 +1,43 @@
+// batch.hpp -- device tables of a batch of fields and the stage launchers
+// that the batched compression (batch.cpp) calls.  Every launcher issues a
+// fixed number of kernels whatever the number of fields; a workgroup finds
+// its field through the stage's prefix of per-field work counts.
+#pragma once
+
+#include "common.hpp"
+
+namespace pa {
+
+// Fields [0, nf32) hold float32 data and [nf32, nf) float64, so each
+// dtype-specialised kernel covers one contiguous range of every prefix.
+struct BatchDev {
+    BatchField* fields;  // device array of nf descriptors
+    int nf, nf32;
+    const int64_t* p_ext;    // extrema workgroups
+    const int64_t* p_tune;   // tuner sample blocks (x 15 candidates in grid.y)
+    const int64_t* p_gap;    // code-gap workgroups
+    const int64_t* p_tile;   // spline tiles
+    const int64_t* p_shuf;   // shuffle workgroups
+    const int64_t* p_chunk;  // codec chunks (one per wave64)
+    const int64_t* p_merge;  // trailer workgroups
+    // host copies of the totals, and of the f32 part where a kernel is
+    // specialised by dtype
+    int64_t n_ext, n_tune, n_gap, n_tile, n_shuf, n_chunk, n_merge;
+    int64_t n_tune32, n_tile32;
+};
+
+// per-field work counts of each stage (host)
+int64_t batch_ext_blocks(const pa_layout* L, int dtype);
+int64_t batch_gap_blocks(const BatchField& f);
+int64_t batch_shuf_blocks(const pa_layout* L);
+int64_t batch_merge_blocks(const pa_layout* L, int dtype);
+// host-side preparation of a descriptor (layout filled): tuner sampling,
+// alpha, code gaps
+void batch_predict_setup(BatchField& f);
+
+int batch_extrema(const BatchDev& b, hipStream_t st);  // extrema.hip
+int batch_predict(const BatchDev& b, hipStream_t st);  // predictor.hip: eb, tuner, code gaps, spline
+int batch_shuffle(const BatchDev& b, hipStream_t st);  // bitplane.hip
+int batch_encode(const BatchDev& b, hipStream_t st);   // codec.hip: encode, scan, gather, trailer
+
+}  // namespace pa

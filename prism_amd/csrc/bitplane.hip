@@ -11,6 +11,7 @@
 // stores (a wave covers 256 contiguous bytes of a row).  Codes are staged
 // through LDS so the global reads are coalesced as well.  A workgroup of 256
 // lanes covers 1024 groups (8192 codes) of one level.
+#include "batch.hpp"
 #include "common.hpp"
 
 namespace pa {
@@ -50,7 +51,7 @@ struct Levels {
     int64_t dev_size, host_size;
 };
 
-Levels make_levels(const pa_layout* L) {
+__host__ __device__ Levels make_levels(const pa_layout* L) {
     Levels v;
     const int64_t cnt[4] = {L->n - L->cnt[0], L->cnt[0] - L->cnt[1], L->cnt[1] - L->cnt[2], L->cnt[2] - L->n4};
     int b = 0;
@@ -77,12 +78,11 @@ __device__ __forceinline__ int block_level(const Levels& v, int blk, int* first)
 }
 
 template <int MODE>
-__global__ __launch_bounds__(kThreads) void shuffle_kernel(const int32_t* __restrict__ qc, Levels v,
-                                                           uint8_t* __restrict__ bp) {
-    __shared__ int32_t stage[kThreads * kRow];
+__device__ __forceinline__ void shuffle_block(const int32_t* __restrict__ qc, const Levels& v, uint8_t* __restrict__ bp,
+                                              int blk, int nblk, int32_t* stage) {
     int first;
-    const int lv = block_level(v, blockIdx.x, &first);
-    const int64_t g0 = (int64_t)(blockIdx.x - first) * kGroupsPerBlock;
+    const int lv = block_level(v, blk, &first);
+    const int64_t g0 = (int64_t)(blk - first) * kGroupsPerBlock;
     const int64_t c0 = 8 * g0;  // first code of the block within the level
     const int64_t real = v.real_cnt[lv];
     const int32_t* src = qc + v.prefix[lv];
@@ -113,10 +113,29 @@ __global__ __launch_bounds__(kThreads) void shuffle_kernel(const int32_t* __rest
 #pragma unroll
         for (int k = 0; k < 32; ++k) *reinterpret_cast<uint32_t*>(row + k * v.stride[lv]) = a[31 - k];
     }
-    if (blockIdx.x == gridDim.x - 1) {  // bytes between the device and host layouts
+    if (blk == nblk - 1) {  // bytes between the device and host layouts
         for (int64_t i = v.dev_size + 4 * threadIdx.x; i < v.host_size; i += 4 * kThreads)
             *reinterpret_cast<uint32_t*>(bp + i) = 0u;
     }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void shuffle_kernel(const int32_t* __restrict__ qc, Levels v,
+                                                           uint8_t* __restrict__ bp) {
+    __shared__ int32_t stage[kThreads * kRow];
+    shuffle_block<MODE>(qc, v, bp, blockIdx.x, gridDim.x, stage);
+}
+
+// one launch over the shuffle workgroups of every field of a batch
+__global__ __launch_bounds__(kThreads) void batch_shuffle_kernel(const BatchField* __restrict__ fields,
+                                                                 const int64_t* __restrict__ prefix, int nf) {
+    __shared__ int32_t stage[kThreads * kRow];
+    const int f = find_field(prefix, 0, nf, blockIdx.x);
+    const BatchField& F = fields[f];
+    const Levels v = make_levels(&F.L);
+    const int blk = (int)(blockIdx.x - prefix[f]), nblk = (int)(prefix[f + 1] - prefix[f]);
+    if (F.mode == PA_MODE_NB) shuffle_block<PA_MODE_NB>(F.qc, v, F.bp, blk, nblk, stage);
+    else shuffle_block<PA_MODE_SM>(F.qc, v, F.bp, blk, nblk, stage);
 }
 
 // Inverse; WINDOW restricts each level to planes [begin[lr], end[lr]) (plus
@@ -209,6 +228,15 @@ extern "C" int pa_inverse_bitshuffle_window(const uint8_t* d_bp, const pa_layout
     else
         hipLaunchKernelGGL((unshuffle_kernel<PA_MODE_SM, true>), dim3(v.blk_start[4]), dim3(kThreads), 0, st,
                            d_bp, v, d_begin, d_end, d_qc);
+    PA_LAUNCH_CHECK();
+    return PA_OK;
+}
+
+int64_t pa::batch_shuf_blocks(const pa_layout* L) { return make_levels(L).blk_start[4]; }
+
+int pa::batch_shuffle(const BatchDev& b, hipStream_t st) {
+    if (b.n_shuf <= 0) return PA_OK;
+    hipLaunchKernelGGL(batch_shuffle_kernel, dim3((unsigned)b.n_shuf), dim3(kThreads), 0, st, b.fields, b.p_shuf, b.nf);
     PA_LAUNCH_CHECK();
     return PA_OK;
 }

@@ -12,6 +12,7 @@
 // [non-zero bytes][RE1 bytes][RE2 bytes][RE3 bytes][B3: 4 bytes][size: 2 B LE];
 // bitmaps LSB-first; an all-zero chunk is the 2-byte footer alone; a chunk
 // that does not shrink below its size is stored raw (csize == osize).
+#include "batch.hpp"
 #include "common.hpp"
 
 namespace pa {
@@ -30,7 +31,7 @@ struct ChunkGeom {
     int64_t nchunks, host_size;
 };
 
-ChunkGeom make_chunk_geom(const pa_layout* L) {
+__host__ __device__ ChunkGeom make_chunk_geom(const pa_layout* L) {
     ChunkGeom c;
     int64_t id = 0;
     for (int lv = 3, i = 0; lv >= 0; --lv, ++i) {
@@ -131,15 +132,11 @@ static_assert(sizeof(WaveSlot) % 16 == 0, "slots must keep 16-byte alignment");
 // One chunk per wave64.  Zero elimination streams the chunk in 16 steps of
 // 1 KiB (lane l owns 16-byte segment 64k+l, so every load is coalesced);
 // the three repetition-elimination rounds work on registers and shuffles.
-__global__ __launch_bounds__(64 * kWaves) void encode_kernel(const uint8_t* __restrict__ bp, ChunkGeom g,
-                                                             uint8_t* __restrict__ dst_stream,
-                                                             uint8_t* __restrict__ staging) {
-    __shared__ WaveSlot slots[kWaves];
+__device__ __forceinline__ void encode_chunk(const uint8_t* __restrict__ bp, const ChunkGeom& g,
+                                             uint8_t* __restrict__ dst_stream, uint8_t* __restrict__ staging,
+                                             int64_t id, WaveSlot& W) {
     const int lane = threadIdx.x & 63;
-    WaveSlot& W = slots[threadIdx.x >> 6];
     uint8_t* out = reinterpret_cast<uint8_t*>(W.out);
-    const int64_t id = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
-    if (id >= g.nchunks) return;
     const ChunkPos cp = chunk_pos(g, id);
     const int osize = cp.osize;
     const uint8_t* src = bp + cp.base;
@@ -254,12 +251,34 @@ __global__ __launch_bounds__(64 * kWaves) void encode_kernel(const uint8_t* __re
     if (lane == 0) reinterpret_cast<uint16_t*>(dst_stream + 8)[id] = (uint16_t)csize;
 }
 
+__global__ __launch_bounds__(64 * kWaves) void encode_kernel(const uint8_t* __restrict__ bp, ChunkGeom g,
+                                                             uint8_t* __restrict__ dst_stream,
+                                                             uint8_t* __restrict__ staging) {
+    __shared__ WaveSlot slots[kWaves];
+    const int64_t id = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if (id >= g.nchunks) return;
+    encode_chunk(bp, g, dst_stream, staging, id, slots[threadIdx.x >> 6]);
+}
+
+// one launch over the chunks of every field of a batch (one chunk per wave)
+__global__ __launch_bounds__(64 * kWaves) void batch_encode_kernel(const BatchField* __restrict__ fields,
+                                                                   const int64_t* __restrict__ prefix, int nf,
+                                                                   int64_t total) {
+    __shared__ WaveSlot slots[kWaves];
+    const int64_t gid = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if (gid >= total) return;
+    const int f = find_field(prefix, 0, nf, gid);
+    const BatchField& F = fields[f];
+    const ChunkGeom g = make_chunk_geom(&F.L);
+    encode_chunk(F.bp, g, F.out, F.staging, gid - prefix[f], slots[threadIdx.x >> 6]);
+}
+
 // Payload offsets by one workgroup scan of the size table, then the stream
 // header, the stream length and the cumulative size table by (level, plane)
 // (zbpc.cu:176-181, 231-262).
-__global__ __launch_bounds__(1024) void stream_scan_kernel(const uint16_t* __restrict__ sizes, ChunkGeom g,
-                                                           int64_t* __restrict__ offsets, uint8_t* __restrict__ dst_stream,
-                                                           uint64_t* __restrict__ cum, uint64_t* __restrict__ outsize) {
+__device__ __forceinline__ void stream_scan(const uint16_t* __restrict__ sizes, const ChunkGeom& g,
+                                            int64_t* __restrict__ offsets, uint8_t* __restrict__ dst_stream,
+                                            uint64_t* __restrict__ cum, uint64_t* __restrict__ outsize) {
     __shared__ int64_t s_w[16];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t n = g.nchunks, per = (n + 1023) / 1024;
@@ -288,15 +307,26 @@ __global__ __launch_bounds__(1024) void stream_scan_kernel(const uint16_t* __res
     }
 }
 
+__global__ __launch_bounds__(1024) void stream_scan_kernel(const uint16_t* __restrict__ sizes, ChunkGeom g,
+                                                           int64_t* __restrict__ offsets, uint8_t* __restrict__ dst_stream,
+                                                           uint64_t* __restrict__ cum, uint64_t* __restrict__ outsize) {
+    stream_scan(sizes, g, offsets, dst_stream, cum, outsize);
+}
+
+// one workgroup per field of a batch
+__global__ __launch_bounds__(1024) void batch_scan_kernel(const BatchField* __restrict__ fields) {
+    const BatchField& F = fields[blockIdx.x];
+    const ChunkGeom g = make_chunk_geom(&F.L);
+    stream_scan(reinterpret_cast<const uint16_t*>(F.out + 8), g, F.offsets, F.out, F.cum, F.outsize);
+}
+
 // One chunk per wave64: csize bytes from staging (or from the bit-planes for
 // a raw chunk) to their offset in the stream: head bytes, funnel-shifted
 // words, tail bytes.  No source word past the chunk's own bytes is read.
-__global__ __launch_bounds__(256) void gather_kernel(const uint8_t* __restrict__ bp, const uint8_t* __restrict__ staging,
-                                                     ChunkGeom g, const int64_t* __restrict__ offsets,
-                                                     uint8_t* __restrict__ dst_stream) {
+__device__ __forceinline__ void gather_chunk(const uint8_t* __restrict__ bp, const uint8_t* __restrict__ staging,
+                                             const ChunkGeom& g, const int64_t* __restrict__ offsets,
+                                             uint8_t* __restrict__ dst_stream, int64_t id) {
     const int lane = threadIdx.x & 63;
-    const int64_t id = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (id >= g.nchunks) return;
     const int csize = reinterpret_cast<const uint16_t*>(dst_stream + 8)[id];
     const ChunkPos cp = chunk_pos(g, id);
     const bool raw = csize == cp.osize;
@@ -318,15 +348,35 @@ __global__ __launch_bounds__(256) void gather_kernel(const uint8_t* __restrict__
     if (lane < csize - tail0) d[tail0 + lane] = sb[tail0 + lane];
 }
 
-// compress_merge: zero pad to 8 bytes, then flags, range, size table, anchors.
-__global__ void merge_kernel(uint8_t* __restrict__ out, const uint64_t* __restrict__ outsize,
-                             const uint8_t* __restrict__ flags8, const double* __restrict__ range,
-                             const uint64_t* __restrict__ cum, const uint8_t* __restrict__ anchors,
-                             int64_t anchor_bytes, uint64_t* __restrict__ total) {
+__global__ __launch_bounds__(256) void gather_kernel(const uint8_t* __restrict__ bp, const uint8_t* __restrict__ staging,
+                                                     ChunkGeom g, const int64_t* __restrict__ offsets,
+                                                     uint8_t* __restrict__ dst_stream) {
+    const int64_t id = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (id >= g.nchunks) return;
+    gather_chunk(bp, staging, g, offsets, dst_stream, id);
+}
+
+__global__ __launch_bounds__(256) void batch_gather_kernel(const BatchField* __restrict__ fields,
+                                                           const int64_t* __restrict__ prefix, int nf, int64_t total) {
+    const int64_t gid = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (gid >= total) return;
+    const int f = find_field(prefix, 0, nf, gid);
+    const BatchField& F = fields[f];
+    const ChunkGeom g = make_chunk_geom(&F.L);
+    gather_chunk(F.bp, F.staging, g, F.offsets, F.out, gid - prefix[f]);
+}
+
+// compress_merge body: zero pad to 8 bytes, then flags, range, size table,
+// anchors; workgroup blk of nblk.
+__device__ __forceinline__ void merge_block(uint8_t* __restrict__ out, const uint64_t* __restrict__ outsize,
+                                            const uint8_t* __restrict__ flags8, const double* __restrict__ range,
+                                            const uint64_t* __restrict__ cum, const uint8_t* __restrict__ anchors,
+                                            int64_t anchor_bytes, uint64_t* __restrict__ total, int64_t blk,
+                                            int64_t nblk) {
     const uint64_t raw = *outsize;
     const uint64_t p = (raw + 7) & ~7ull;
     uint8_t* t = out + p;
-    if (blockIdx.x == 0) {
+    if (blk == 0) {
         if (threadIdx.x < p - raw) out[raw + threadIdx.x] = 0;
         if (threadIdx.x < 8) t[threadIdx.x] = flags8[threadIdx.x];
         if (threadIdx.x == 0) {
@@ -338,8 +388,25 @@ __global__ void merge_kernel(uint8_t* __restrict__ out, const uint64_t* __restri
     // anchors: 8-byte multiples (n4 * sizeof(T) with T of 4 bytes may leave 4)
     uint8_t* a = t + 1040;
     const int64_t nw = anchor_bytes >> 2;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nw; i += (int64_t)gridDim.x * blockDim.x)
+    for (int64_t i = blk * blockDim.x + threadIdx.x; i < nw; i += nblk * blockDim.x)
         reinterpret_cast<uint32_t*>(a)[i] = reinterpret_cast<const uint32_t*>(anchors)[i];
+}
+
+// compress_merge: zero pad to 8 bytes, then flags, range, size table, anchors.
+__global__ void merge_kernel(uint8_t* __restrict__ out, const uint64_t* __restrict__ outsize,
+                             const uint8_t* __restrict__ flags8, const double* __restrict__ range,
+                             const uint64_t* __restrict__ cum, const uint8_t* __restrict__ anchors,
+                             int64_t anchor_bytes, uint64_t* __restrict__ total) {
+    merge_block(out, outsize, flags8, range, cum, anchors, anchor_bytes, total, blockIdx.x, gridDim.x);
+}
+
+// trailer of every field of a batch; the range is the one the stream
+// records (1.0 in absolute mode)
+__global__ void batch_merge_kernel(const BatchField* __restrict__ fields, const int64_t* __restrict__ prefix, int nf) {
+    const int f = find_field(prefix, 0, nf, blockIdx.x);
+    const BatchField& F = fields[f];
+    merge_block(F.out, F.outsize, F.flags, F.ebinfo + 1, F.cum, static_cast<const uint8_t*>(F.anchors),
+                F.L.n4 * F.dtype, F.result, blockIdx.x - prefix[f], prefix[f + 1] - prefix[f]);
 }
 
 // exclusive offsets of the payloads from the size table; one workgroup
@@ -621,9 +688,7 @@ extern "C" int pa_compress_merge(uint8_t* d_out, const pa_layout* L, int dtype, 
     if (!L || !d_out || !d_outsize || !d_flags8 || !d_cum || !d_anchors || !d_total) return PA_ERR_ARG;
     if (dtype != PA_F32 && dtype != PA_F64) return PA_ERR_ARG;
     const int64_t ab = L->n4 * dtype;
-    int blocks = (int)((ab / 4 + 255) / 256);
-    if (blocks < 1) blocks = 1;
-    if (blocks > 2048) blocks = 2048;
+    const int blocks = (int)batch_merge_blocks(L, dtype);
     hipLaunchKernelGGL(merge_kernel, dim3(blocks), dim3(256), 0, as_stream(stream), d_out, d_outsize, d_flags8,
                        d_range, d_cum, static_cast<const uint8_t*>(d_anchors), ab, d_total);
     PA_LAUNCH_CHECK();
@@ -682,4 +747,28 @@ extern "C" int pa_lossless_decode_window(const uint8_t* d_in, int64_t in_len, co
     if (!d_begin || !d_end) return PA_ERR_ARG;
     return decode_common(d_in, in_len, L, d_begin, d_end, d_bp, d_bytes_read, true, offsets_ready != 0, d_ws, ws_bytes,
                          as_stream(stream));
+}
+
+int64_t pa::batch_merge_blocks(const pa_layout* L, int dtype) {
+    const int64_t ab = L->n4 * dtype;
+    int64_t blocks = (ab / 4 + 255) / 256;
+    return blocks < 1 ? 1 : (blocks > 2048 ? 2048 : blocks);
+}
+
+int pa::batch_encode(const BatchDev& b, hipStream_t st) {
+    if (b.n_chunk > 0) {
+        hipLaunchKernelGGL(batch_encode_kernel, dim3((unsigned)((b.n_chunk + kWaves - 1) / kWaves)), dim3(64 * kWaves),
+                           0, st, b.fields, b.p_chunk, b.nf, b.n_chunk);
+        PA_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(batch_scan_kernel, dim3((unsigned)b.nf), dim3(1024), 0, st, b.fields);
+    PA_LAUNCH_CHECK();
+    if (b.n_chunk > 0) {
+        hipLaunchKernelGGL(batch_gather_kernel, dim3((unsigned)((b.n_chunk + 3) / 4)), dim3(256), 0, st, b.fields,
+                           b.p_chunk, b.nf, b.n_chunk);
+        PA_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(batch_merge_kernel, dim3((unsigned)b.n_merge), dim3(256), 0, st, b.fields, b.p_merge, b.nf);
+    PA_LAUNCH_CHECK();
+    return PA_OK;
 }

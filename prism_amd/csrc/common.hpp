@@ -33,7 +33,7 @@ struct Geom {
     int64_t prefix[5];           // level-major region starts (absolute)
 };
 
-inline Geom make_geom(const pa_layout* L) {
+__host__ __device__ inline Geom make_geom(const pa_layout* L) {
     Geom g;
     g.nx = (int)L->nx;
     g.ny = (int)L->ny;
@@ -47,6 +47,52 @@ inline Geom make_geom(const pa_layout* L) {
         g.prefix[i] = L->prefix[i];
     }
     return g;
+}
+
+// ------------------------------------------------------------------ batch --
+// One field of a batch: its input, its layout, and the device buffers every
+// stage of the compression reads and writes.  The batched kernels find the
+// field of a workgroup in a per-stage prefix of workgroup counts.
+struct BatchField {
+    const void* data;
+    int dtype;     // PA_F32 / PA_F64
+    int mode;      // PA_MODE_NB / PA_MODE_SM
+    int rel;       // 1: eb is relative to the value range
+    double eb;     // relative or absolute bound
+    double alpha;  // level scaling of the relative bound (alpha_of(rel eb); 1 in absolute mode)
+    pa_layout L;
+    Geom g;  // make_geom(&L)
+    // auto-tuner sampling (host-computed; tune_ok = 0 leaves all flags false)
+    int tune_ok, tsx0, tsy0, tsz0, tnsx, tnsy, tnblocks;
+    // code-array ranges the predictor does not write (zeroed)
+    int64_t gap_lo[5], gap_hi[5];
+    // device buffers
+    double* minmax;     // [min, max] + two 64-bit order keys
+    double* ebinfo;     // eb_abs, range, eb_r[4] f64, ebx2[4] f64, then eb_r[4] | ebx2[4] f32
+    uint8_t* flags;     // 8
+    double* scores;     // 15
+    double* tune_part;  // tnblocks x 15
+    int32_t* qc;        // code_len
+    void* anchors;      // n4 values
+    uint8_t* bp;        // host_size
+    int64_t* offsets;   // nchunks
+    uint8_t* staging;   // nchunks x 16 KiB
+    uint8_t* out;       // pa_compress_bound bytes
+    uint64_t* cum;      // 128
+    uint64_t* outsize;  // unpadded stream length
+    uint64_t* result;   // [total stream bytes, status, range (f64 bits), absolute eb (f64 bits)]
+};
+
+// field of global work item b among fields [lo, hi): the largest f with
+// prefix[f] <= b (fields with no work items share their successor's prefix)
+__device__ __forceinline__ int find_field(const int64_t* prefix, int lo, int hi, int64_t b) {
+    --hi;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
 // Level-major position of a point of level LEVEL (< 4): its rank among the

@@ -5,6 +5,7 @@
 // folded into two 64-bit words with atomicMin / atomicMax on an
 // order-preserving integer image of the double value.  Min and max are
 // exact, so the result does not depend on the reduction order.
+#include "batch.hpp"
 #include "common.hpp"
 
 namespace pa {
@@ -100,6 +101,91 @@ __global__ void extrema_finish_kernel(const unsigned long long* keys, double* ou
     out[1] = key_value(keys[1]);
 }
 
+// ----------------------------------------------------------------- batch --
+constexpr int64_t kExtSpan = 8 * 256;  // vectors per batched extrema workgroup
+
+__global__ void batch_extrema_init_kernel(BatchField* fields, int nf) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    unsigned long long* keys = reinterpret_cast<unsigned long long*>(fields[f].minmax + 2);
+    keys[0] = ~0ull;
+    keys[1] = 0ull;
+}
+
+// Workgroup b of the batch streams kExtSpan 16-byte vectors of its field
+// (the field's tail elements go to its last workgroup); the running keys of
+// each field live in its own descriptor.
+template <typename T>
+__device__ void extrema_span(const T* in, int64_t n, int64_t wg, unsigned long long* keys) {
+    __shared__ double s_min[4], s_max[4];
+    constexpr int V = 16 / sizeof(T);
+    using Vec = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+    const int64_t nv = n / V;
+    const Vec* vin = reinterpret_cast<const Vec*>(in);
+    double mn = __builtin_inf(), mx = -__builtin_inf();
+    const int64_t v0 = wg * kExtSpan;
+    Vec v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int64_t i = v0 + threadIdx.x + k * 256;
+        if (i < nv) v[k] = vin[i];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int64_t i = v0 + threadIdx.x + k * 256;
+        if (i < nv) {
+            const T* e = reinterpret_cast<const T*>(&v[k]);
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                mn = fmin(mn, (double)e[j]);
+                mx = fmax(mx, (double)e[j]);
+            }
+        }
+    }
+    if (v0 + kExtSpan >= nv) {  // last workgroup of the field: scalar tail
+        for (int64_t i = nv * V + threadIdx.x; i < n; i += 256) {
+            mn = fmin(mn, (double)in[i]);
+            mx = fmax(mx, (double)in[i]);
+        }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        mn = fmin(mn, __shfl_xor(mn, d, 64));
+        mx = fmax(mx, __shfl_xor(mx, d, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s_min[threadIdx.x >> 6] = mn;
+        s_max[threadIdx.x >> 6] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) {
+            mn = fmin(mn, s_min[w]);
+            mx = fmax(mx, s_max[w]);
+        }
+        const unsigned long long kmn = order_key(mn), kmx = order_key(mx);
+        if (kmn < __hip_atomic_load(keys, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(keys, kmn);
+        if (kmx > __hip_atomic_load(keys + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(keys + 1, kmx);
+    }
+}
+
+__global__ __launch_bounds__(256) void batch_extrema_kernel(BatchField* fields, const int64_t* prefix, int nf) {
+    const int f = find_field(prefix, 0, nf, blockIdx.x);
+    const BatchField& F = fields[f];
+    unsigned long long* keys = reinterpret_cast<unsigned long long*>(F.minmax + 2);
+    const int64_t wg = blockIdx.x - prefix[f];
+    if (F.dtype == PA_F32) extrema_span<float>((const float*)F.data, F.L.n, wg, keys);
+    else extrema_span<double>((const double*)F.data, F.L.n, wg, keys);
+}
+
+__global__ void batch_extrema_finish_kernel(BatchField* fields, int nf) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    const unsigned long long* keys = reinterpret_cast<const unsigned long long*>(fields[f].minmax + 2);
+    fields[f].minmax[0] = key_value(keys[0]);
+    fields[f].minmax[1] = key_value(keys[1]);
+}
+
 }  // namespace
 }  // namespace pa
 
@@ -129,4 +215,22 @@ extern "C" int pa_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
     return n;
+}
+
+int64_t pa::batch_ext_blocks(const pa_layout* L, int dtype) {
+    const int64_t nv = L->n / (16 / dtype);  // 16-byte vectors of the field
+    return nv / kExtSpan + 1;                // the last workgroup also takes the scalar tail
+}
+
+int pa::batch_extrema(const BatchDev& b, hipStream_t st) {
+    const int fb = (b.nf + 255) / 256;
+    hipLaunchKernelGGL(batch_extrema_init_kernel, dim3(fb), dim3(256), 0, st, b.fields, b.nf);
+    PA_LAUNCH_CHECK();
+    if (b.n_ext > 0) {
+        hipLaunchKernelGGL(batch_extrema_kernel, dim3((unsigned)b.n_ext), dim3(256), 0, st, b.fields, b.p_ext, b.nf);
+        PA_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(batch_extrema_finish_kernel, dim3(fb), dim3(256), 0, st, b.fields, b.nf);
+    PA_LAUNCH_CHECK();
+    return PA_OK;
 }

@@ -12,6 +12,7 @@
 // Reference: InterpolationPredictor.cu -- stage maps 1786-1809 / 23-106,
 // 1-D rule 1229-1352, multi-dimensional rule 1414-1750, level loop
 // 1823-2026, tile staging 997-1170, compaction 2028-2088, auto-tuner 110-967.
+#include "batch.hpp"
 #include "common.hpp"
 
 namespace pa {
@@ -911,23 +912,21 @@ struct Gaps {
     int64_t lo[5], hi[5];
 };
 
-__global__ void zero_gaps_kernel(int32_t* p, Gaps g) {
+__device__ __forceinline__ void zero_gaps(int32_t* p, const int64_t* lo, const int64_t* hi, int64_t blk,
+                                          int64_t nblk) {
     for (int r = 0; r < 5; ++r)
-        for (int64_t i = g.lo[r] + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < g.hi[r];
-             i += (int64_t)gridDim.x * blockDim.x)
-            p[i] = 0;
+        for (int64_t i = lo[r] + blk * blockDim.x + threadIdx.x; i < hi[r]; i += nblk * blockDim.x) p[i] = 0;
 }
+
+__global__ void zero_gaps_kernel(int32_t* p, Gaps g) { zero_gaps(p, g.lo, g.hi, blockIdx.x, gridDim.x); }
 
 // ---------------------------------------------------------------- tuner --
 // Grid (sample block, candidate).  Lane p mod 256 of each stage accumulates
 // |x - pred| in double in visiting order; lanes are added by a halving tree.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void tune_kernel(const T* __restrict__ data, Geom g, int sx0,
-                                                        int sy0, int sz0, int nsx, int nsy,
-                                                        double* __restrict__ part) {
-    __shared__ T tile[kTileVals];
-    __shared__ double red[kThreads];
-    const int bid = blockIdx.x, cand = blockIdx.y;
+__device__ __forceinline__ void tune_block(const T* __restrict__ data, const Geom& g, int sx0, int sy0, int sz0,
+                                           int nsx, int nsy, int bid, int cand, double* __restrict__ part, T* tile,
+                                           double* red) {
     Ctx<T> t;
     t.s = tile;
     t.ox = sx0 + (bid % nsx) * 128;
@@ -967,11 +966,21 @@ __global__ __launch_bounds__(kThreads) void tune_kernel(const T* __restrict__ da
     if (threadIdx.x == 0) part[(int64_t)bid * 15 + cand] = red[0];
 }
 
+// Grid (sample block, candidate).  Lane p mod 256 of each stage accumulates
+// |x - pred| in double in visiting order; lanes are added by a halving tree.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void tune_kernel(const T* __restrict__ data, Geom g, int sx0,
+                                                        int sy0, int sz0, int nsx, int nsy,
+                                                        double* __restrict__ part) {
+    __shared__ T tile[kTileVals];
+    __shared__ double red[kThreads];
+    tune_block<T>(data, g, sx0, sy0, sz0, nsx, nsy, blockIdx.x, blockIdx.y, part, tile, red);
+}
+
 // Sum over sample blocks in block order, then the host's pick rule
 // (InterpolationPredictor.cu:2306-2355) on the device.
-__global__ void tune_pick_kernel(const double* __restrict__ part, int nblocks, double* __restrict__ scores,
-                                 uint8_t* __restrict__ flags) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ void pick_flags(const double* __restrict__ part, int nblocks, double* __restrict__ scores,
+                           uint8_t* __restrict__ flags) {
     double e[15];
     for (int c = 0; c < 15; ++c) {
         double s = 0.0;
@@ -998,14 +1007,20 @@ __global__ void tune_pick_kernel(const double* __restrict__ part, int nblocks, d
     for (int i = 0; i < 8; ++i) flags[i] = f[i];
 }
 
+__global__ void tune_pick_kernel(const double* __restrict__ part, int nblocks, double* __restrict__ scores,
+                                 uint8_t* __restrict__ flags) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    pick_flags(part, nblocks, scores, flags);
+}
+
 __global__ void zero_flags_kernel(double* scores, uint8_t* flags) {
     const int i = threadIdx.x;
     if (i < 8) flags[i] = 0;
     if (scores && i < 15) scores[i] = 0.0;
 }
 
-// ------------------------------------------------------------- host side --
-double alpha_of(double rel_eb) {  // InterpolationPredictor.cu:2248-2271
+// ------------------------------------------------------- error bounds --
+__host__ __device__ double alpha_of(double rel_eb) {  // InterpolationPredictor.cu:2248-2271
     const double a1 = 2.0, a2 = 1.75, a3 = 1.5, a4 = 1.25, a5 = 1;
     const double e1 = 1e-1, e2 = 1e-2, e3 = 1e-3, e4 = 1e-4, e5 = 1e-5;
     if (rel_eb >= e1) return a1;
@@ -1017,11 +1032,14 @@ double alpha_of(double rel_eb) {  // InterpolationPredictor.cu:2248-2271
 }
 
 // calc_eb (InterpolationPredictor.cu:1825-1839): every step rounds to T.
+// Used on the host (plain launches) and on the device (batched launches,
+// where the absolute bound comes from the device-side value range); the
+// steps are single IEEE double operations, correctly rounded on both.
 template <typename T>
-LevelEb<T> level_ebs(double eb, double rel_eb) {
+__host__ __device__ LevelEb<T> level_ebs_alpha(double eb, double alpha) {
     LevelEb<T> le;
     const T ebx2 = (T)(eb * 2), eb_r = (T)(1 / eb);
-    const double alpha = alpha_of(rel_eb), beta = 4.0;
+    const double beta = 4.0;
     for (int lv = 0; lv < 4; ++lv) {
         const int unit = 1 << lv;
         T r = eb_r, x2 = ebx2;
@@ -1039,6 +1057,144 @@ LevelEb<T> level_ebs(double eb, double rel_eb) {
         le.ebx2[lv] = x2;
     }
     return le;
+}
+
+template <typename T>
+LevelEb<T> level_ebs(double eb, double rel_eb) {
+    return level_ebs_alpha<T>(eb, alpha_of(rel_eb));
+}
+
+// ------------------------------------------------------------- batched --
+// Per field: the absolute bound from the value range (REL) and its level
+// bounds; a bound that is not positive (a constant field in REL mode) marks
+// the field failed and the stages run on a placeholder bound of 1.
+__global__ void batch_eb_kernel(BatchField* fields, int nf) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    BatchField& F = fields[f];
+    double range = 1.0, eb = F.eb;
+    if (F.rel) {
+        range = F.minmax[1] - F.minmax[0];
+        eb = F.eb * range;
+    }
+    uint64_t status = PA_OK;
+    if (!(eb > 0)) {
+        status = (uint64_t)(int64_t)PA_ERR_ARG;
+        eb = 1.0;
+    }
+    double* e = F.ebinfo;
+    e[0] = eb;
+    e[1] = range;
+    if (F.dtype == PA_F32) {
+        const LevelEb<float> le = level_ebs_alpha<float>(eb, F.alpha);
+        float* q = reinterpret_cast<float*>(e + 10);
+        for (int i = 0; i < 4; ++i) {
+            q[i] = le.eb_r[i];
+            q[4 + i] = le.ebx2[i];
+        }
+    } else {
+        const LevelEb<double> le = level_ebs_alpha<double>(eb, F.alpha);
+        for (int i = 0; i < 4; ++i) {
+            e[2 + i] = le.eb_r[i];
+            e[6 + i] = le.ebx2[i];
+        }
+    }
+    F.result[1] = status;
+    F.result[2] = (uint64_t)__double_as_longlong(range);
+    F.result[3] = (uint64_t)__double_as_longlong(eb);
+}
+
+template <typename T>
+__device__ __forceinline__ LevelEb<T> load_level_ebs(const double* e) {
+    LevelEb<T> le;
+    if constexpr (sizeof(T) == 4) {
+        const float* q = reinterpret_cast<const float*>(e + 10);
+        for (int i = 0; i < 4; ++i) {
+            le.eb_r[i] = q[i];
+            le.ebx2[i] = q[4 + i];
+        }
+    } else {
+        for (int i = 0; i < 4; ++i) {
+            le.eb_r[i] = e[2 + i];
+            le.ebx2[i] = e[6 + i];
+        }
+    }
+    return le;
+}
+
+// tuner over the sample blocks of fields [lo, hi) (one dtype); grid.y = candidate
+template <typename T>
+__global__ __launch_bounds__(kThreads) void batch_tune_kernel(const BatchField* __restrict__ fields,
+                                                              const int64_t* __restrict__ prefix, int lo, int hi,
+                                                              int64_t base) {
+    __shared__ T tile[kTileVals];
+    __shared__ double red[kThreads];
+    const int64_t gb = base + blockIdx.x;
+    const int f = find_field(prefix, lo, hi, gb);
+    const BatchField& F = fields[f];
+    tune_block<T>(static_cast<const T*>(F.data), F.g, F.tsx0, F.tsy0, F.tsz0, F.tnsx, F.tnsy,
+                  (int)(gb - prefix[f]), blockIdx.y, F.tune_part, tile, red);
+}
+
+__global__ void batch_pick_kernel(BatchField* fields, int nf) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    BatchField& F = fields[f];
+    if (F.tune_ok) {
+        pick_flags(F.tune_part, F.tnblocks, F.scores, F.flags);
+    } else {
+        for (int i = 0; i < 8; ++i) F.flags[i] = 0;
+        for (int i = 0; i < 15; ++i) F.scores[i] = 0.0;
+    }
+}
+
+__global__ void batch_gaps_kernel(const BatchField* __restrict__ fields, const int64_t* __restrict__ prefix, int nf) {
+    const int f = find_field(prefix, 0, nf, blockIdx.x);
+    const BatchField& F = fields[f];
+    zero_gaps(F.qc, F.gap_lo, F.gap_hi, blockIdx.x - prefix[f], prefix[f + 1] - prefix[f]);
+}
+
+// Compression tiles of fields [lo, hi) (one dtype), with the XCD-aware
+// order of spline_kernel taken over the concatenated tile list.
+template <typename T>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void batch_spline_kernel(
+    const BatchField* __restrict__ fields, const int64_t* __restrict__ prefix, int lo, int hi, int64_t base,
+    int64_t ntiles) {
+    __shared__ T tile[kTileVals];
+    __shared__ int32_t rowtab[1];
+    const int64_t b = blockIdx.x, xcd = b % kXcds, per = ntiles / kXcds, extra = ntiles % kXcds;
+    const int64_t gt = base + xcd * per + (xcd < extra ? xcd : extra) + b / kXcds;
+    const int f = find_field(prefix, lo, hi, gt);
+    const BatchField& F = fields[f];
+    const SplineArgs<T> a{static_cast<const T*>(F.data), F.g, load_level_ebs<T>(F.ebinfo), F.flags,
+                          nullptr, F.qc, nullptr, static_cast<T*>(F.anchors), nullptr, nullptr, nullptr};
+    const Geom& g = a.g;
+    const int64_t tix = gt - prefix[f];
+    const int bx = (int)(tix % g.gx), by = (int)((tix / g.gx) % g.gy), bz = (int)(tix / ((int64_t)g.gx * g.gy));
+    const bool interior = bx * 16 + 16 < g.nx && by * 16 + 16 < g.ny && bz * 16 + 16 < g.nz;
+    if (interior) tile_run<T, COMPRESS, true>(a, tile, rowtab, bx, by, bz);
+    else tile_run<T, COMPRESS, false>(a, tile, rowtab, bx, by, bz);
+}
+
+// ------------------------------------------------------------- host side --
+Gaps make_gaps(const pa_layout* L) {
+    Gaps gaps;
+    const int64_t cnt[4] = {L->n - L->cnt[0], L->cnt[0] - L->cnt[1], L->cnt[1] - L->cnt[2], L->cnt[2] - L->n4};
+    gaps.lo[0] = 0;
+    gaps.hi[0] = L->n4;
+    for (int lv = 3; lv >= 1; --lv) {
+        gaps.lo[4 - lv] = L->prefix[lv] + cnt[lv];
+        gaps.hi[4 - lv] = L->prefix[lv - 1];
+    }
+    gaps.lo[4] = L->real_end;
+    gaps.hi[4] = L->code_len;
+    return gaps;
+}
+
+int gap_blocks(const Gaps& gaps) {
+    int64_t most = 0;
+    for (int r = 0; r < 5; ++r) most = gaps.hi[r] - gaps.lo[r] > most ? gaps.hi[r] - gaps.lo[r] : most;
+    return (int)((most + 255) / 256 < 1024 ? (most + 255) / 256 + 1 : 1024);
 }
 
 template <typename T, int M>
@@ -1117,19 +1273,8 @@ extern "C" int pa_spline_construct(const void* d_data, int dtype, const pa_layou
     if (!L || !d_data || !d_flags8 || !d_qc || !d_anchors || !valid_dtype(dtype) || !(eb > 0))
         return PA_ERR_ARG;
     hipStream_t st = as_stream(stream);
-    Gaps gaps;
-    const int64_t cnt[4] = {L->n - L->cnt[0], L->cnt[0] - L->cnt[1], L->cnt[1] - L->cnt[2], L->cnt[2] - L->n4};
-    gaps.lo[0] = 0;
-    gaps.hi[0] = L->n4;
-    for (int lv = 3; lv >= 1; --lv) {
-        gaps.lo[4 - lv] = L->prefix[lv] + cnt[lv];
-        gaps.hi[4 - lv] = L->prefix[lv - 1];
-    }
-    gaps.lo[4] = L->real_end;
-    gaps.hi[4] = L->code_len;
-    int64_t most = 0;
-    for (int r = 0; r < 5; ++r) most = gaps.hi[r] - gaps.lo[r] > most ? gaps.hi[r] - gaps.lo[r] : most;
-    const int blocks = (int)((most + 255) / 256 < 1024 ? (most + 255) / 256 + 1 : 1024);
+    const Gaps gaps = make_gaps(L);
+    const int blocks = gap_blocks(gaps);
     hipLaunchKernelGGL(zero_gaps_kernel, dim3(blocks), dim3(256), 0, st, d_qc, gaps);
     PA_LAUNCH_CHECK();
     if (dtype == PA_F32)
@@ -1161,4 +1306,65 @@ extern "C" int pa_spline_delta_reconstruct(const int32_t* d_qc, int dtype, const
                                                 nullptr, nullptr, (const float*)d_old, (float*)d_out, st);
     return launch_spline<double, DECOMPRESS>(L, nullptr, eb, rel_eb, d_flags8, d_maxlevel, nullptr, d_qc, nullptr,
                                              nullptr, (const double*)d_old, (double*)d_out, st);
+}
+
+// ------------------------------------------------------------- batched --
+int64_t pa::batch_gap_blocks(const BatchField& f) {
+    Gaps g;
+    for (int r = 0; r < 5; ++r) {
+        g.lo[r] = f.gap_lo[r];
+        g.hi[r] = f.gap_hi[r];
+    }
+    return gap_blocks(g);
+}
+
+void pa::batch_predict_setup(BatchField& f) {
+    f.g = make_geom(&f.L);
+    const Gaps g = make_gaps(&f.L);
+    for (int r = 0; r < 5; ++r) {
+        f.gap_lo[r] = g.lo[r];
+        f.gap_hi[r] = g.hi[r];
+    }
+    f.alpha = alpha_of(f.rel ? f.eb : 0.0);
+    int s[3] = {0, 0, 0}, n[3] = {0, 0, 0};
+    f.tune_ok = tune_axis(f.L.nx, &s[0], &n[0]) && tune_axis(f.L.ny, &s[1], &n[1]) && tune_axis(f.L.nz, &s[2], &n[2]);
+    f.tsx0 = s[0];
+    f.tsy0 = s[1];
+    f.tsz0 = s[2];
+    f.tnsx = n[0];
+    f.tnsy = n[1];
+    f.tnblocks = f.tune_ok ? n[0] * n[1] * n[2] : 0;
+}
+
+int pa::batch_predict(const BatchDev& b, hipStream_t st) {
+    const unsigned fb = (unsigned)((b.nf + 63) / 64);
+    hipLaunchKernelGGL(batch_eb_kernel, dim3(fb), dim3(64), 0, st, b.fields, b.nf);
+    PA_LAUNCH_CHECK();
+    if (b.n_tune32 > 0) {
+        hipLaunchKernelGGL(batch_tune_kernel<float>, dim3((unsigned)b.n_tune32, 15), dim3(kThreads), 0, st, b.fields,
+                           b.p_tune, 0, b.nf32, (int64_t)0);
+        PA_LAUNCH_CHECK();
+    }
+    if (b.n_tune - b.n_tune32 > 0) {
+        hipLaunchKernelGGL(batch_tune_kernel<double>, dim3((unsigned)(b.n_tune - b.n_tune32), 15), dim3(kThreads), 0,
+                           st, b.fields, b.p_tune, b.nf32, b.nf, b.n_tune32);
+        PA_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(batch_pick_kernel, dim3(fb), dim3(64), 0, st, b.fields, b.nf);
+    PA_LAUNCH_CHECK();
+    if (b.n_gap > 0) {
+        hipLaunchKernelGGL(batch_gaps_kernel, dim3((unsigned)b.n_gap), dim3(256), 0, st, b.fields, b.p_gap, b.nf);
+        PA_LAUNCH_CHECK();
+    }
+    if (b.n_tile32 > 0) {
+        hipLaunchKernelGGL(batch_spline_kernel<float>, dim3((unsigned)b.n_tile32), dim3(kThreads), 0, st, b.fields,
+                           b.p_tile, 0, b.nf32, (int64_t)0, b.n_tile32);
+        PA_LAUNCH_CHECK();
+    }
+    if (b.n_tile - b.n_tile32 > 0) {
+        hipLaunchKernelGGL(batch_spline_kernel<double>, dim3((unsigned)(b.n_tile - b.n_tile32)), dim3(kThreads), 0, st,
+                           b.fields, b.p_tile, b.nf32, b.nf, b.n_tile32, b.n_tile - b.n_tile32);
+        PA_LAUNCH_CHECK();
+    }
+    return PA_OK;
 }

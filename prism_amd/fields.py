@@ -173,3 +173,33 @@ def modes(nx, ny, nz, seed, dtype=torch.float32, device="cuda", count=64):
         f += kk ** (-5.0 / 6.0) * torch.cos(2 * math.pi * (k[0] * x / nx + k[1] * y / ny + k[2] * z / nz) + ph)
     f = (f - f.mean()) / f.std()
     return f.to(dtype)
+
+
+def batch_spec(i: int, seed: int, max3d: int = 256, max2d: int = 2048, max1d: float = 1e7):
+    """Shape, dtype, generator and relative bound of field i of the mixed
+    batch: float32 for even i; by i mod 4, 3-D (axes U{32..max3d}), 2-D (axes
+    U{64..max2d}) or 1-D (n log-uniform in [1e5, max1d]); the density
+    generator for i mod 8 < 4, else the mode spectrum; rel eb log-uniform in
+    [1e-6, 1e-2].  The draws come from the field's own seed."""
+    g = Xoshiro(seed)
+    dtype = torch.float32 if i % 2 == 0 else torch.float64
+    cls = i % 4
+    if cls <= 1:
+        shape = (g.randint(32, max3d), g.randint(32, max3d), g.randint(32, max3d))
+    elif cls == 2:
+        shape = (g.randint(64, max2d), g.randint(64, max2d), 1)
+    else:
+        shape = (int(round(10 ** (5 + g.uniform() * (math.log10(max1d) - 5)))), 1, 1)
+    rel_eb = 10 ** (-6 + 4 * g.uniform())
+    gen = "density" if i % 8 < 4 else "modes"
+    return shape, dtype, gen, rel_eb
+
+
+def batch_field(i: int, seed: int, device="cuda", **limits):
+    """(tensor [nz][ny][nx], rel_eb) of field i of the mixed batch."""
+    (nx, ny, nz), dtype, gen, rel_eb = batch_spec(i, seed, **limits)
+    if gen == "density":
+        t = density(nx, ny, nz, seed=seed, dtype=dtype, device=device)
+    else:
+        t = modes(nx, ny, nz, seed=seed, dtype=dtype, device=device)
+    return t, rel_eb

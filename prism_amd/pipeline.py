@@ -198,6 +198,106 @@ def compress(data: torch.Tensor, eb: float, rel: bool = True, progressive: bool 
     return c
 
 
+class BatchCompressor:
+    """Compresses a set of device fields of mixed shapes and dtypes with one
+    fixed launch sequence (one launch per stage for all fields, captured in
+    a HIP graph when `graph`); each stream equals compress() of that field.
+
+    The plan is built for the shapes, dtypes, bound kinds and modes of
+    `fields`; run() accepts new tensors and bounds of the same kinds.  The
+    returned streams are views of a buffer that the next run() reuses."""
+
+    def __init__(self, fields, ebs, rel: bool = True, progressive: bool = True, graph: bool = True):
+        if not fields:
+            raise ValueError("empty batch")
+        self.rel, self.progressive = rel, progressive
+        self.mode = PA_MODE_NB if progressive else PA_MODE_SM
+        dev = fields[0].device
+        self.nf = len(fields)
+        self.shapes, self.dtypes = [], []
+        offs, tot = [], 0
+        for t in fields:
+            if not t.is_cuda or t.dtype not in DTYPES:
+                raise ValueError("fields must be float32/float64 CUDA tensors")
+            nx, ny, nz = shape_xyz(t)
+            L = layout(nx, ny, nz)
+            self.shapes.append((nx, ny, nz))
+            self.dtypes.append(t.dtype)
+            offs.append(tot)
+            tot += (lib().pa_compress_bound(C.byref(L), DTYPES[t.dtype]) + 255) // 256 * 256
+        self.offs = offs
+        self.outbuf = torch.empty(max(tot, 1), dtype=torch.uint8, device=dev)
+        self.result = torch.zeros(4 * self.nf, dtype=torch.int64, device=dev)
+        self._keep = None
+        desc = self._descs(fields, ebs)
+        wsb = lib().pa_batch_workspace(desc, self.nf)
+        if wsb == 0:
+            raise _lib.PrismError("pa_batch_workspace: invalid field in the batch")
+        self.ws = _empty(wsb)
+        plan = C.c_void_p()
+        check(lib().pa_batch_plan_create(desc, self.nf, _p(self.ws), self.ws.numel(), _p(self.result), int(graph),
+                                         C.byref(plan)), "pa_batch_plan_create")
+        self.plan = plan
+        self.graph = graph
+
+    def _descs(self, fields, ebs):
+        if len(fields) != self.nf:
+            raise ValueError("batch size differs from the plan")
+        ebs = list(ebs) if hasattr(ebs, "__len__") else [ebs] * self.nf
+        arr = (_lib.BatchFieldDesc * self.nf)()
+        keep = []
+        for i, t in enumerate(fields):
+            t = t.contiguous()
+            keep.append(t)
+            nx, ny, nz = shape_xyz(t)
+            d = arr[i]
+            d.data, d.dtype, d.mode, d.rel = t.data_ptr(), DTYPES[t.dtype], self.mode, int(self.rel)
+            d.eb, d.nx, d.ny, d.nz = float(ebs[i]), nx, ny, nz
+            d.out = self.outbuf.data_ptr() + self.offs[i]
+        self._keep = keep  # inputs stay alive while the plan points at them
+        self._ebs = ebs
+        return arr
+
+    @property
+    def launches(self) -> int:
+        return int(lib().pa_batch_plan_launches(self.plan))
+
+    def enqueue(self, fields=None, ebs=None):
+        """Rebinds (when given new inputs) and launches; no synchronisation."""
+        if fields is not None:
+            desc = self._descs(fields, ebs if ebs is not None else self._ebs)
+            check(lib().pa_batch_plan_rebind(self.plan, desc, self.nf), "pa_batch_plan_rebind")
+        check(lib().pa_batch_run(self.plan, _stream()), "pa_batch_run")
+
+    def results(self):
+        """Per-field Compressed streams of the last run (one synchronisation)."""
+        r = self.result.cpu().view(-1, 4)
+        out = []
+        for i in range(self.nf):
+            total, status = int(r[i, 0]), int(r[i, 1])
+            if status != 0:
+                raise _lib.PrismError(f"batch field {i}: non-positive error bound (constant field in REL mode?)")
+            rng = float(r[i, 2:3].view(torch.float64).item())
+            s = self.outbuf[self.offs[i]:self.offs[i] + total]
+            out.append(Compressed(stream=s, shape=self.shapes[i], dtype=self.dtypes[i], eb_input=float(self._ebs[i]),
+                                  rel=self.rel, progressive=self.progressive, range=rng if self.rel else 1.0))
+        return out
+
+    def run(self, fields=None, ebs=None):
+        self.enqueue(fields, ebs)
+        return self.results()
+
+    def __del__(self):
+        plan = getattr(self, "plan", None)
+        if plan:
+            try:
+                torch.cuda.synchronize()  # no run of the plan may be in flight
+                lib().pa_batch_plan_destroy(plan)
+            except Exception:
+                pass
+            self.plan = None
+
+
 def _trailer(c: Compressed, L):
     item = 4 if c.dtype == torch.float32 else 8
     tsize = L.trailer_fixed + L.n4 * item
