@@ -170,7 +170,7 @@ int upload(pa_batch_plan* p) {
         const int64_t cnt[kPrefixes] = {f.rel ? pa::batch_ext_blocks(&f.L, f.dtype) : 0,
                                         f.tnblocks,
                                         pa::batch_gap_blocks(f),
-                                        (int64_t)f.L.cnt[3],  // tiles = anchor-grid points
+                                        pa::batch_tile_units(&f.L),
                                         pa::batch_shuf_blocks(&f.L),
                                         f.L.nchunks,
                                         pa::batch_merge_blocks(&f.L, f.dtype)};

@@ -16,7 +16,7 @@ struct BatchDev {
     const int64_t* p_ext;    // extrema workgroups
     const int64_t* p_tune;   // tuner sample blocks (x 15 candidates in grid.y)
     const int64_t* p_gap;    // code-gap workgroups
-    const int64_t* p_tile;   // spline tiles
+    const int64_t* p_tile;   // spline work units (a tile; 256 tiles of a 1-D field)
     const int64_t* p_shuf;   // shuffle workgroups
     const int64_t* p_chunk;  // codec chunks (one per wave64)
     const int64_t* p_merge;  // trailer workgroups
@@ -30,6 +30,7 @@ struct BatchDev {
 int64_t batch_ext_blocks(const pa_layout* L, int dtype);
 int64_t batch_gap_blocks(const BatchField& f);
 int64_t batch_shuf_blocks(const pa_layout* L);
+int64_t batch_tile_units(const pa_layout* L);  // spline workgroups
 int64_t batch_merge_blocks(const pa_layout* L, int dtype);
 // host-side preparation of a descriptor (layout filled): tuner sampling,
 // alpha, code gaps

@@ -346,15 +346,20 @@ __device__ __forceinline__ void lds_barrier() {
 
 // Quantise v against pred, write the reconstruction back and return the
 // code (InterpolationPredictor.cu:1327-1341).
-template <typename T, bool IN>
-__device__ __forceinline__ int32_t quantize(const Ctx<T, IN>& t, T& v, T pred) {
+template <typename T>
+__device__ __forceinline__ int32_t quantize_val(T& v, T pred, T eb_r, T ebx2) {
     const T err = v - pred;
-    T code = (err < (T)0 ? -err : err) * t.eb_r + (T)1;
+    T code = (err < (T)0 ? -err : err) * eb_r + (T)1;
     code = err < (T)0 ? -code : code;
     const int qv = (int)(code / (T)2);
     code = (T)qv;
-    v = pred + code * t.ebx2;
+    v = pred + code * ebx2;
     return qv;
+}
+
+template <typename T, bool IN>
+__device__ __forceinline__ int32_t quantize(const Ctx<T, IN>& t, T& v, T pred) {
+    return quantize_val(v, pred, t.eb_r, t.ebx2);
 }
 
 // Decompression first fills the LDS slot of every point with the bit
@@ -883,6 +888,196 @@ __device__ __forceinline__ void tile_run(const SplineArgs<T>& a, T* tile, int32_
     }
 }
 
+// ------------------------------------------------------------ 1-D fields --
+// A field with ny = nz = 1 holds only the line y = z = 0 of each tile: no
+// point of a y or z stage, and of the multi-dimensional stages only the
+// LINE points along x, lies inside it, so every level reduces to the x
+// stage of the line rule above (same stencil choice, same operations)
+// whatever the flags say.  Lane l owns one whole tile, its 16 points and
+// the halo point, in registers; a workgroup covers 256 consecutive tiles.
+// Field values, codes and output pass through LDS so that every global
+// access is coalesced.
+constexpr int kLineTiles = kThreads;             // tiles per workgroup, one per lane
+constexpr int kLinePts = 16 * kLineTiles;        // field points per workgroup
+constexpr int kLineSlots = 17 * kLineTiles + 1;  // staged values: tile l point k at 17 l + k (k < 16)
+constexpr int kLineCodes = 15 * kLineTiles;      // codes per workgroup (levels 3..0: 1, 2, 4, 8 per tile)
+
+__device__ __forceinline__ int line_slot(int i) { return i + (i >> 4); }
+
+template <int L>
+constexpr int line_code_off() {  // level L's codes in the LDS code block
+    return L == 3 ? 0 : (L == 2 ? kLineTiles : (L == 1 ? 3 * kLineTiles : 7 * kLineTiles));
+}
+
+// Points x = (2j + 1) U of one tile, j < N = 8 / U, in the order of the
+// line stage: values at even multiples of U are known.
+template <typename T, int L, int M>
+__device__ __forceinline__ void line_level(T (&v)[17], int32_t (&q)[8 >> L], T eb_r, T ebx2, int ox, int nx,
+                                           bool c1) {
+    constexpr int U = 1 << L, N = 8 >> L;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const int c = (2 * j + 1) * U;
+        if (ox + c >= nx) break;  // beyond the field; so are later points
+        const T pred = apply_stencil(stencil_of<U>(j, N, c1, ox, nx), v[j > 0 ? c - 3 * U : 0], v[c - U], v[c + U],
+                                     v[j + 2 <= N ? c + 3 * U : 0]);
+        if constexpr (M == COMPRESS) q[j] = quantize_val(v[c], pred, eb_r, ebx2);
+        else v[c] = pred + (T)q[j] * ebx2;
+    }
+}
+
+// codes of level L of the workgroup (code i = point m = unit 256 N + i of
+// the level, at x = (2m + 1) U) between global memory and the LDS block
+template <int L>
+__device__ __forceinline__ void line_codes_out(const Geom& g, int64_t unit, const int32_t* ci, int32_t* qc) {
+    constexpr int U = 1 << L, N = 8 >> L;
+    const int lane = lane_id();
+    const int64_t m0 = unit * (int64_t)(kLineTiles * N);
+#pragma unroll
+    for (int it = 0; it < N; ++it) {
+        const int i = lane + it * kThreads;
+        const int64_t m = m0 + i;
+        if ((2 * m + 1) * U < g.nx) qc[g.prefix[L] + m] = ci[line_code_off<L>() + i];
+    }
+}
+
+template <int L>
+__device__ __forceinline__ void line_codes_load(const Geom& g, int64_t unit, bool load, const int32_t* qc,
+                                                int32_t (&r)[8 >> L]) {
+    constexpr int U = 1 << L, N = 8 >> L;
+    const int lane = lane_id();
+    const int64_t m0 = unit * (int64_t)(kLineTiles * N);
+#pragma unroll
+    for (int it = 0; it < N; ++it) {
+        const int64_t m = m0 + lane + it * kThreads;
+        r[it] = (load && (2 * m + 1) * U < g.nx) ? qc[g.prefix[L] + m] : 0;
+    }
+}
+
+template <int L>
+__device__ __forceinline__ void line_codes_stage(const int32_t (&r)[8 >> L], int32_t* ci) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int it = 0; it < (8 >> L); ++it) ci[line_code_off<L>() + lane + it * kThreads] = r[it];
+}
+
+template <int L>
+__device__ __forceinline__ void line_codes_take(const int32_t* ci, int32_t (&q)[8 >> L]) {
+    constexpr int N = 8 >> L;
+    const int lane = lane_id();
+#pragma unroll
+    for (int j = 0; j < N; ++j) q[j] = ci[line_code_off<L>() + lane * N + j];
+}
+
+template <typename T, int M>
+__device__ __forceinline__ void line_tiles(const SplineArgs<T>& a, T* lds, int64_t unit) {
+    static_assert(kLineCodes * sizeof(int32_t) <= kLineSlots * sizeof(float), "code block fits the staging area");
+    const Geom& g = a.g;
+    const int nx = g.nx;
+    const int64_t p0 = unit * kLinePts;
+    const int lane = lane_id();
+    const int64_t tile = unit * kLineTiles + lane;
+    const bool active = tile < g.gx;
+    const int ox = (int)(tile * 16);
+    const bool c1 = (int64_t)ox + 16 < nx;
+    int32_t* ci = reinterpret_cast<int32_t*>(lds);
+    int32_t q3[1] = {0}, q2[2] = {0, 0}, q1[4] = {0, 0, 0, 0}, q0[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    T v[17];
+    if constexpr (M == COMPRESS) {
+        constexpr int IT = (kLinePts + 1 + kThreads - 1) / kThreads;
+        T s[IT];
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int i = lane + it * kThreads;
+            s[it] = (i <= kLinePts && p0 + i < nx) ? a.data[p0 + i] : (T)0;
+        }
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int i = lane + it * kThreads;
+            if (i <= kLinePts) lds[line_slot(i)] = s[it];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = lds[17 * lane + k];
+        v[16] = lds[17 * (lane + 1)];  // halo: point 0 of the next tile
+        if (active) a.anchors[tile] = v[0];
+        __syncthreads();  // the staging area becomes the code block
+        const LevelEb<T>& le = a.le;
+        line_level<T, 3, M>(v, q3, le.eb_r[3], le.ebx2[3], ox, nx, c1);
+        line_level<T, 2, M>(v, q2, le.eb_r[2], le.ebx2[2], ox, nx, c1);
+        line_level<T, 1, M>(v, q1, le.eb_r[1], le.ebx2[1], ox, nx, c1);
+        line_level<T, 0, M>(v, q0, le.eb_r[0], le.ebx2[0], ox, nx, c1);
+        ci[line_code_off<3>() + lane] = q3[0];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) ci[line_code_off<2>() + 2 * lane + j] = q2[j];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) ci[line_code_off<1>() + 4 * lane + j] = q1[j];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) ci[line_code_off<0>() + 8 * lane + j] = q0[j];
+        __syncthreads();
+        line_codes_out<3>(g, unit, ci, a.qc_out);
+        line_codes_out<2>(g, unit, ci, a.qc_out);
+        line_codes_out<1>(g, unit, ci, a.qc_out);
+        line_codes_out<0>(g, unit, ci, a.qc_out);
+    } else {
+        const int maxlevel = a.d_maxlevel ? *a.d_maxlevel : 4;
+        constexpr int OIT = kLinePts / kThreads;
+        T prev[OIT];
+        if (a.old) {
+#pragma unroll
+            for (int it = 0; it < OIT; ++it) {
+                const int64_t x = p0 + lane + it * kThreads;
+                prev[it] = x < nx ? a.old[x] : (T)0;
+            }
+        }
+        int32_t r3[1], r2[2], r1[4], r0[8];
+        line_codes_load<3>(g, unit, 3 <= maxlevel, a.qc_in, r3);
+        line_codes_load<2>(g, unit, 2 <= maxlevel, a.qc_in, r2);
+        line_codes_load<1>(g, unit, 1 <= maxlevel, a.qc_in, r1);
+        line_codes_load<0>(g, unit, 0 <= maxlevel, a.qc_in, r0);
+        const T a0 = (a.anchors_in && active) ? a.anchors_in[tile] : (T)0;
+        const T a16 = (a.anchors_in && tile + 1 < g.gx) ? a.anchors_in[tile + 1] : (T)0;
+        line_codes_stage<3>(r3, ci);
+        line_codes_stage<2>(r2, ci);
+        line_codes_stage<1>(r1, ci);
+        line_codes_stage<0>(r0, ci);
+        __syncthreads();
+        line_codes_take<3>(ci, q3);
+        line_codes_take<2>(ci, q2);
+        line_codes_take<1>(ci, q1);
+        line_codes_take<0>(ci, q0);
+#pragma unroll
+        for (int k = 1; k < 16; ++k) v[k] = (T)0;  // points of skipped levels stay 0
+        v[0] = a0;
+        v[16] = a16;
+        const LevelEb<T>& le = a.le;
+        if (3 <= maxlevel) line_level<T, 3, M>(v, q3, le.eb_r[3], le.ebx2[3], ox, nx, c1);
+        if (2 <= maxlevel) line_level<T, 2, M>(v, q2, le.eb_r[2], le.ebx2[2], ox, nx, c1);
+        if (1 <= maxlevel) line_level<T, 1, M>(v, q1, le.eb_r[1], le.ebx2[1], ox, nx, c1);
+        if (0 <= maxlevel) line_level<T, 0, M>(v, q0, le.eb_r[0], le.ebx2[0], ox, nx, c1);
+        __syncthreads();  // the code block becomes the output staging area
+#pragma unroll
+        for (int k = 0; k < 16; ++k) lds[17 * lane + k] = v[k];
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < OIT; ++it) {
+            const int i = lane + it * kThreads;
+            if (p0 + i < nx) {
+                const T r = lds[line_slot(i)];
+                a.out[p0 + i] = a.old ? r + prev[it] : r;
+            }
+        }
+    }
+}
+
+template <typename T, int M>
+__global__ __launch_bounds__(kThreads) void line_kernel(SplineArgs<T> a) {
+    __shared__ T lds[kLineSlots];
+    line_tiles<T, M>(a, lds, blockIdx.x);
+}
+
+__host__ __device__ inline bool line_field(const Geom& g) { return g.ny == 1 && g.nz == 1; }
+
 // kernel `interpolation` (compress) and `interpolation_reverse` /
 // `interpolation_progressive_reverse` (decompress: anchors, or zero anchors
 // plus old for the delta form).  Tiles away from the far faces of the field
@@ -1155,7 +1350,8 @@ __global__ void batch_gaps_kernel(const BatchField* __restrict__ fields, const i
 }
 
 // Compression tiles of fields [lo, hi) (one dtype), with the XCD-aware
-// order of spline_kernel taken over the concatenated tile list.
+// order of spline_kernel taken over the concatenated list of work units: a
+// tile, or 256 tiles of a 1-D field.
 template <typename T>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void batch_spline_kernel(
     const BatchField* __restrict__ fields, const int64_t* __restrict__ prefix, int lo, int hi, int64_t base,
@@ -1170,6 +1366,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                           nullptr, F.qc, nullptr, static_cast<T*>(F.anchors), nullptr, nullptr, nullptr};
     const Geom& g = a.g;
     const int64_t tix = gt - prefix[f];
+    if (line_field(g)) {
+        line_tiles<T, COMPRESS>(a, tile, tix);
+        return;
+    }
     const int bx = (int)(tix % g.gx), by = (int)((tix / g.gx) % g.gy), bz = (int)(tix / ((int64_t)g.gx * g.gy));
     const bool interior = bx * 16 + 16 < g.nx && by * 16 + 16 < g.ny && bz * 16 + 16 < g.nz;
     if (interior) tile_run<T, COMPRESS, true>(a, tile, rowtab, bx, by, bz);
@@ -1204,10 +1404,13 @@ int launch_spline(const pa_layout* L, const T* data, double eb, double rel_eb, c
     Geom g = make_geom(L);
     const int64_t ntiles = (int64_t)g.gx * g.gy * g.gz;
     if (ntiles > 0x7fffffff) return PA_ERR_ARG;
-    dim3 grid((unsigned)ntiles);
     LevelEb<T> le = level_ebs<T>(eb, rel_eb);
     const SplineArgs<T> args{data, g, le, flags8, d_maxlevel, qc_out, qc_in, anchors, anchors_in, old, out};
-    hipLaunchKernelGGL((spline_kernel<T, M>), grid, dim3(kThreads), 0, st, args);
+    if (line_field(g))
+        hipLaunchKernelGGL((line_kernel<T, M>), dim3((unsigned)((ntiles + kLineTiles - 1) / kLineTiles)),
+                           dim3(kThreads), 0, st, args);
+    else
+        hipLaunchKernelGGL((spline_kernel<T, M>), dim3((unsigned)ntiles), dim3(kThreads), 0, st, args);
     PA_LAUNCH_CHECK();
     return PA_OK;
 }
@@ -1367,4 +1570,10 @@ int pa::batch_predict(const BatchDev& b, hipStream_t st) {
         PA_LAUNCH_CHECK();
     }
     return PA_OK;
+}
+
+int64_t pa::batch_tile_units(const pa_layout* L) {
+    const Geom g = make_geom(L);
+    const int64_t ntiles = (int64_t)g.gx * g.gy * g.gz;
+    return line_field(g) ? (ntiles + kLineTiles - 1) / kLineTiles : ntiles;
 }
