@@ -185,17 +185,56 @@ size_t pa_batch_workspace(const pa_batch_field* fields, int nf);
  * d_result: device uint64[4 * nf]; after each run, entry 4i.. of field i =
  * { stream length, status (0 or PA_ERR_ARG for a non-positive bound, e.g. a
  * constant field in relative mode), value range and absolute bound as f64
- * bit patterns }.  use_graph != 0 captures the launch sequence once in a HIP
- * graph that every run replays. */
+ * bit patterns }.  The tables are uploaded on `stream` (runs on another
+ * stream must be ordered after it).  use_graph != 0 captures the launch
+ * sequence once in a HIP graph that every run replays. */
 int pa_batch_plan_create(const pa_batch_field* fields, int nf, void* d_ws, size_t ws_bytes,
-                         uint64_t* d_result, int use_graph, pa_batch_plan** plan);
+                         uint64_t* d_result, int use_graph, void* stream, pa_batch_plan** plan);
 /* Points the plan at new inputs / outputs of the same shapes, dtypes, modes
- * and bound kinds (bounds may change); synchronous upload. */
-int pa_batch_plan_rebind(pa_batch_plan* plan, const pa_batch_field* fields, int nf);
+ * and bound kinds (bounds may change).  Synchronises `stream` first (earlier
+ * runs read the tables being replaced), then uploads on it. */
+int pa_batch_plan_rebind(pa_batch_plan* plan, const pa_batch_field* fields, int nf, void* stream);
 int pa_batch_run(pa_batch_plan* plan, void* stream);
 /* kernel launches one run issues (graph nodes when captured) */
 int pa_batch_plan_launches(const pa_batch_plan* plan);
 int pa_batch_plan_destroy(pa_batch_plan* plan);
+
+/* ---- Batched progressive retrieval (no reference counterpart) ----
+ * One round of decompress_progressive_pipeline (compressor.cu:166-190) for
+ * each of many resident fields, by one fixed sequence of launches (strategy
+ * search and update, offsets and window, count scan, decode, inverse
+ * shuffle, reconstruction: at most 8 kernels) whatever the number of jobs.
+ * Each job's state and output equal those of the plain calls
+ * pa_find_strategy, pa_lossless_decode_window, pa_inverse_bitshuffle_window
+ * and pa_spline_reconstruct (round 0) / pa_spline_delta_reconstruct. */
+typedef struct pa_retrieve_job {
+    const uint8_t* stream; /* device: the field's progressive stream */
+    int64_t stream_len;
+    int32_t dtype;
+    uint32_t nx, ny, nz;
+    double eb;        /* absolute bound of the stream */
+    double rel_eb;    /* relative bound of the stream (0 in absolute mode) */
+    double target;    /* absolute target error of this round */
+    int32_t* state;   /* device int32[9]: begin[4], end[4], maxlevel (zeros before round 0) */
+    uint8_t* bp;      /* device, layout host_size bytes, kept between rounds */
+    int32_t* qc;      /* device, code_len int32, kept between rounds */
+    void* dws;        /* device, pa_decode_workspace bytes, kept between rounds */
+    uint64_t* bytes_read; /* device, may be NULL: accumulates payload bytes read */
+    int32_t round;    /* rounds already run on this field */
+    const void* old;  /* device: the previous reconstruction (round > 0) */
+    void* out;        /* device: the new reconstruction (n values) */
+} pa_retrieve_job;
+
+/* bytes of the job table staged from the host */
+size_t pa_retrieve_table_bytes(int nj);
+/* device workspace of a batch of nj jobs (job table, strategy keys, scan) */
+size_t pa_retrieve_workspace(int nj);
+/* h_table: pinned host memory of pa_retrieve_table_bytes(nj) bytes, left
+ * untouched until the work on `stream` has completed (the table is copied
+ * from it asynchronously); NULL: the call synchronises `stream` and copies
+ * from memory of its own.  Jobs must name distinct fields. */
+int pa_retrieve_batch(const pa_retrieve_job* jobs, int nj, void* h_table, size_t table_bytes, void* d_ws,
+                      size_t ws_bytes, void* stream);
 
 /* Library / device information (no reference counterpart). */
 const char* pa_version(void);

@@ -62,6 +62,16 @@ class BatchFieldDesc(C.Structure):
     ]
 
 
+class RetrieveJobDesc(C.Structure):
+    """pa_retrieve_job"""
+    _fields_ = [
+        ("stream", C.c_void_p), ("stream_len", C.c_int64), ("dtype", C.c_int32), ("nx", C.c_uint32),
+        ("ny", C.c_uint32), ("nz", C.c_uint32), ("eb", C.c_double), ("rel_eb", C.c_double), ("target", C.c_double),
+        ("state", C.c_void_p), ("bp", C.c_void_p), ("qc", C.c_void_p), ("dws", C.c_void_p),
+        ("bytes_read", C.c_void_p), ("round", C.c_int32), ("old", C.c_void_p), ("out", C.c_void_p),
+    ]
+
+
 P = C.c_void_p
 I32 = C.c_int
 I64 = C.c_int64
@@ -94,11 +104,14 @@ SIGNATURES = {
     "pa_quality_workspace": (SZ, [I64]),
     "pa_quality": (I32, [P, P, I32, I64, P, P, SZ, P]),
     "pa_batch_workspace": (SZ, [P, I32]),
-    "pa_batch_plan_create": (I32, [P, I32, P, SZ, P, I32, C.POINTER(C.c_void_p)]),
-    "pa_batch_plan_rebind": (I32, [P, P, I32]),
+    "pa_batch_plan_create": (I32, [P, I32, P, SZ, P, I32, P, C.POINTER(C.c_void_p)]),
+    "pa_batch_plan_rebind": (I32, [P, P, I32, P]),
     "pa_batch_run": (I32, [P, P]),
     "pa_batch_plan_launches": (I32, [P]),
     "pa_batch_plan_destroy": (I32, [P]),
+    "pa_retrieve_table_bytes": (SZ, [I32]),
+    "pa_retrieve_workspace": (SZ, [I32]),
+    "pa_retrieve_batch": (I32, [P, I32, P, SZ, P, SZ, P]),
     "pa_version": (C.c_char_p, []),
     "pa_device_count": (I32, []),
 }

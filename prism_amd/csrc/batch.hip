@@ -17,6 +17,7 @@ struct pa_batch_plan {
     int nf = 0, nf32 = 0;
     std::vector<int> order;  // order[k] = caller index of descriptor k
     std::vector<pa::BatchField> host;
+    std::vector<int64_t> pre;  // prefix tables (host copy, source of the upload)
     pa::BatchDev dev{};
     uint8_t* table = nullptr;  // descriptors and prefixes inside the workspace
     int launches = 0;
@@ -154,15 +155,11 @@ int build_descriptors(const pa_batch_field* fields, int nf, const std::vector<in
     return PA_OK;
 }
 
-size_t scratch_bytes(const std::vector<BatchField>& fs) {
-    size_t s = 0;
-    for (const BatchField& f : fs) s += field_sizes(f).total();
-    return s;
-}
-
-int upload(pa_batch_plan* p) {
+// descriptor and prefix tables into the workspace, ordered on `st`
+int upload(pa_batch_plan* p, hipStream_t st) {
     const int nf = p->nf;
-    std::vector<int64_t> pre((size_t)kPrefixes * (nf + 1), 0);
+    std::vector<int64_t>& pre = p->pre;
+    pre.assign((size_t)kPrefixes * (nf + 1), 0);
     int64_t* P[kPrefixes];
     for (int s = 0; s < kPrefixes; ++s) P[s] = pre.data() + (size_t)s * (nf + 1);
     for (int k = 0; k < nf; ++k) {
@@ -202,9 +199,11 @@ int upload(pa_batch_plan* p) {
     if (d.n_ext > lim || d.n_tune > lim || d.n_gap > lim || d.n_tile > lim || d.n_shuf > lim ||
         (d.n_chunk + 3) / 4 > lim || d.n_merge > lim)
         return PA_ERR_ARG;
-    if (hipMemcpy(p->table, p->host.data(), (size_t)nf * sizeof(BatchField), hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMemcpyAsync(p->table, p->host.data(), (size_t)nf * sizeof(BatchField), hipMemcpyHostToDevice, st) !=
+        hipSuccess)
         return PA_ERR_HIP;
-    if (hipMemcpy(p->table + desc, pre.data(), pre.size() * 8, hipMemcpyHostToDevice) != hipSuccess) return PA_ERR_HIP;
+    if (hipMemcpyAsync(p->table + desc, pre.data(), pre.size() * 8, hipMemcpyHostToDevice, st) != hipSuccess)
+        return PA_ERR_HIP;
     p->launches = count_launches(d);
     return PA_OK;
 }
@@ -244,7 +243,7 @@ extern "C" size_t pa_batch_workspace(const pa_batch_field* fields, int nf) {
 }
 
 extern "C" int pa_batch_plan_create(const pa_batch_field* fields, int nf, void* d_ws, size_t ws_bytes,
-                                    uint64_t* d_result, int use_graph, pa_batch_plan** plan) {
+                                    uint64_t* d_result, int use_graph, void* stream, pa_batch_plan** plan) {
     if (!fields || nf <= 0 || !d_result || !plan) return PA_ERR_ARG;
     *plan = nullptr;
     const size_t need = pa_batch_workspace(fields, nf);
@@ -258,7 +257,7 @@ extern "C" int pa_batch_plan_create(const pa_batch_field* fields, int nf, void* 
     int st = build_descriptors(fields, nf, p->order, ws + table_bytes(nf), d_result, p->host, nullptr);
     if (st == PA_OK) {
         p->table = ws;
-        st = upload(p);
+        st = upload(p, pa::as_stream(stream));
     }
     if (st == PA_OK && use_graph) st = capture(p);
     if (st != PA_OK) {
@@ -269,8 +268,11 @@ extern "C" int pa_batch_plan_create(const pa_batch_field* fields, int nf, void* 
     return PA_OK;
 }
 
-extern "C" int pa_batch_plan_rebind(pa_batch_plan* p, const pa_batch_field* fields, int nf) {
+extern "C" int pa_batch_plan_rebind(pa_batch_plan* p, const pa_batch_field* fields, int nf, void* stream) {
     if (!p || !fields || nf != p->nf) return PA_ERR_ARG;
+    hipStream_t hs = pa::as_stream(stream);
+    // earlier uploads and runs read the tables being replaced
+    if (hipStreamSynchronize(hs) != hipSuccess) return PA_ERR_HIP;
     std::vector<int> order;
     plan_order(fields, nf, order);
     if (order != p->order) return PA_ERR_ARG;
@@ -281,7 +283,8 @@ extern "C" int pa_batch_plan_rebind(pa_batch_plan* p, const pa_batch_field* fiel
     const int st = build_descriptors(fields, nf, order, scratch, d_result, fs, &p->host);
     if (st != PA_OK) return st;
     p->host.swap(fs);
-    if (hipMemcpy(p->table, p->host.data(), (size_t)nf * sizeof(BatchField), hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMemcpyAsync(p->table, p->host.data(), (size_t)nf * sizeof(BatchField), hipMemcpyHostToDevice, hs) !=
+        hipSuccess)
         return PA_ERR_HIP;
     return PA_OK;
 }
@@ -301,4 +304,128 @@ extern "C" int pa_batch_plan_destroy(pa_batch_plan* p) {
     if (p->exec && hipGraphExecDestroy(p->exec) != hipSuccess) st = PA_ERR_HIP;
     delete p;
     return st;
+}
+
+// ---------------------------------------------------------------- retrieval --
+namespace {
+
+using pa::RetrJob;
+
+struct RetrTables {
+    size_t jobs, pre, total;  // bytes: job table, two prefix tables
+};
+RetrTables retr_tables(int nj) {
+    RetrTables t;
+    t.jobs = up((size_t)nj * sizeof(RetrJob));
+    t.pre = up((size_t)2 * (nj + 1) * 8);
+    t.total = t.jobs + t.pre;
+    return t;
+}
+size_t retr_keys_bytes(int nj) { return up((size_t)nj * 33 * 33 * sizeof(unsigned long long)); }
+
+int retr_describe(const pa_retrieve_job& in, RetrJob& j) {
+    if (!in.stream || !in.state || !in.bp || !in.qc || !in.dws || !in.out || (in.dtype != PA_F32 && in.dtype != PA_F64) ||
+        !(in.eb > 0) || !(in.rel_eb >= 0) || in.round < 0 || (in.round > 0 && !in.old))
+        return PA_ERR_ARG;
+    j = RetrJob{};
+    int st = pa_layout_init(&j.L, in.nx, in.ny, in.nz);
+    if (st != PA_OK) return st;
+    j.g = pa::make_geom(&j.L);
+    j.dtype = in.dtype;
+    // trailer: flags8 | range | cum[4][32] | anchors (compressor.cu:139-151)
+    const int64_t t0 = in.stream_len - (j.L.trailer_fixed + j.L.n4 * in.dtype);
+    if (t0 < 8 + 2 * j.L.nchunks || t0 % 8) return PA_ERR_STREAM;
+    j.stream = in.stream;
+    j.stream_len = in.stream_len;
+    j.flags = in.stream + t0;
+    j.cum = reinterpret_cast<const uint64_t*>(in.stream + t0 + 16);
+    j.anchors = in.stream + t0 + 1040;
+    // findStrategy_h 181, 193 (as pa_find_strategy)
+    if (in.target * 0.954 >= in.eb) {
+        const double q = in.target * 0.954 / in.eb;
+        if (!(q < 1.8e19)) return PA_ERR_ARG;
+        j.search = 1;
+        j.target_cost = (uint64_t)__builtin_floor(q) - 1;
+    }
+    j.offsets_ready = in.round > 0;
+    j.first = in.round == 0;
+    pa::retr_level_ebs(j, in.eb, in.rel_eb);
+    j.state = in.state;
+    j.bp = in.bp;
+    j.qc = in.qc;
+    j.dws = static_cast<uint8_t*>(in.dws);
+    j.bytes_read = in.bytes_read;
+    j.old = in.old;
+    j.out = in.out;
+    return PA_OK;
+}
+
+}  // namespace
+
+extern "C" size_t pa_retrieve_table_bytes(int nj) { return nj > 0 ? retr_tables(nj).total : 0; }
+
+extern "C" size_t pa_retrieve_workspace(int nj) {
+    if (nj <= 0) return 0;
+    return retr_tables(nj).total + retr_keys_bytes(nj) + up((size_t)(nj + 1) * 8);
+}
+
+extern "C" int pa_retrieve_batch(const pa_retrieve_job* jobs, int nj, void* h_table, size_t table_bytes, void* d_ws,
+                                 size_t ws_bytes, void* stream) {
+    if (!jobs || nj <= 0 || nj > 65535) return PA_ERR_ARG;
+    if (!d_ws || ws_bytes < pa_retrieve_workspace(nj)) return PA_ERR_WORKSPACE;
+    const RetrTables tb = retr_tables(nj);
+    if (h_table && table_bytes < tb.total) return PA_ERR_WORKSPACE;
+    std::vector<uint8_t> local;
+    uint8_t* ht = static_cast<uint8_t*>(h_table);
+    if (!ht) {
+        local.resize(tb.total);
+        ht = local.data();
+    }
+    uint8_t* ws = static_cast<uint8_t*>(d_ws);
+    RetrJob* hj = reinterpret_cast<RetrJob*>(ht);
+    int64_t* p_unshuf = reinterpret_cast<int64_t*>(ht + tb.jobs);
+    int64_t* p_unit = p_unshuf + (nj + 1);
+    unsigned long long* keys = reinterpret_cast<unsigned long long*>(ws + tb.total);
+    int64_t* p_chunk = reinterpret_cast<int64_t*>(ws + tb.total + retr_keys_bytes(nj));
+    // float32 jobs first
+    int k = 0, nj32 = 0;
+    int64_t max_chunks = 0;
+    p_unshuf[0] = p_unit[0] = 0;
+    for (int pass = 0; pass < 2; ++pass)
+        for (int i = 0; i < nj; ++i) {
+            if ((jobs[i].dtype == PA_F32) != (pass == 0)) continue;
+            RetrJob& j = hj[k];
+            const int st = retr_describe(jobs[i], j);
+            if (st != PA_OK) return st;
+            j.keys = keys + (size_t)k * 33 * 33;
+            p_unshuf[k + 1] = p_unshuf[k] + pa::retr_unshuf_blocks(&j.L);
+            p_unit[k + 1] = p_unit[k] + pa::batch_tile_units(&j.L);
+            max_chunks += j.L.nchunks;
+            nj32 += pass == 0;
+            ++k;
+        }
+    pa::RetrDev r;
+    r.jobs = reinterpret_cast<const RetrJob*>(ws);
+    r.nj = nj;
+    r.nj32 = nj32;
+    r.p_unshuf = reinterpret_cast<const int64_t*>(ws + tb.jobs);
+    r.p_unit = r.p_unshuf + (nj + 1);
+    r.p_chunk = p_chunk;
+    r.n_unshuf = p_unshuf[nj];
+    r.n_unit = p_unit[nj];
+    r.n_unit32 = p_unit[nj32];
+    r.max_chunks = max_chunks;
+    if (r.n_unshuf > 0x7fffffff || r.n_unit > 0x7fffffff) return PA_ERR_ARG;
+    hipStream_t hs = pa::as_stream(stream);
+    if (h_table) {
+        if (hipMemcpyAsync(ws, ht, tb.total, hipMemcpyHostToDevice, hs) != hipSuccess) return PA_ERR_HIP;
+    } else {
+        if (hipStreamSynchronize(hs) != hipSuccess) return PA_ERR_HIP;
+        if (hipMemcpy(ws, ht, tb.total, hipMemcpyHostToDevice) != hipSuccess) return PA_ERR_HIP;
+    }
+    int e;
+    if ((e = pa::retr_strategy(r, hs)) != PA_OK) return e;
+    if ((e = pa::retr_decode(r, hs)) != PA_OK) return e;
+    if ((e = pa::retr_unshuffle(r, hs)) != PA_OK) return e;
+    return pa::retr_reconstruct(r, hs);
 }

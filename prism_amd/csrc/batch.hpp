@@ -36,6 +36,24 @@ int64_t batch_merge_blocks(const pa_layout* L, int dtype);
 // alpha, code gaps
 void batch_predict_setup(BatchField& f);
 
+// A batch of retrieval rounds: jobs [0, nj32) hold float32 fields.
+struct RetrDev {
+    const RetrJob* jobs;
+    int nj, nj32;
+    const int64_t* p_unshuf;  // inverse-shuffle workgroups
+    const int64_t* p_unit;    // reconstruction work units
+    int64_t* p_chunk;         // selected chunks (device-computed exclusive prefix, nj + 1)
+    int64_t n_unshuf, n_unit, n_unit32, max_chunks;
+};
+
+int retr_strategy(const RetrDev& r, hipStream_t st);     // strategy.hip
+int retr_decode(const RetrDev& r, hipStream_t st);       // codec.hip: offsets, window, decode
+int retr_unshuffle(const RetrDev& r, hipStream_t st);    // bitplane.hip
+int retr_reconstruct(const RetrDev& r, hipStream_t st);  // predictor.hip
+// host helpers for job set-up
+void retr_level_ebs(RetrJob& j, double eb, double rel_eb);
+int64_t retr_unshuf_blocks(const pa_layout* L);
+
 int batch_extrema(const BatchDev& b, hipStream_t st);  // extrema.hip
 int batch_predict(const BatchDev& b, hipStream_t st);  // predictor.hip: eb, tuner, code gaps, spline
 int batch_shuffle(const BatchDev& b, hipStream_t st);  // bitplane.hip

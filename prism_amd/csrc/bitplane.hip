@@ -141,14 +141,12 @@ __global__ __launch_bounds__(kThreads) void batch_shuffle_kernel(const BatchFiel
 // Inverse; WINDOW restricts each level to planes [begin[lr], end[lr]) (plus
 // the sign plane in SM mode, bitplane.cu:467-471) and reads only those.
 template <int MODE, bool WINDOW>
-__global__ __launch_bounds__(kThreads) void unshuffle_kernel(const uint8_t* __restrict__ bp, Levels v,
-                                                             const int32_t* __restrict__ begin,
-                                                             const int32_t* __restrict__ end,
-                                                             int32_t* __restrict__ qc) {
-    __shared__ int32_t stage[kThreads * kRow];
+__device__ __forceinline__ void unshuffle_block(const uint8_t* __restrict__ bp, const Levels& v,
+                                                const int32_t* __restrict__ begin, const int32_t* __restrict__ end,
+                                                int32_t* __restrict__ qc, int blk, int32_t* stage) {
     int first;
-    const int lv = block_level(v, blockIdx.x, &first);
-    const int64_t g0 = (int64_t)(blockIdx.x - first) * kGroupsPerBlock;
+    const int lv = block_level(v, blk, &first);
+    const int64_t g0 = (int64_t)(blk - first) * kGroupsPerBlock;
     uint32_t planes = 0xFFFFFFFFu;
     if (WINDOW) {
         const int lr = 3 - lv;
@@ -181,6 +179,25 @@ __global__ __launch_bounds__(kThreads) void unshuffle_kernel(const uint8_t* __re
         const int64_t r = c0 + i;
         if (r < seg) dst[r] = stage[(i >> 5) * kRow + (i & 31)];
     }
+}
+
+template <int MODE, bool WINDOW>
+__global__ __launch_bounds__(kThreads) void unshuffle_kernel(const uint8_t* __restrict__ bp, Levels v,
+                                                             const int32_t* __restrict__ begin,
+                                                             const int32_t* __restrict__ end,
+                                                             int32_t* __restrict__ qc) {
+    __shared__ int32_t stage[kThreads * kRow];
+    unshuffle_block<MODE, WINDOW>(bp, v, begin, end, qc, blockIdx.x, stage);
+}
+
+// window inverse (NB) of every job of a batch of retrieval rounds
+__global__ __launch_bounds__(kThreads) void retr_unshuffle_kernel(const RetrJob* __restrict__ jobs,
+                                                                  const int64_t* __restrict__ prefix, int nj) {
+    __shared__ int32_t stage[kThreads * kRow];
+    const int j = find_field(prefix, 0, nj, blockIdx.x);
+    const RetrJob& J = jobs[j];
+    const Levels v = make_levels(&J.L);
+    unshuffle_block<PA_MODE_NB, true>(J.bp, v, J.state, J.state + 4, J.qc, (int)(blockIdx.x - prefix[j]), stage);
 }
 
 }  // namespace
@@ -237,6 +254,16 @@ int64_t pa::batch_shuf_blocks(const pa_layout* L) { return make_levels(L).blk_st
 int pa::batch_shuffle(const BatchDev& b, hipStream_t st) {
     if (b.n_shuf <= 0) return PA_OK;
     hipLaunchKernelGGL(batch_shuffle_kernel, dim3((unsigned)b.n_shuf), dim3(kThreads), 0, st, b.fields, b.p_shuf, b.nf);
+    PA_LAUNCH_CHECK();
+    return PA_OK;
+}
+
+int64_t pa::retr_unshuf_blocks(const pa_layout* L) { return make_levels(L).blk_start[4]; }
+
+int pa::retr_unshuffle(const RetrDev& r, hipStream_t st) {
+    if (r.n_unshuf <= 0) return PA_OK;
+    hipLaunchKernelGGL(retr_unshuffle_kernel, dim3((unsigned)r.n_unshuf), dim3(kThreads), 0, st, r.jobs, r.p_unshuf,
+                       r.nj);
     PA_LAUNCH_CHECK();
     return PA_OK;
 }

@@ -410,8 +410,7 @@ __global__ void batch_merge_kernel(const BatchField* __restrict__ fields, const 
 }
 
 // exclusive offsets of the payloads from the size table; one workgroup
-__global__ __launch_bounds__(1024) void size_scan_kernel(const uint16_t* __restrict__ sizes, int64_t n,
-                                                         int64_t* __restrict__ offsets) {
+__device__ __forceinline__ void size_scan(const uint16_t* __restrict__ sizes, int64_t n, int64_t* __restrict__ offsets) {
     __shared__ int64_t s_w[16];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t per = (n + 1023) / 1024;
@@ -435,6 +434,11 @@ __global__ __launch_bounds__(1024) void size_scan_kernel(const uint16_t* __restr
     }
 }
 
+__global__ __launch_bounds__(1024) void size_scan_kernel(const uint16_t* __restrict__ sizes, int64_t n,
+                                                         int64_t* __restrict__ offsets) {
+    size_scan(sizes, n, offsets);
+}
+
 // Whether chunk cp holds a byte of a requested device bit-plane: planes
 // [begin[lr], end[lr]) of level lv span device bytes
 // [A_l + begin * s_l, A_l + end * s_l).
@@ -456,11 +460,10 @@ __device__ __forceinline__ bool chunk_selected(const ChunkGeom& g, const ChunkPo
 
 // One workgroup: the ids of the selected chunks in chunk order, their count,
 // and the payload bytes they hold (one atomic per call).
-__global__ __launch_bounds__(1024) void window_select_kernel(const uint16_t* __restrict__ sizes, ChunkGeom g,
-                                                             const int32_t* __restrict__ begin,
-                                                             const int32_t* __restrict__ end,
-                                                             int32_t* __restrict__ list, int32_t* __restrict__ count,
-                                                             uint64_t* __restrict__ bytes_read) {
+__device__ __forceinline__ void window_select(const uint16_t* __restrict__ sizes, const ChunkGeom& g,
+                                              const int32_t* __restrict__ begin, const int32_t* __restrict__ end,
+                                              int32_t* __restrict__ list, int32_t* __restrict__ count,
+                                              uint64_t* __restrict__ bytes_read) {
     __shared__ int s_w[16];
     __shared__ unsigned long long s_b[16];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -495,6 +498,14 @@ __global__ __launch_bounds__(1024) void window_select_kernel(const uint16_t* __r
         for (int w = 0; w < 16; ++w) t += s_b[w];
         atomicAdd((unsigned long long*)bytes_read, t);
     }
+}
+
+__global__ __launch_bounds__(1024) void window_select_kernel(const uint16_t* __restrict__ sizes, ChunkGeom g,
+                                                             const int32_t* __restrict__ begin,
+                                                             const int32_t* __restrict__ end,
+                                                             int32_t* __restrict__ list, int32_t* __restrict__ count,
+                                                             uint64_t* __restrict__ bytes_read) {
+    window_select(sizes, g, begin, end, list, count, bytes_read);
 }
 
 // Decoder: one workgroup per chunk (chunks are independent once the
@@ -650,10 +661,96 @@ __global__ __launch_bounds__(kThreads) void decode_kernel(const uint8_t* __restr
     }
 }
 
+// ------------------------------------------------- batched retrieval --
+// decode workspace of a job: error and count words | offsets | chunk list
+struct DecodeWs {
+    int32_t* err;
+    int32_t* count;
+    int64_t* offsets;
+    int32_t* list;
+};
+__device__ __forceinline__ DecodeWs decode_ws(const RetrJob& J) {
+    DecodeWs w;
+    w.err = reinterpret_cast<int32_t*>(J.dws);
+    w.count = w.err + 1;
+    w.offsets = reinterpret_cast<int64_t*>(J.dws + 64);
+    w.list = reinterpret_cast<int32_t*>(w.offsets + J.L.nchunks);
+    return w;
+}
+
+// One workgroup per job: on the field's first round the error words and the
+// payload offsets, then the chunks of the round's window.
+__global__ __launch_bounds__(1024) void retr_prep_kernel(const RetrJob* __restrict__ jobs) {
+    const RetrJob& J = jobs[blockIdx.x];
+    const DecodeWs w = decode_ws(J);
+    const uint16_t* sizes = reinterpret_cast<const uint16_t*>(J.stream + 8);
+    if (!J.offsets_ready) {
+        if (threadIdx.x < 16) reinterpret_cast<uint32_t*>(J.dws)[threadIdx.x] = 0u;
+        size_scan(sizes, J.L.nchunks, w.offsets);
+        __syncthreads();
+    }
+    const ChunkGeom g = make_chunk_geom(&J.L);
+    window_select(sizes, g, J.state, J.state + 4, w.list, w.count, J.bytes_read);
+}
+
+// exclusive prefix of the per-job chunk counts (one workgroup)
+__global__ __launch_bounds__(1024) void retr_count_scan_kernel(const RetrJob* __restrict__ jobs, int nj,
+                                                               int64_t* __restrict__ prefix) {
+    __shared__ int64_t s_w[16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int per = (nj + 1023) / 1024;
+    const int lo = tid * per, hi = lo + per < nj ? lo + per : nj;
+    int64_t sum = 0;
+    for (int j = lo; j < hi; ++j) sum += *decode_ws(jobs[j]).count;
+    int64_t x = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int64_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) s_w[wave] = x;
+    __syncthreads();
+    int64_t run = x - sum;
+    for (int w = 0; w < wave; ++w) run += s_w[w];
+    for (int j = lo; j < hi; ++j) {
+        prefix[j] = run;
+        run += *decode_ws(jobs[j]).count;
+    }
+    if (tid == 1023) prefix[nj] = run;
+}
+
+// the selected chunks of every job, as one list walked by a fixed grid
+__global__ __launch_bounds__(kThreads) void retr_decode_kernel(const RetrJob* __restrict__ jobs, int nj,
+                                                               const int64_t* __restrict__ prefix) {
+    __shared__ DecodeLds S;
+    const int64_t total = prefix[nj];
+    for (int64_t i = blockIdx.x; i < total; i += gridDim.x) {
+        const int j = find_field(prefix, 0, nj, i);
+        const RetrJob& J = jobs[j];
+        const DecodeWs w = decode_ws(J);
+        const ChunkGeom g = make_chunk_geom(&J.L);
+        decode_chunk(J.stream, g, w.offsets, J.bp, w.err, J.stream_len, w.list[i - prefix[j]], S);
+        __syncthreads();  // the next chunk reuses the LDS
+    }
+}
+
 }  // namespace
 }  // namespace pa
 
 using namespace pa;
+
+int pa::retr_decode(const RetrDev& r, hipStream_t st) {
+    hipLaunchKernelGGL(retr_prep_kernel, dim3((unsigned)r.nj), dim3(1024), 0, st, r.jobs);
+    PA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(retr_count_scan_kernel, dim3(1), dim3(1024), 0, st, r.jobs, r.nj, r.p_chunk);
+    PA_LAUNCH_CHECK();
+    if (r.max_chunks > 0) {
+        const unsigned grid = (unsigned)(r.max_chunks < 2048 ? r.max_chunks : 2048);
+        hipLaunchKernelGGL(retr_decode_kernel, dim3(grid), dim3(kThreads), 0, st, r.jobs, r.nj, r.p_chunk);
+        PA_LAUNCH_CHECK();
+    }
+    return PA_OK;
+}
 
 // offsets (8 B per chunk), then a 16 KiB staging slot per chunk (+16 B of
 // slack for the rounded-up copy of the last slot)

@@ -83,6 +83,34 @@ struct BatchField {
     uint64_t* result;   // [total stream bytes, status, range (f64 bits), absolute eb (f64 bits)]
 };
 
+// One progressive-retrieval round of one field in a batch of rounds: the
+// field's stream and resident state, the round's target, and where the
+// new reconstruction goes.
+struct RetrJob {
+    pa_layout L;
+    Geom g;  // make_geom(&L)
+    int dtype;
+    int search;            // strategy: 1 = search under target_cost, 0 = load every plane
+    uint64_t target_cost;
+    int offsets_ready;     // decode offsets of an earlier round are in the workspace
+    int first;             // first round: reconstruction from the anchors, else delta on old
+    double le64[8];        // level bounds eb_r[4], ebx2[4] of a float64 field
+    float le32[8];         // of a float32 field
+    const uint8_t* stream;
+    int64_t stream_len;
+    const uint64_t* cum;   // trailer: cumulative payload sizes [4][32]
+    const uint8_t* flags;  // trailer: flags8
+    const void* anchors;   // trailer: anchors
+    int32_t* state;        // begin[4], end[4], maxlevel
+    unsigned long long* keys;  // strategy workgroup keys (33 x 33)
+    uint8_t* bp;           // decoded bit-planes (host layout), kept between rounds
+    int32_t* qc;           // codes, code_len
+    uint8_t* dws;          // decode workspace: error and count words | offsets | chunk list
+    uint64_t* bytes_read;  // may be null
+    const void* old;
+    void* out;
+};
+
 // field of global work item b among fields [lo, hi): the largest f with
 // prefix[f] <= b (fields with no work items share their successor's prefix)
 __device__ __forceinline__ int find_field(const int64_t* prefix, int lo, int hi, int64_t b) {

@@ -1376,6 +1376,40 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     else tile_run<T, COMPRESS, false>(a, tile, rowtab, bx, by, bz);
 }
 
+// Reconstruction units of a batch of retrieval rounds, jobs [lo, hi) (one
+// dtype): the anchored form on a field's first round, the delta form on old
+// after that; levels above the round's maxlevel are skipped.
+template <typename T>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void retr_spline_kernel(
+    const RetrJob* __restrict__ jobs, const int64_t* __restrict__ prefix, int lo, int hi, int64_t base,
+    int64_t nunits) {
+    __shared__ T tile[kTileVals];
+    __shared__ int32_t rowtab[kRowTab];
+    const int64_t b = blockIdx.x, xcd = b % kXcds, per = nunits / kXcds, extra = nunits % kXcds;
+    const int64_t gt = base + xcd * per + (xcd < extra ? xcd : extra) + b / kXcds;
+    const int j = find_field(prefix, lo, hi, gt);
+    const RetrJob& J = jobs[j];
+    LevelEb<T> le;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        le.eb_r[i] = sizeof(T) == 4 ? (T)J.le32[i] : (T)J.le64[i];
+        le.ebx2[i] = sizeof(T) == 4 ? (T)J.le32[4 + i] : (T)J.le64[4 + i];
+    }
+    const SplineArgs<T> a{nullptr, J.g, le, J.flags, J.state + 8, nullptr, J.qc, nullptr,
+                          J.first ? static_cast<const T*>(J.anchors) : nullptr,
+                          J.first ? nullptr : static_cast<const T*>(J.old), static_cast<T*>(J.out)};
+    const Geom& g = a.g;
+    const int64_t tix = gt - prefix[j];
+    if (line_field(g)) {
+        line_tiles<T, DECOMPRESS>(a, tile, tix);
+        return;
+    }
+    const int bx = (int)(tix % g.gx), by = (int)((tix / g.gx) % g.gy), bz = (int)(tix / ((int64_t)g.gx * g.gy));
+    const bool interior = bx * 16 + 16 < g.nx && by * 16 + 16 < g.ny && bz * 16 + 16 < g.nz;
+    if (interior) tile_run<T, DECOMPRESS, true>(a, tile, rowtab, bx, by, bz);
+    else tile_run<T, DECOMPRESS, false>(a, tile, rowtab, bx, by, bz);
+}
+
 // ------------------------------------------------------------- host side --
 Gaps make_gaps(const pa_layout* L) {
     Gaps gaps;
@@ -1576,4 +1610,29 @@ int64_t pa::batch_tile_units(const pa_layout* L) {
     const Geom g = make_geom(L);
     const int64_t ntiles = (int64_t)g.gx * g.gy * g.gz;
     return line_field(g) ? (ntiles + kLineTiles - 1) / kLineTiles : ntiles;
+}
+
+void pa::retr_level_ebs(RetrJob& j, double eb, double rel_eb) {
+    const LevelEb<float> l32 = level_ebs<float>(eb, rel_eb);
+    const LevelEb<double> l64 = level_ebs<double>(eb, rel_eb);
+    for (int i = 0; i < 4; ++i) {
+        j.le32[i] = l32.eb_r[i];
+        j.le32[4 + i] = l32.ebx2[i];
+        j.le64[i] = l64.eb_r[i];
+        j.le64[4 + i] = l64.ebx2[i];
+    }
+}
+
+int pa::retr_reconstruct(const RetrDev& r, hipStream_t st) {
+    if (r.n_unit32 > 0) {
+        hipLaunchKernelGGL(retr_spline_kernel<float>, dim3((unsigned)r.n_unit32), dim3(kThreads), 0, st, r.jobs,
+                           r.p_unit, 0, r.nj32, (int64_t)0, r.n_unit32);
+        PA_LAUNCH_CHECK();
+    }
+    if (r.n_unit - r.n_unit32 > 0) {
+        hipLaunchKernelGGL(retr_spline_kernel<double>, dim3((unsigned)(r.n_unit - r.n_unit32)), dim3(kThreads), 0, st,
+                           r.jobs, r.p_unit, r.nj32, r.nj, r.n_unit32, r.n_unit - r.n_unit32);
+        PA_LAUNCH_CHECK();
+    }
+    return PA_OK;
 }

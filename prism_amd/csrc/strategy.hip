@@ -9,6 +9,7 @@
 //     value << 22 | (63 - rev6(b0)) << 16 | (1023 - rev10(tid)) << 6 | (63 - k)
 // (pos = tid + 1024k), which any reduction order finds: each workgroup
 // writes the maximum key of its candidates and one workgroup reduces them.
+#include "batch.hpp"
 #include "common.hpp"
 
 namespace pa {
@@ -58,9 +59,8 @@ constexpr int kSearchBlocks = 33 * 33;
 // Grid (b0, b1) = 33 x 33 workgroups; lane p of a workgroup scores b2 =
 // p / 33, b3 = p mod 33 (p < 1089) and the workgroup writes its best key to
 // its own slot (no shared atomics: they serialise across workgroups).
-__global__ __launch_bounds__(256) void strategy_search_kernel(const uint64_t* __restrict__ cum,
-                                                              uint64_t target_cost,
-                                                              unsigned long long* __restrict__ keys) {
+__device__ __forceinline__ void strategy_search(const uint64_t* __restrict__ cum, uint64_t target_cost,
+                                                unsigned long long* __restrict__ keys, int b0, int b1) {
     __shared__ uint64_t s_val[4][33], s_cost[4][33];
     __shared__ unsigned long long s_best[4];
     const int tid = threadIdx.x;
@@ -74,7 +74,6 @@ __global__ __launch_bounds__(256) void strategy_search_kernel(const uint64_t* __
         s_cost[lr][b] = plane_cost(lr, b);
     }
     __syncthreads();
-    const int b0 = blockIdx.x, b1 = blockIdx.y;
     const uint64_t c01 = s_cost[0][b0] + s_cost[1][b1], v01 = s_val[0][b0] + s_val[1][b1];
     const uint64_t tie0 = (uint64_t)(63u - (__brev((uint32_t)b0) >> 26)) << 16;
     unsigned long long best_key = 0ull;
@@ -102,13 +101,19 @@ __global__ __launch_bounds__(256) void strategy_search_kernel(const uint64_t* __
     if (tid == 0) {
         unsigned long long k = s_best[0];
         for (int w = 1; w < 4; ++w) k = s_best[w] > k ? s_best[w] : k;
-        keys[blockIdx.y * gridDim.x + blockIdx.x] = k;
+        keys[b1 * 33 + b0] = k;
     }
 }
 
+__global__ __launch_bounds__(256) void strategy_search_kernel(const uint64_t* __restrict__ cum,
+                                                              uint64_t target_cost,
+                                                              unsigned long long* __restrict__ keys) {
+    strategy_search(cum, target_cost, keys, blockIdx.x, blockIdx.y);
+}
+
 // maximum of the workgroup keys, then the update of begin / end / maxlevel
-__global__ __launch_bounds__(256) void strategy_apply_kernel(const unsigned long long* __restrict__ keys,
-                                                             int32_t* begin, int32_t* end, int32_t* maxlevel) {
+__device__ __forceinline__ void strategy_reduce_apply(const unsigned long long* __restrict__ keys, int32_t* begin,
+                                                      int32_t* end, int32_t* maxlevel) {
     __shared__ unsigned long long s_best[4];
     unsigned long long k = 0ull;
     for (int i = threadIdx.x; i < kSearchBlocks; i += 256) k = keys[i] > k ? keys[i] : k;
@@ -125,13 +130,35 @@ __global__ __launch_bounds__(256) void strategy_apply_kernel(const unsigned long
     }
 }
 
+__global__ __launch_bounds__(256) void strategy_apply_kernel(const unsigned long long* __restrict__ keys,
+                                                             int32_t* begin, int32_t* end, int32_t* maxlevel) {
+    strategy_reduce_apply(keys, begin, end, maxlevel);
+}
+
 // loadall (dataloader.cu:21-28)
-__global__ void load_all_kernel(int32_t* begin, int32_t* end, int32_t* maxlevel) {
-    if (threadIdx.x != 0) return;
+__device__ __forceinline__ void load_all(int32_t* begin, int32_t* end, int32_t* maxlevel) {
     for (int i = 0; i < 4; ++i) begin[i] = end[i];
     for (int i = 0; i < 4; ++i) end[i] = 32;
     const int ml = begin[0] < 32 ? 3 : begin[1] < 32 ? 2 : begin[2] < 32 ? 1 : begin[3] < 32 ? 0 : 4;
     if (maxlevel) *maxlevel = ml;
+}
+
+__global__ void load_all_kernel(int32_t* begin, int32_t* end, int32_t* maxlevel) {
+    if (threadIdx.x != 0) return;
+    load_all(begin, end, maxlevel);
+}
+
+// batched rounds: grid (33, 33, jobs); jobs that load every plane skip the search
+__global__ __launch_bounds__(256) void retr_search_kernel(const RetrJob* __restrict__ jobs) {
+    const RetrJob& J = jobs[blockIdx.z];
+    if (!J.search) return;
+    strategy_search(J.cum, J.target_cost, J.keys, blockIdx.x, blockIdx.y);
+}
+
+__global__ __launch_bounds__(256) void retr_apply_kernel(const RetrJob* __restrict__ jobs) {
+    const RetrJob& J = jobs[blockIdx.x];
+    if (J.search) strategy_reduce_apply(J.keys, J.state, J.state + 4, J.state + 8);
+    else if (threadIdx.x == 0) load_all(J.state, J.state + 4, J.state + 8);
 }
 
 }  // namespace
@@ -159,5 +186,13 @@ extern "C" int pa_find_strategy(const uint64_t* d_cum, int32_t* d_begin, int32_t
         hipLaunchKernelGGL(load_all_kernel, dim3(1), dim3(64), 0, st, d_begin, d_end, d_maxlevel);
         PA_LAUNCH_CHECK();
     }
+    return PA_OK;
+}
+
+int pa::retr_strategy(const RetrDev& r, hipStream_t st) {
+    hipLaunchKernelGGL(retr_search_kernel, dim3(33, 33, (unsigned)r.nj), dim3(256), 0, st, r.jobs);
+    PA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(retr_apply_kernel, dim3((unsigned)r.nj), dim3(256), 0, st, r.jobs);
+    PA_LAUNCH_CHECK();
     return PA_OK;
 }

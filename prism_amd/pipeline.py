@@ -236,7 +236,7 @@ class BatchCompressor:
         self.ws = _empty(wsb)
         plan = C.c_void_p()
         check(lib().pa_batch_plan_create(desc, self.nf, _p(self.ws), self.ws.numel(), _p(self.result), int(graph),
-                                         C.byref(plan)), "pa_batch_plan_create")
+                                         _stream(), C.byref(plan)), "pa_batch_plan_create")
         self.plan = plan
         self.graph = graph
 
@@ -266,7 +266,7 @@ class BatchCompressor:
         """Rebinds (when given new inputs) and launches; no synchronisation."""
         if fields is not None:
             desc = self._descs(fields, ebs if ebs is not None else self._ebs)
-            check(lib().pa_batch_plan_rebind(self.plan, desc, self.nf), "pa_batch_plan_rebind")
+            check(lib().pa_batch_plan_rebind(self.plan, desc, self.nf, _stream()), "pa_batch_plan_rebind")
         check(lib().pa_batch_run(self.plan, _stream()), "pa_batch_run")
 
     def results(self):
@@ -395,6 +395,25 @@ class Retriever:
         self.rounds += 1
         return new
 
+    def job(self, target: float, out: torch.Tensor, desc) -> None:
+        """Fills a pa_retrieve_job for one round of this field (batched
+        retrieval); the round is committed by `commit(out)` once enqueued."""
+        c = self.c
+        nx, ny, nz = c.shape
+        desc.stream, desc.stream_len, desc.dtype = c.stream.data_ptr(), c.nbytes, self.dt
+        desc.nx, desc.ny, desc.nz = nx, ny, nz
+        desc.eb, desc.rel_eb = self.eb_abs, self.rel_eb
+        desc.target = target * self.range if c.rel else target
+        desc.state, desc.bp, desc.qc = self.state.data_ptr(), self.bp.data_ptr(), self.qc.data_ptr()
+        desc.dws, desc.bytes_read = self.dws.data_ptr(), self.bytes_read.data_ptr()
+        desc.round = self.rounds
+        desc.old = self.out.data_ptr() if self.out is not None else None
+        desc.out = out.data_ptr()
+
+    def commit(self, out: torch.Tensor) -> None:
+        self.out = out
+        self.rounds += 1
+
     def _reconstruct(self, new, st):
         L = self.L
         if self.out is None:
@@ -405,3 +424,43 @@ class Retriever:
             check(lib().pa_spline_delta_reconstruct(_p(self.qc), self.dt, C.byref(L), self.eb_abs, self.rel_eb,
                                                     _p(self.flags), _p(self.maxlevel), _p(self.out), _p(new),
                                                     st), "pa_spline_delta_reconstruct")
+
+
+class RetrievalBatch:
+    """One enqueued batch of retrieval rounds: the new reconstructions and
+    the buffers that must outlive the device work."""
+
+    def __init__(self, outs, keep):
+        self.outs = outs
+        self._keep = keep
+
+
+def retrieve_batch(items, pinned: bool = False) -> RetrievalBatch:
+    """One retrieval round for each (Retriever, target) of `items` (distinct
+    fields) with one fixed launch sequence; each result equals
+    Retriever.retrieve(target) on that field.  With `pinned`, the job table
+    goes through pinned host memory asynchronously (the returned object then
+    holds it until the caller has synchronised); otherwise the call
+    synchronises the current stream before its upload."""
+    nj = len(items)
+    if nj == 0:
+        return RetrievalBatch([], None)
+    if len({id(r) for r, _ in items}) != nj:
+        raise ValueError("a batch takes one round per field")
+    arr = (_lib.RetrieveJobDesc * nj)()
+    outs = []
+    for k, (r, target) in enumerate(items):
+        nx, ny, nz = r.c.shape
+        new = torch.empty((nz, ny, nx), dtype=r.c.dtype, device=r.qc.device)
+        r.job(float(target), new, arr[k])
+        outs.append(new)
+    ws = _empty(lib().pa_retrieve_workspace(nj))
+    h = None
+    tb = 0
+    if pinned:
+        tb = lib().pa_retrieve_table_bytes(nj)
+        h = torch.empty(tb, dtype=torch.uint8, pin_memory=True)
+    check(lib().pa_retrieve_batch(arr, nj, _p(h), tb, _p(ws), ws.numel(), _stream()), "pa_retrieve_batch")
+    for (r, _), new in zip(items, outs):
+        r.commit(new)
+    return RetrievalBatch(outs, (ws, h, [r.out for r, _ in items]))

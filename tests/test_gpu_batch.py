@@ -119,3 +119,32 @@ def test_constant_field_is_reported(pl):
     assert int(r[0, 1]) == 0 and int(r[1, 1]) != 0
     with pytest.raises(Exception):
         b.results()
+
+
+def test_batched_retrieval_equals_plain(pl):
+    """Rounds of many fields in one batched launch sequence against the
+    plain per-field Retriever: same reconstructions (bit for bit), same
+    begin / end / maxlevel state and bytes read; rounds include looser
+    targets after tighter ones, targets under the stream's bound (every
+    plane) and fields that sit a round out."""
+    announce("batched retrieval rounds")
+    data, ebs = mixed_fields(8, 600, max3d=72, max2d=280, max1d=1.6e5)
+    comp = [pl.compress(t, e, rel=True, progressive=True) for t, e in zip(data, ebs)]
+    plain = [pl.Retriever(c) for c in comp]
+    batched = [pl.Retriever(c) for c in comp]
+    rng = np.random.default_rng(11)
+    targets = [1e-2, 1e-3, 1e-4, 1e-5, 1e-6]
+    for rnd in range(5):
+        items, want = [], []
+        for i in range(len(comp)):
+            if rnd > 0 and rng.random() < 0.25:
+                continue  # this field sits the round out
+            t = float(targets[rng.integers(0, 5)]) if rnd < 4 else comp[i].eb_input * 0.5
+            want.append((i, plain[i].retrieve(t)))
+            items.append((batched[i], t))
+        got = pl.retrieve_batch(items, pinned=(rnd % 2 == 1))
+        torch.cuda.synchronize()
+        for (i, w), g in zip(want, got.outs):
+            assert torch.equal(w.view(torch.uint8), g.view(torch.uint8)), (rnd, i)
+            assert torch.equal(plain[i].state, batched[i].state), (rnd, i)
+            assert int(plain[i].bytes_read.item()) == int(batched[i].bytes_read.item()), (rnd, i)
