@@ -228,6 +228,8 @@ class BatchCompressor:
         self.offs = offs
         self.outbuf = torch.empty(max(tot, 1), dtype=torch.uint8, device=dev)
         self.result = torch.zeros(4 * self.nf, dtype=torch.int64, device=dev)
+        self._res_host = torch.empty(4 * self.nf, dtype=torch.int64, pin_memory=True)
+        self._done = None
         self._keep = None
         desc = self._descs(fields, ebs)
         wsb = lib().pa_batch_workspace(desc, self.nf)
@@ -268,15 +270,30 @@ class BatchCompressor:
             desc = self._descs(fields, ebs if ebs is not None else self._ebs)
             check(lib().pa_batch_plan_rebind(self.plan, desc, self.nf, _stream()), "pa_batch_plan_rebind")
         check(lib().pa_batch_run(self.plan, _stream()), "pa_batch_run")
+        self._res_host.copy_(self.result, non_blocking=True)
+        self._done = torch.cuda.Event()
+        self._done.record()
 
-    def results(self):
-        """Per-field Compressed streams of the last run (one synchronisation)."""
-        r = self.result.cpu().view(-1, 4)
+    @property
+    def done(self) -> bool:
+        """The last run (and its result copy) has completed; does not block."""
+        return self._done is None or self._done.query()
+
+    def results(self, strict: bool = True):
+        """Per-field Compressed streams of the last run; waits for that run
+        only.  A field whose bound is not positive (a constant field in REL
+        mode) raises, or gives None when not `strict`."""
+        if self._done is not None:
+            self._done.synchronize()
+        r = self._res_host.view(-1, 4)
         out = []
         for i in range(self.nf):
             total, status = int(r[i, 0]), int(r[i, 1])
             if status != 0:
-                raise _lib.PrismError(f"batch field {i}: non-positive error bound (constant field in REL mode?)")
+                if strict:
+                    raise _lib.PrismError(f"batch field {i}: non-positive error bound (constant field in REL mode?)")
+                out.append(None)
+                continue
             rng = float(r[i, 2:3].view(torch.float64).item())
             s = self.outbuf[self.offs[i]:self.offs[i] + total]
             out.append(Compressed(stream=s, shape=self.shapes[i], dtype=self.dtypes[i], eb_input=float(self._ebs[i]),
@@ -291,7 +308,8 @@ class BatchCompressor:
         plan = getattr(self, "plan", None)
         if plan:
             try:
-                torch.cuda.synchronize()  # no run of the plan may be in flight
+                if self._done is not None:
+                    self._done.synchronize()  # no run of the plan may be in flight
                 lib().pa_batch_plan_destroy(plan)
             except Exception:
                 pass
