@@ -40,6 +40,9 @@ def parse():
     ap.add_argument("--batch-fields", type=int, default=64,
                     help="fields of the mixed batch (batched vs plain per-field launches); 0 skips it")
     ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--serve-requests", type=int, default=2000,
+                    help="requests of the serving run (Poisson arrivals); 0 skips it")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="mean arrival rate, requests/s")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a known-size kernel for "
                          "scaling HBM byte counters)")
@@ -177,6 +180,7 @@ def main():
     roofline.update(measured_traffic(dominant, nx, ny, nz, s))
 
     batched = batched_section(args) if args.batch_fields > 0 else None
+    serving = serving_section(args) if args.serve_requests > 0 else None
 
     cpu = None
     if args.cpu_baseline == "on" and int(os.environ.get("RANK", "0")) == 0:
@@ -191,6 +195,7 @@ def main():
         "compress_GBps": cmp_gbps, "decompress_GBps": dec_gbps,
         "compression_ratio": field_bytes / stream_bytes, "stream_bytes": stream_bytes,
         "levels": levels, "stages": stats, "roofline": roofline, "cpu_baseline": cpu, "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(line))
 
@@ -256,6 +261,80 @@ def batched_section(args):
             "plain_GBps": total_bytes / t_plain / 1e9, "batched_GBps": total_bytes / t_graph / 1e9,
             "speedup_vs_plain": t_plain / t_graph, "launches_per_batch": b.launches,
             "shapes_zyx": specs, "dtypes": ["f32" if t.dtype == torch.float32 else "f64" for t in data]}
+
+
+def serving_section(args):
+    """16 resident progressive fields of the mixed-batch kind (seeds
+    200+i), a Poisson request stream (exponential gaps, seed 5): 80 %
+    retrieval rounds of a random resident field at a target drawn from
+    {1e-2 .. 1e-6}, refined from the field's current level, 20 %
+    compressions of new fields of the same mix (seeds 300+j).  Memory-based
+    admission under a 100 GB budget.  Latency = arrival -> result in HBM
+    and observed by the host loop."""
+    import numpy as np
+    import torch
+
+    from prism_amd import fields, pipeline, serve
+
+    comp = []
+    for i in range(16):
+        t, e = fields.batch_field(i, 200 + i, device="cuda")
+        comp.append(pipeline.compress(t, e, rel=True, progressive=True))
+        del t
+    pool = [fields.batch_field(j, 300 + j, device="cuda") for j in range(16)]
+    torch.cuda.synchronize()
+    # warm-up on a throwaway server (allocator, first launches)
+    warm = serve.Server(budget_bytes=100e9)
+    for k, c in enumerate(comp):
+        warm.add_field(k, c)
+    warm.run(serve.poisson_requests(64, rate=5000.0, seed=99, field_ids=list(range(16)), pool=pool), pace=True)
+    torch.cuda.synchronize()
+    del warm
+    srv = serve.Server(budget_bytes=100e9, max_batch=64, max_inflight=2)
+    for k, c in enumerate(comp):
+        srv.add_field(k, c)
+    reqs = serve.poisson_requests(args.serve_requests, rate=args.serve_rate, seed=5, field_ids=list(range(16)),
+                                  pool=pool)
+    out = serving_run(srv, reqs)
+    out["rate_offered"] = args.serve_rate
+    # the same request mix offered as a burst (1e6 req/s): what the loop
+    # sustains when requests queue up and batches fill
+    sat = serve.Server(budget_bytes=100e9, max_batch=64, max_inflight=2)
+    for k, c in enumerate(comp):
+        sat.add_field(k, c)
+    reqs = serve.poisson_requests(args.serve_requests, rate=1e6, seed=5, field_ids=list(range(16)), pool=pool)
+    out["saturated"] = serving_run(sat, reqs)
+    out["saturated"]["rate_offered"] = 1e6
+    return out
+
+
+def serving_run(srv, reqs):
+    import numpy as np
+    import torch
+
+    from prism_amd import serve
+
+    t0 = srv.run(reqs, pace=True)
+    torch.cuda.synchronize()
+    done = [r for r in reqs if r.state == "done"]
+    lat = np.array([r.latency for r in done]) * 1e3
+    wall = max(r.done for r in done) - t0
+    payload = sum(int(res.retriever.bytes_read.item()) for res in srv.fields.values())
+    nbytes = sum(serve.field_bytes(srv, r) for r in done) + payload
+
+    def pct(a, q):
+        return float(np.percentile(a, q)) if len(a) else None
+
+    by = {}
+    for kind in ("retrieve", "compress"):
+        a = np.array([r.latency for r in done if r.kind == kind]) * 1e3
+        by[kind] = {"count": int(len(a)), "p50_ms": pct(a, 50), "p99_ms": pct(a, 99)}
+    return {"requests": len(reqs), "completed": len(done),
+            "wall_s": wall, "throughput_req_s": len(done) / wall, "GBps": nbytes / wall / 1e9,
+            "algorithmic_bytes": nbytes, "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "mean_ms": float(lat.mean()),
+            "max_ms": float(lat.max()), "by_kind": by, "batches": srv.batches,
+            "mean_batch": len(done) / max(srv.batches, 1), "resident_fields": len(srv.fields),
+            "resident_bytes": srv.resident_bytes, "budget_bytes": srv.budget}
 
 
 def measured_traffic(stage, nx, ny, nz, s):
