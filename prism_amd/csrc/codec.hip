@@ -439,77 +439,72 @@ __global__ __launch_bounds__(1024) void size_scan_kernel(const uint16_t* __restr
     size_scan(sizes, n, offsets);
 }
 
-// Whether chunk cp holds a byte of a requested device bit-plane: planes
-// [begin[lr], end[lr]) of level lv span device bytes
-// [A_l + begin * s_l, A_l + end * s_l).
-__device__ __forceinline__ bool chunk_selected(const ChunkGeom& g, const ChunkPos& cp, const int32_t* begin,
-                                               const int32_t* end) {
-    bool sel = false;
+// Chunks tile the bit-plane buffer in id order (levels 3..0, plane, index),
+// so the chunks overlapping one device window form one id range and a
+// round's selection is at most four ranges in increasing order.
+__device__ __forceinline__ int64_t chunk_of_byte(const ChunkGeom& g, int64_t x) {
+    int i = 0;  // levels 3, 2, 1, 0 in increasing host offset
+    while (i < 3 && x >= g.hbase[2 - i]) ++i;
+    const int lv = 3 - i;
+    const int64_t r = x - g.hbase[lv];
+    const int64_t p = r / g.e[lv];
+    return g.chunk_start[i] + p * g.cb[lv] + (r - p * g.e[lv]) / kCS;
+}
+
+struct Window {
+    int64_t lo[4], hi[4];  // disjoint, increasing; [lo, hi) may be empty
+    int64_t count;
+};
+
+// The chunks holding a byte of a requested device plane: for each level with
+// begin < end, those overlapping [A_l + begin s_l, A_l + end s_l) (device
+// strides are at least 8 bytes, so a window is never empty).  A chunk shared
+// by two windows belongs to the first.
+__device__ __forceinline__ Window window_ranges(const ChunkGeom& g, const int32_t* begin, const int32_t* end) {
+    Window w;
+    int64_t prev = 0;
+    w.count = 0;
 #pragma unroll
-    for (int lv = 0; lv < 4; ++lv) {
-        const int lr = 3 - lv;
-        const int b = begin[lr], e = end[lr];
+    for (int i = 0; i < 4; ++i) {  // lr = i, level 3 - i: device windows in increasing offset
+        const int lv = 3 - i;
+        const int b = begin[i], e = end[i];
+        int64_t lo = prev, hi = prev;
         if (b < e) {
-            const int64_t lo = g.dbase[lv] + (int64_t)b * g.dstride[lv];
-            const int64_t hi = g.dbase[lv] + (int64_t)e * g.dstride[lv];
-            if (cp.base < hi && cp.base + cp.osize > lo) sel = true;
+            lo = chunk_of_byte(g, g.dbase[lv] + (int64_t)b * g.dstride[lv]);
+            hi = chunk_of_byte(g, g.dbase[lv] + (int64_t)e * g.dstride[lv] - 1) + 1;
+            lo = lo < prev ? prev : lo;
+            hi = hi < lo ? lo : hi;
+            prev = hi;
         }
+        w.lo[i] = lo;
+        w.hi[i] = hi;
+        w.count += hi - lo;
     }
-    return sel;
+    return w;
 }
 
-// One workgroup: the ids of the selected chunks in chunk order, their count,
-// and the payload bytes they hold (one atomic per call).
-__device__ __forceinline__ void window_select(const uint16_t* __restrict__ sizes, const ChunkGeom& g,
-                                              const int32_t* __restrict__ begin, const int32_t* __restrict__ end,
-                                              int32_t* __restrict__ list, int32_t* __restrict__ count,
-                                              uint64_t* __restrict__ bytes_read) {
-    __shared__ int s_w[16];
-    __shared__ unsigned long long s_b[16];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t n = g.nchunks, per = (n + 1023) / 1024;
-    const int64_t lo = tid * per, hi = lo + per < n ? lo + per : n;
-    int cnt = 0;
-    unsigned long long bytes = 0;
-    for (int64_t i = lo; i < hi; ++i)
-        if (chunk_selected(g, chunk_pos(g, i), begin, end)) {
-            ++cnt;
-            bytes += sizes[i];
-        }
-    int x = cnt;
+// k-th selected chunk (k < count)
+__device__ __forceinline__ int64_t window_chunk(const Window& w, int64_t k) {
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(x, d, 64);
-        if (lane >= d) x += y;
+    for (int i = 0; i < 3; ++i) {
+        const int64_t n = w.hi[i] - w.lo[i];
+        if (k < n) return w.lo[i] + k;
+        k -= n;
     }
-    unsigned long long bsum = bytes;
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) bsum += __shfl_xor(bsum, d, 64);
-    if (lane == 63) s_w[wave] = x;
-    if (lane == 0) s_b[wave] = bsum;
-    __syncthreads();
-    int pos = x - cnt;
-    for (int w = 0; w < wave; ++w) pos += s_w[w];
-    for (int64_t i = lo; i < hi; ++i)
-        if (chunk_selected(g, chunk_pos(g, i), begin, end)) list[pos++] = (int32_t)i;
-    if (tid == 1023) *count = pos;
-    if (tid == 0 && bytes_read) {
-        unsigned long long t = 0;
-        for (int w = 0; w < 16; ++w) t += s_b[w];
-        atomicAdd((unsigned long long*)bytes_read, t);
-    }
+    return w.lo[3] + k;
 }
 
-__global__ __launch_bounds__(1024) void window_select_kernel(const uint16_t* __restrict__ sizes, ChunkGeom g,
-                                                             const int32_t* __restrict__ begin,
-                                                             const int32_t* __restrict__ end,
-                                                             int32_t* __restrict__ list, int32_t* __restrict__ count,
-                                                             uint64_t* __restrict__ bytes_read) {
-    window_select(sizes, g, begin, end, list, count, bytes_read);
+// payload bytes of the selected chunks, from the exclusive offsets
+__device__ __forceinline__ uint64_t window_bytes(const Window& w, const int64_t* offsets, const uint16_t* sizes) {
+    uint64_t t = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (w.hi[i] > w.lo[i]) t += (uint64_t)(offsets[w.hi[i] - 1] + sizes[w.hi[i] - 1] - offsets[w.lo[i]]);
+    return t;
 }
 
 // Decoder: one workgroup per chunk (chunks are independent once the
-// offsets are known).  The windowed form walks the list of selected chunks.
+// offsets are known).  The windowed form walks the selected id ranges.
 struct DecodeLds {
     uint32_t c[kCS / 4 + 2];
     uint64_t b0[256];
@@ -646,54 +641,49 @@ template <bool WINDOW>
 __global__ __launch_bounds__(kThreads) void decode_kernel(const uint8_t* __restrict__ in, ChunkGeom g,
                                                           const int64_t* __restrict__ offsets,
                                                           uint8_t* __restrict__ bp,
-                                                          const int32_t* __restrict__ list,
-                                                          const int32_t* __restrict__ count,
+                                                          const int32_t* __restrict__ begin,
+                                                          const int32_t* __restrict__ end,
+                                                          uint64_t* __restrict__ bytes_read,
                                                           int32_t* __restrict__ err, int64_t in_len) {
     __shared__ DecodeLds S;
     if (!WINDOW) {
         decode_chunk(in, g, offsets, bp, err, in_len, blockIdx.x, S);
         return;
     }
-    const int n = *count;
-    for (int i = blockIdx.x; i < n; i += gridDim.x) {
-        decode_chunk(in, g, offsets, bp, err, in_len, list[i], S);
+    const Window w = window_ranges(g, begin, end);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && bytes_read)
+        *bytes_read += window_bytes(w, offsets, reinterpret_cast<const uint16_t*>(in + 8));
+    for (int64_t k = blockIdx.x; k < w.count; k += gridDim.x) {
+        decode_chunk(in, g, offsets, bp, err, in_len, window_chunk(w, k), S);
         __syncthreads();  // the next chunk reuses the LDS
     }
 }
 
 // ------------------------------------------------- batched retrieval --
-// decode workspace of a job: error and count words | offsets | chunk list
+// decode workspace of a job: error word (64-byte slot) | offsets
 struct DecodeWs {
     int32_t* err;
-    int32_t* count;
     int64_t* offsets;
-    int32_t* list;
 };
 __device__ __forceinline__ DecodeWs decode_ws(const RetrJob& J) {
     DecodeWs w;
     w.err = reinterpret_cast<int32_t*>(J.dws);
-    w.count = w.err + 1;
     w.offsets = reinterpret_cast<int64_t*>(J.dws + 64);
-    w.list = reinterpret_cast<int32_t*>(w.offsets + J.L.nchunks);
     return w;
 }
 
-// One workgroup per job: on the field's first round the error words and the
-// payload offsets, then the chunks of the round's window.
+// One workgroup per job, on the field's first round: the error words and the
+// payload offsets.
 __global__ __launch_bounds__(1024) void retr_prep_kernel(const RetrJob* __restrict__ jobs) {
     const RetrJob& J = jobs[blockIdx.x];
+    if (J.offsets_ready) return;
     const DecodeWs w = decode_ws(J);
-    const uint16_t* sizes = reinterpret_cast<const uint16_t*>(J.stream + 8);
-    if (!J.offsets_ready) {
-        if (threadIdx.x < 16) reinterpret_cast<uint32_t*>(J.dws)[threadIdx.x] = 0u;
-        size_scan(sizes, J.L.nchunks, w.offsets);
-        __syncthreads();
-    }
-    const ChunkGeom g = make_chunk_geom(&J.L);
-    window_select(sizes, g, J.state, J.state + 4, w.list, w.count, J.bytes_read);
+    if (threadIdx.x < 16) reinterpret_cast<uint32_t*>(J.dws)[threadIdx.x] = 0u;
+    size_scan(reinterpret_cast<const uint16_t*>(J.stream + 8), J.L.nchunks, w.offsets);
 }
 
-// exclusive prefix of the per-job chunk counts (one workgroup)
+// exclusive prefix of the per-job selected-chunk counts (one workgroup);
+// each job's payload bytes go to its counter
 __global__ __launch_bounds__(1024) void retr_count_scan_kernel(const RetrJob* __restrict__ jobs, int nj,
                                                                int64_t* __restrict__ prefix) {
     __shared__ int64_t s_w[16];
@@ -701,7 +691,14 @@ __global__ __launch_bounds__(1024) void retr_count_scan_kernel(const RetrJob* __
     const int per = (nj + 1023) / 1024;
     const int lo = tid * per, hi = lo + per < nj ? lo + per : nj;
     int64_t sum = 0;
-    for (int j = lo; j < hi; ++j) sum += *decode_ws(jobs[j]).count;
+    for (int j = lo; j < hi; ++j) {
+        const RetrJob& J = jobs[j];
+        const ChunkGeom g = make_chunk_geom(&J.L);
+        const Window w = window_ranges(g, J.state, J.state + 4);
+        sum += w.count;
+        if (J.bytes_read)
+            *J.bytes_read += window_bytes(w, decode_ws(J).offsets, reinterpret_cast<const uint16_t*>(J.stream + 8));
+    }
     int64_t x = sum;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -714,12 +711,13 @@ __global__ __launch_bounds__(1024) void retr_count_scan_kernel(const RetrJob* __
     for (int w = 0; w < wave; ++w) run += s_w[w];
     for (int j = lo; j < hi; ++j) {
         prefix[j] = run;
-        run += *decode_ws(jobs[j]).count;
+        const RetrJob& J = jobs[j];
+        run += window_ranges(make_chunk_geom(&J.L), J.state, J.state + 4).count;
     }
     if (tid == 1023) prefix[nj] = run;
 }
 
-// the selected chunks of every job, as one list walked by a fixed grid
+// the selected chunks of every job, as one sequence walked by a fixed grid
 __global__ __launch_bounds__(kThreads) void retr_decode_kernel(const RetrJob* __restrict__ jobs, int nj,
                                                                const int64_t* __restrict__ prefix) {
     __shared__ DecodeLds S;
@@ -729,7 +727,8 @@ __global__ __launch_bounds__(kThreads) void retr_decode_kernel(const RetrJob* __
         const RetrJob& J = jobs[j];
         const DecodeWs w = decode_ws(J);
         const ChunkGeom g = make_chunk_geom(&J.L);
-        decode_chunk(J.stream, g, w.offsets, J.bp, w.err, J.stream_len, w.list[i - prefix[j]], S);
+        const Window win = window_ranges(g, J.state, J.state + 4);
+        decode_chunk(J.stream, g, w.offsets, J.bp, w.err, J.stream_len, window_chunk(win, i - prefix[j]), S);
         __syncthreads();  // the next chunk reuses the LDS
     }
 }
@@ -792,10 +791,10 @@ extern "C" int pa_compress_merge(uint8_t* d_out, const pa_layout* L, int dtype, 
     return PA_OK;
 }
 
-// workspace: int32 error word (+pad) | int64 offsets[nchunks]
+// workspace: int32 error word (64-byte slot) | int64 offsets[nchunks]
 extern "C" size_t pa_decode_workspace(const pa_layout* L) {
     if (!L) return 0;
-    return 64 + (size_t)L->nchunks * 8 + ((size_t)L->nchunks * 4 + 63) / 64 * 64;
+    return 64 + (size_t)L->nchunks * 8;
 }
 
 static int decode_common(const uint8_t* d_in, int64_t in_len, const pa_layout* L, const int32_t* d_begin, const int32_t* d_end,
@@ -815,17 +814,12 @@ static int decode_common(const uint8_t* d_in, int64_t in_len, const pa_layout* L
         PA_LAUNCH_CHECK();
     }
     if (window) {
-        int32_t* list = reinterpret_cast<int32_t*>(offsets + L->nchunks);
-        int32_t* count = err + 1;
-        hipLaunchKernelGGL(window_select_kernel, dim3(1), dim3(1024), 0, st,
-                           reinterpret_cast<const uint16_t*>(d_in + 8), g, d_begin, d_end, list, count, d_bytes_read);
-        PA_LAUNCH_CHECK();
         const unsigned grid = (unsigned)(L->nchunks < 2048 ? L->nchunks : 2048);
-        hipLaunchKernelGGL(decode_kernel<true>, dim3(grid), dim3(kThreads), 0, st, d_in, g, offsets, d_bp, list, count,
-                           err, in_len);
+        hipLaunchKernelGGL(decode_kernel<true>, dim3(grid), dim3(kThreads), 0, st, d_in, g, offsets, d_bp, d_begin,
+                           d_end, d_bytes_read, err, in_len);
     } else {
         hipLaunchKernelGGL(decode_kernel<false>, dim3((unsigned)L->nchunks), dim3(kThreads), 0, st, d_in, g, offsets,
-                           d_bp, nullptr, nullptr, err, in_len);
+                           d_bp, nullptr, nullptr, nullptr, err, in_len);
     }
     PA_LAUNCH_CHECK();
     return PA_OK;

@@ -105,7 +105,7 @@ struct RetrJob {
     unsigned long long* keys;  // strategy workgroup keys (33 x 33)
     uint8_t* bp;           // decoded bit-planes (host layout), kept between rounds
     int32_t* qc;           // codes, code_len
-    uint8_t* dws;          // decode workspace: error and count words | offsets | chunk list
+    uint8_t* dws;          // decode workspace: error word (64-byte slot) | offsets
     uint64_t* bytes_read;  // may be null
     const void* old;
     void* out;

@@ -882,7 +882,8 @@ __device__ __forceinline__ void tile_run(const SplineArgs<T>& a, T* tile, int32_
         for (int z = 0; z < OIT; ++z) {
             if (in_xy && t.template inf<2>(t.oz + z)) {
                 const T v = tile[sidx(x, y, z)];
-                dst[z * nxy] = a.old ? v + prev[z] : v;
+                // written once and read only by a later round: no cache allocation
+                __builtin_nontemporal_store(a.old ? v + prev[z] : v, &dst[z * nxy]);
             }
         }
     }
@@ -1064,7 +1065,7 @@ __device__ __forceinline__ void line_tiles(const SplineArgs<T>& a, T* lds, int64
             const int i = lane + it * kThreads;
             if (p0 + i < nx) {
                 const T r = lds[line_slot(i)];
-                a.out[p0 + i] = a.old ? r + prev[it] : r;
+                __builtin_nontemporal_store(a.old ? r + prev[it] : r, &a.out[p0 + i]);
             }
         }
     }
