@@ -16,7 +16,7 @@ struct BatchDev {
     const int64_t* p_ext;    // extrema workgroups
     const int64_t* p_tune;   // tuner sample blocks (x 15 candidates in grid.y)
     const int64_t* p_gap;    // code-gap workgroups
-    const int64_t* p_tile;   // spline work units (a tile; 256 tiles of a 1-D field)
+    const int64_t* p_tile;   // spline work units (a tile; 4 tiles of a 2-D, 256 of a 1-D field)
     const int64_t* p_shuf;   // shuffle workgroups
     const int64_t* p_chunk;  // codec chunks (one per wave64)
     const int64_t* p_merge;  // trailer workgroups

@@ -889,6 +889,184 @@ __device__ __forceinline__ void tile_run(const SplineArgs<T>& a, T* tile, int32_
     }
 }
 
+// ------------------------------------------------------------ 2-D fields --
+// A field with nz = 1 (and ny > 1) holds only the plane z = 0 of each tile.
+// Of the stages only these have points inside it: YELLOW / YELLOW_R
+// (y-lines), HOLLOW / HOLLOW_R (x-lines), and in the multi-dimensional form
+// the LINE points along x and y and the FACE points with z = 0; every z
+// stage is empty.  One wave owns one tile's 17 x 17 plane in LDS (z = 0 of
+// the tile indexing, so the per-point rules above apply unchanged) and runs
+// the stages of each level in the same order, with the same inclusive /
+// exclusive sweeps, over 64 lanes; points of one stage never read each
+// other, so only the order of stages matters and a wave-level fence
+// separates them.  A workgroup holds four tiles.
+constexpr int kPlaneWaves = kThreads / 64;  // tiles per workgroup
+constexpr int kPlaneVals = kTS * kTS;       // 289
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+enum PlaneKind { P_YLINE, P_XLINE, P_FACE };
+
+// Points of one stage of level U on the plane: y-lines at x = 0, S, 2S, ..
+// <= 16 (points y = (2j+1)U), x-lines at y = 0, S, .. (points x = (2j+1)U;
+// y < 16 for the exclusive last stage of level 0), or the face points.
+template <typename T, int U, int KIND, int S, bool EXCL, int M>
+__device__ __forceinline__ void plane_stage(Ctx<T>& t, int lane) {
+    constexpr int N = 16 / (2 * U);
+    constexpr int LINES = EXCL ? 16 : 16 / S + 1;
+    constexpr int TOTAL = KIND == P_FACE ? N * N : LINES * N;
+    constexpr int LEVEL = level_of<U>();
+#pragma unroll
+    for (int it = 0; it < (TOTAL + 63) / 64; ++it) {
+        const int p = lane + 64 * it;
+        if (TOTAL % 64 != 0 && p >= TOTAL) break;
+        int x, y;
+        if constexpr (KIND == P_YLINE) { x = (p / N) * S; y = (2 * (p % N) + 1) * U; }
+        else if constexpr (KIND == P_XLINE) { y = (p / N) * (EXCL ? 1 : S); x = (2 * (p % N) + 1) * U; }
+        else { y = (2 * (p / N) + 1) * U; x = (2 * (p % N) + 1) * U; }
+        if (!(t.ox + x < t.g.nx && t.oy + y < t.g.ny)) continue;
+        T pred;
+        if constexpr (KIND == P_YLINE) pred = pred_1d<T, U, 1>(t, x, y, 0);
+        else if constexpr (KIND == P_XLINE) pred = pred_1d<T, U, 0>(t, x, y, 0);
+        else pred = pred_face<T, U>(t, x, y, 0);
+        T& v = t.s[sidx(x, y, 0)];
+        if constexpr (M == COMPRESS) {
+            const int32_t qv = quantize(t, v, pred);
+            if (x < 16 && y < 16) t.qc_out[lm_index<LEVEL>(t.g, t.ox + x, t.oy + y, 0)] = qv;
+        } else {
+            v = pred + (T)slot_code(v) * t.ebx2;
+        }
+    }
+    wave_sync();
+}
+
+template <typename T, int U, int M>
+__device__ __forceinline__ void plane_level(Ctx<T>& t, int lane, bool md, bool rev) {
+    constexpr bool EX = U == 1;
+    if (md) {  // LINE (x-lines and y-lines at even multiples), then FACE
+        plane_stage<T, U, P_XLINE, 2 * U, false, M>(t, lane);
+        plane_stage<T, U, P_YLINE, 2 * U, false, M>(t, lane);
+        plane_stage<T, U, P_FACE, 2 * U, false, M>(t, lane);
+    } else if (rev) {  // HOLLOW_R, YELLOW_R (BLUE_R is empty)
+        plane_stage<T, U, P_XLINE, 2 * U, false, M>(t, lane);
+        plane_stage<T, U, P_YLINE, U, false, M>(t, lane);
+    } else {  // YELLOW, HOLLOW (BLUE is empty); HOLLOW of level 0 is the exclusive last stage
+        plane_stage<T, U, P_YLINE, 2 * U, false, M>(t, lane);
+        plane_stage<T, U, P_XLINE, U, EX, M>(t, lane);
+    }
+}
+
+// level of plane point (x, y) of a tile: 4 for the corners (anchors)
+__device__ __forceinline__ int plane_point_level(int x, int y) {
+    const int m = x | y | 16;
+    return __builtin_ctz(m) > 4 ? 4 : __builtin_ctz(m);
+}
+
+template <typename T, int M>
+__device__ __forceinline__ void plane_tiles(const SplineArgs<T>& a, T* lds, int64_t unit) {
+    const Geom& g = a.g;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t tile = unit * kPlaneWaves + wave;
+    if (tile >= (int64_t)g.gx * g.gy) return;  // no workgroup barrier follows
+    const int bx = (int)(tile % g.gx), by = (int)(tile / g.gx);
+    Ctx<T> t;
+    t.s = lds + wave * kPlaneVals;
+    t.ox = bx * 16;
+    t.oy = by * 16;
+    t.oz = 0;
+    t.c1x = t.ox + 16 < g.nx;
+    t.c1y = t.oy + 16 < g.ny;
+    t.c1z = false;
+    t.g = g;
+    t.qc_out = a.qc_out;
+    t.qc_in = a.qc_in;
+    t.natural = false;
+    t.acc = 0.0;
+    uint8_t fl[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) fl[i] = a.flags8[i];
+    const int maxlevel = (M == DECOMPRESS && a.d_maxlevel) ? *a.d_maxlevel : 4;
+    constexpr int IT = (kPlaneVals + 63) / 64;
+    if constexpr (M == COMPRESS) {
+        T v[IT];
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int i = lane + 64 * it, x = i % kTS, y = i / kTS;
+            v[it] = (T)0;
+            if (i < kPlaneVals && t.ox + x < g.nx && t.oy + y < g.ny)
+                v[it] = a.data[(int64_t)(t.ox + x) + (int64_t)(t.oy + y) * g.nx];
+        }
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int i = lane + 64 * it;
+            if (i < kPlaneVals) t.s[i] = v[it];
+        }
+        wave_sync();
+        if (lane == 0) a.anchors[bx + (int64_t)by * g.gx] = t.s[0];
+    } else {
+        // every slot: its code (levels up to maxlevel), its anchor, or 0
+        int32_t q[IT];
+        T anc[IT];
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int i = lane + 64 * it, x = i % kTS, y = i / kTS;
+            q[it] = 0;
+            anc[it] = (T)0;
+            if (i < kPlaneVals && t.ox + x < g.nx && t.oy + y < g.ny) {
+                const int lv = plane_point_level(x, y);
+                const int gx = t.ox + x, gy = t.oy + y;
+                if (lv == 4) {
+                    if (a.anchors_in) anc[it] = a.anchors_in[(gx >> 4) + (int64_t)(gy >> 4) * g.gx];
+                } else if (lv <= maxlevel) {
+                    int64_t idx;
+                    switch (lv) {
+                        case 0: idx = lm_index<0>(g, gx, gy, 0); break;
+                        case 1: idx = lm_index<1>(g, gx, gy, 0); break;
+                        case 2: idx = lm_index<2>(g, gx, gy, 0); break;
+                        default: idx = lm_index<3>(g, gx, gy, 0); break;
+                    }
+                    q[it] = t.qc_in[idx];
+                }
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int i = lane + 64 * it;
+            if (i < kPlaneVals) t.s[i] = plane_point_level(i % kTS, i / kTS) == 4 ? anc[it] : code_slot(q[it], T());
+        }
+        wave_sync();
+    }
+    const LevelEb<T>& le = a.le;
+    if (3 <= maxlevel) { t.eb_r = le.eb_r[3]; t.ebx2 = le.ebx2[3]; plane_level<T, 8, M>(t, lane, fl[3], fl[7]); }
+    if (2 <= maxlevel) { t.eb_r = le.eb_r[2]; t.ebx2 = le.ebx2[2]; plane_level<T, 4, M>(t, lane, fl[2], fl[6]); }
+    if (1 <= maxlevel) { t.eb_r = le.eb_r[1]; t.ebx2 = le.ebx2[1]; plane_level<T, 2, M>(t, lane, fl[1], fl[5]); }
+    if (0 <= maxlevel) { t.eb_r = le.eb_r[0]; t.ebx2 = le.ebx2[0]; plane_level<T, 1, M>(t, lane, fl[0], fl[4]); }
+    if constexpr (M == DECOMPRESS) {
+#pragma unroll
+        for (int it = 0; it < 4; ++it) {  // the 16 x 16 own points: lane -> x = lane mod 16, y = lane / 16 + 4 it
+            const int x = lane & 15, y = (lane >> 4) + 4 * it;
+            const int gx = t.ox + x, gy = t.oy + y;
+            if (gx < g.nx && gy < g.ny) {
+                const int64_t o = (int64_t)gx + (int64_t)gy * g.nx;
+                const T v = t.s[sidx(x, y, 0)];
+                __builtin_nontemporal_store(a.old ? v + a.old[o] : v, &a.out[o]);
+            }
+        }
+    }
+}
+
+template <typename T, int M>
+__global__ __launch_bounds__(kThreads) void plane_kernel(SplineArgs<T> a) {
+    __shared__ T lds[kPlaneWaves * kPlaneVals];
+    plane_tiles<T, M>(a, lds, blockIdx.x);
+}
+
+__host__ __device__ inline bool plane_field(const Geom& g) { return g.nz == 1 && g.ny > 1; }
+
 // ------------------------------------------------------------ 1-D fields --
 // A field with ny = nz = 1 holds only the line y = z = 0 of each tile: no
 // point of a y or z stage, and of the multi-dimensional stages only the
@@ -1352,7 +1530,7 @@ __global__ void batch_gaps_kernel(const BatchField* __restrict__ fields, const i
 
 // Compression tiles of fields [lo, hi) (one dtype), with the XCD-aware
 // order of spline_kernel taken over the concatenated list of work units: a
-// tile, or 256 tiles of a 1-D field.
+// tile, 4 tiles of a 2-D field or 256 tiles of a 1-D field.
 template <typename T>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void batch_spline_kernel(
     const BatchField* __restrict__ fields, const int64_t* __restrict__ prefix, int lo, int hi, int64_t base,
@@ -1369,6 +1547,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     const int64_t tix = gt - prefix[f];
     if (line_field(g)) {
         line_tiles<T, COMPRESS>(a, tile, tix);
+        return;
+    }
+    if (plane_field(g)) {
+        plane_tiles<T, COMPRESS>(a, tile, tix);
         return;
     }
     const int bx = (int)(tix % g.gx), by = (int)((tix / g.gx) % g.gy), bz = (int)(tix / ((int64_t)g.gx * g.gy));
@@ -1403,6 +1585,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     const int64_t tix = gt - prefix[j];
     if (line_field(g)) {
         line_tiles<T, DECOMPRESS>(a, tile, tix);
+        return;
+    }
+    if (plane_field(g)) {
+        plane_tiles<T, DECOMPRESS>(a, tile, tix);
         return;
     }
     const int bx = (int)(tix % g.gx), by = (int)((tix / g.gx) % g.gy), bz = (int)(tix / ((int64_t)g.gx * g.gy));
@@ -1443,6 +1629,9 @@ int launch_spline(const pa_layout* L, const T* data, double eb, double rel_eb, c
     const SplineArgs<T> args{data, g, le, flags8, d_maxlevel, qc_out, qc_in, anchors, anchors_in, old, out};
     if (line_field(g))
         hipLaunchKernelGGL((line_kernel<T, M>), dim3((unsigned)((ntiles + kLineTiles - 1) / kLineTiles)),
+                           dim3(kThreads), 0, st, args);
+    else if (plane_field(g))
+        hipLaunchKernelGGL((plane_kernel<T, M>), dim3((unsigned)((ntiles + kPlaneWaves - 1) / kPlaneWaves)),
                            dim3(kThreads), 0, st, args);
     else
         hipLaunchKernelGGL((spline_kernel<T, M>), dim3((unsigned)ntiles), dim3(kThreads), 0, st, args);
@@ -1610,7 +1799,9 @@ int pa::batch_predict(const BatchDev& b, hipStream_t st) {
 int64_t pa::batch_tile_units(const pa_layout* L) {
     const Geom g = make_geom(L);
     const int64_t ntiles = (int64_t)g.gx * g.gy * g.gz;
-    return line_field(g) ? (ntiles + kLineTiles - 1) / kLineTiles : ntiles;
+    if (line_field(g)) return (ntiles + kLineTiles - 1) / kLineTiles;
+    if (plane_field(g)) return (ntiles + kPlaneWaves - 1) / kPlaneWaves;
+    return ntiles;
 }
 
 void pa::retr_level_ebs(RetrJob& j, double eb, double rel_eb) {

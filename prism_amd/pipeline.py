@@ -151,10 +151,19 @@ def compress(data: torch.Tensor, eb: float, rel: bool = True, progressive: bool 
     st = _stream()
     rng = 1.0
     rng_dev = None
+    mm = None
     if rel:
         with tm("extrema"):
             mm = extrema(data)
         rng_dev = (mm[1:2] - mm[0:1]).contiguous()
+    # the tuner needs no bound: it is queued before the host waits for the range
+    flags = torch.empty(8, dtype=torch.uint8, device=data.device)
+    scores = torch.empty(15, dtype=torch.float64, device=data.device)
+    tws = _empty(max(8, lib().pa_autotune_workspace(C.byref(L))))
+    with tm("autotune"):
+        check(lib().pa_autotune(_p(data), dt, C.byref(L), _p(scores), _p(flags), _p(tws), tws.numel(), st),
+              "pa_autotune")
+    if rel:
         lo, hi = mm[:2].tolist()
         rng = hi - lo
         eb_abs, rel_eb = eb * rng, eb
@@ -162,12 +171,6 @@ def compress(data: torch.Tensor, eb: float, rel: bool = True, progressive: bool 
         eb_abs, rel_eb = eb, 0.0
     if not eb_abs > 0:
         raise ValueError("error bound must be positive (constant field in REL mode?)")
-    flags = torch.empty(8, dtype=torch.uint8, device=data.device)
-    scores = torch.empty(15, dtype=torch.float64, device=data.device)
-    tws = _empty(max(8, lib().pa_autotune_workspace(C.byref(L))))
-    with tm("autotune"):
-        check(lib().pa_autotune(_p(data), dt, C.byref(L), _p(scores), _p(flags), _p(tws), tws.numel(), st),
-              "pa_autotune")
     qc = torch.empty(L.code_len, dtype=torch.int32, device=data.device)
     anchors = torch.empty(L.n4, dtype=data.dtype, device=data.device)
     with tm("predict"):

@@ -57,9 +57,10 @@ def S():
 
 
 # (nz, ny, nx); 1-D shapes span one and several 256-tile workgroups of the
-# line-field kernel, with partial last tiles
+# line-field kernel, 2-D shapes one and several 4-tile workgroups of the
+# plane kernel, with partial last tiles
 SHAPES = [(33, 47, 29), (16, 16, 16), (1, 1, 1000), (1, 40, 70), (35, 18, 20), (64, 64, 64), (17, 17, 17),
-          (1, 1, 5), (1, 1, 4097), (1, 1, 12345)]
+          (1, 1, 5), (1, 1, 4097), (1, 1, 12345), (1, 2, 300), (1, 17, 33), (1, 130, 75)]
 FLAGSETS = [[0] * 8, [1] * 4 + [0] * 4, [0] * 4 + [1] * 4, [1, 0, 1, 0, 0, 1, 0, 1]]
 
 
@@ -250,7 +251,7 @@ def test_strategy_parity(pa, oracle):
 
 CONFIGS = [((33, 47, 29), np.float32, 1e-3), ((16, 16, 16), np.float64, 1e-2), ((1, 1, 5000), np.float64, 1e-4),
            ((1, 70, 90), np.float32, 1e-4), ((64, 64, 64), np.float32, 1e-3), ((40, 50, 60), np.float64, 1e-6),
-           ((1, 1, 70001), np.float32, 1e-5)]
+           ((1, 1, 70001), np.float32, 1e-5), ((1, 300, 257), np.float64, 1e-5)]
 
 
 @pytest.mark.parametrize("shape,dtype,rel", CONFIGS)
