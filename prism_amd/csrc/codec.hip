@@ -63,10 +63,11 @@ __device__ __forceinline__ ChunkPos chunk_pos(const ChunkGeom& g, int64_t id) {
     while (i < 3 && id >= g.chunk_start[i + 1]) ++i;
     ChunkPos p;
     p.lv = 3 - i;
-    const int64_t real = id - g.chunk_start[i];
-    const int64_t cb = g.cb[p.lv];
+    // chunk ids and per-plane counts fit 32 bits (nchunks < 2^31)
+    const uint32_t real = (uint32_t)(id - g.chunk_start[i]);
+    const uint32_t cb = (uint32_t)g.cb[p.lv];
     p.plane = (int)(real / cb);
-    p.idx = real % cb;
+    p.idx = real - (uint32_t)p.plane * cb;
     p.base = g.hbase[p.lv] + p.plane * g.e[p.lv] + p.idx * kCS;
     const int64_t rem = g.e[p.lv] - p.idx * kCS;
     p.osize = (int)(rem < kCS ? rem : kCS);

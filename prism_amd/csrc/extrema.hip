@@ -102,7 +102,8 @@ __global__ void extrema_finish_kernel(const unsigned long long* keys, double* ou
 }
 
 // ----------------------------------------------------------------- batch --
-constexpr int64_t kExtSpan = 8 * 256;  // vectors per batched extrema workgroup
+constexpr int64_t kExtBatch = 8 * 256;      // vectors per load batch (8 per lane in flight)
+constexpr int64_t kExtSpan = 8 * kExtBatch;  // vectors per batched extrema workgroup (256 KiB)
 
 __global__ void batch_extrema_init_kernel(BatchField* fields, int nf) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -112,9 +113,9 @@ __global__ void batch_extrema_init_kernel(BatchField* fields, int nf) {
     keys[1] = 0ull;
 }
 
-// Workgroup b of the batch streams kExtSpan 16-byte vectors of its field
-// (the field's tail elements go to its last workgroup); the running keys of
-// each field live in its own descriptor.
+// Workgroup b of the batch streams kExtSpan 16-byte vectors of its field in
+// batches of 8 loads per lane (the field's tail elements go to its last
+// workgroup); the running keys of each field live in its own descriptor.
 template <typename T>
 __device__ void extrema_span(const T* in, int64_t n, int64_t wg, unsigned long long* keys) {
     __shared__ double s_min[4], s_max[4];
@@ -124,21 +125,24 @@ __device__ void extrema_span(const T* in, int64_t n, int64_t wg, unsigned long l
     const Vec* vin = reinterpret_cast<const Vec*>(in);
     double mn = __builtin_inf(), mx = -__builtin_inf();
     const int64_t v0 = wg * kExtSpan;
-    Vec v[8];
+    const int64_t v1 = v0 + kExtSpan < nv ? v0 + kExtSpan : nv;
+    for (int64_t b = v0; b < v1; b += kExtBatch) {
+        Vec v[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const int64_t i = v0 + threadIdx.x + k * 256;
-        if (i < nv) v[k] = vin[i];
-    }
+        for (int k = 0; k < 8; ++k) {
+            const int64_t i = b + threadIdx.x + k * 256;
+            if (i < v1) v[k] = vin[i];
+        }
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const int64_t i = v0 + threadIdx.x + k * 256;
-        if (i < nv) {
-            const T* e = reinterpret_cast<const T*>(&v[k]);
+        for (int k = 0; k < 8; ++k) {
+            const int64_t i = b + threadIdx.x + k * 256;
+            if (i < v1) {
+                const T* e = reinterpret_cast<const T*>(&v[k]);
 #pragma unroll
-            for (int j = 0; j < V; ++j) {
-                mn = fmin(mn, (double)e[j]);
-                mx = fmax(mx, (double)e[j]);
+                for (int j = 0; j < V; ++j) {
+                    mn = fmin(mn, (double)e[j]);
+                    mx = fmax(mx, (double)e[j]);
+                }
             }
         }
     }
