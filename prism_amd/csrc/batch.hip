@@ -76,6 +76,8 @@ int describe(const pa_batch_field& in, BatchField& f) {
     f.out = in.out;
     const int st = pa_layout_init(&f.L, in.nx, in.ny, in.nz);
     if (st != PA_OK) return st;
+    f.cg = pa::make_chunk_geom(&f.L);
+    f.lv = pa::make_levels(&f.L);
     pa::batch_predict_setup(f);
     return PA_OK;
 }
@@ -331,6 +333,8 @@ int retr_describe(const pa_retrieve_job& in, RetrJob& j) {
     int st = pa_layout_init(&j.L, in.nx, in.ny, in.nz);
     if (st != PA_OK) return st;
     j.g = pa::make_geom(&j.L);
+    j.cg = pa::make_chunk_geom(&j.L);
+    j.lv = pa::make_levels(&j.L);
     j.dtype = in.dtype;
     // trailer: flags8 | range | cum[4][32] | anchors (compressor.cu:139-151)
     const int64_t t0 = in.stream_len - (j.L.trailer_fixed + j.L.n4 * in.dtype);

@@ -18,7 +18,8 @@ namespace pa {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kGroupsPerBlock = 4 * kThreads;         // 1024 groups
+constexpr int kGroupsPerBlock = kShuffleGroups;       // 1024 groups
+static_assert(kShuffleGroups == 4 * kThreads, "one group quartet per lane");
 constexpr int kCodesPerBlock = 8 * kGroupsPerBlock;   // 8192 codes
 constexpr int kRow = 33;                              // padded LDS row (32 codes)
 
@@ -40,35 +41,6 @@ __device__ __forceinline__ void transpose32(uint32_t (&a)[32]) {
 // position of code q of a 4-group lane slice inside the transpose input:
 // reversing codes within each group puts code j of a group at bit 7-j.
 __device__ __forceinline__ int slot(int q) { return (q & ~7) | (7 - (q & 7)); }
-
-struct Levels {
-    int64_t prefix[4];     // absolute start of each level's codes
-    int64_t real_cnt[4];   // real codes per level
-    int64_t seg_len[4];    // codes per level segment (pads and phantom tail included)
-    int64_t stride[4];     // s_l
-    int64_t base[4];       // A_l
-    int blk_start[5];      // first workgroup of each level (levels 3,2,1,0 in order)
-    int64_t dev_size, host_size;
-};
-
-__host__ __device__ Levels make_levels(const pa_layout* L) {
-    Levels v;
-    const int64_t cnt[4] = {L->n - L->cnt[0], L->cnt[0] - L->cnt[1], L->cnt[1] - L->cnt[2], L->cnt[2] - L->n4};
-    int b = 0;
-    for (int lv = 3, i = 0; lv >= 0; --lv, ++i) {
-        v.prefix[lv] = L->prefix[lv];
-        v.real_cnt[lv] = cnt[lv];
-        v.seg_len[lv] = L->dev_len[lv];
-        v.stride[lv] = L->dev_stride[lv];
-        v.base[lv] = L->dev_base[lv];
-        v.blk_start[i] = b;
-        b += (int)((L->dev_stride[lv] + kGroupsPerBlock - 1) / kGroupsPerBlock);
-    }
-    v.blk_start[4] = b;
-    v.dev_size = L->dev_size;
-    v.host_size = L->host_size;
-    return v;
-}
 
 __device__ __forceinline__ int block_level(const Levels& v, int blk, int* first) {
     int lv = 3, i = 0;
@@ -132,10 +104,9 @@ __global__ __launch_bounds__(kThreads) void batch_shuffle_kernel(const BatchFiel
     __shared__ int32_t stage[kThreads * kRow];
     const int f = find_field(prefix, 0, nf, blockIdx.x);
     const BatchField& F = fields[f];
-    const Levels v = make_levels(&F.L);
     const int blk = (int)(blockIdx.x - prefix[f]), nblk = (int)(prefix[f + 1] - prefix[f]);
-    if (F.mode == PA_MODE_NB) shuffle_block<PA_MODE_NB>(F.qc, v, F.bp, blk, nblk, stage);
-    else shuffle_block<PA_MODE_SM>(F.qc, v, F.bp, blk, nblk, stage);
+    if (F.mode == PA_MODE_NB) shuffle_block<PA_MODE_NB>(F.qc, F.lv, F.bp, blk, nblk, stage);
+    else shuffle_block<PA_MODE_SM>(F.qc, F.lv, F.bp, blk, nblk, stage);
 }
 
 // Inverse; WINDOW restricts each level to planes [begin[lr], end[lr]) (plus
@@ -196,8 +167,7 @@ __global__ __launch_bounds__(kThreads) void retr_unshuffle_kernel(const RetrJob*
     __shared__ int32_t stage[kThreads * kRow];
     const int j = find_field(prefix, 0, nj, blockIdx.x);
     const RetrJob& J = jobs[j];
-    const Levels v = make_levels(&J.L);
-    unshuffle_block<PA_MODE_NB, true>(J.bp, v, J.state, J.state + 4, J.qc, (int)(blockIdx.x - prefix[j]), stage);
+    unshuffle_block<PA_MODE_NB, true>(J.bp, J.lv, J.state, J.state + 4, J.qc, (int)(blockIdx.x - prefix[j]), stage);
 }
 
 }  // namespace

@@ -21,36 +21,6 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kAvail = kCS - 2;
 
-struct ChunkGeom {
-    int64_t chunk_start[5];  // first chunk id of levels 3,2,1,0 (index 0..3), total at [4]
-    int64_t cb[4];           // chunks per plane, by level
-    int64_t e[4];            // host stride, by level
-    int64_t hbase[4];        // host base, by level
-    int64_t dbase[4];        // device base, by level
-    int64_t dstride[4];      // device stride, by level
-    int64_t nchunks, host_size;
-};
-
-__host__ __device__ ChunkGeom make_chunk_geom(const pa_layout* L) {
-    ChunkGeom c;
-    int64_t id = 0;
-    for (int lv = 3, i = 0; lv >= 0; --lv, ++i) {
-        c.chunk_start[i] = id;
-        id += 32 * L->chunks_per_plane[lv];
-    }
-    c.chunk_start[4] = id;
-    for (int lv = 0; lv < 4; ++lv) {
-        c.cb[lv] = L->chunks_per_plane[lv];
-        c.e[lv] = L->host_stride[lv];
-        c.hbase[lv] = L->host_base[lv];
-        c.dbase[lv] = L->dev_base[lv];
-        c.dstride[lv] = L->dev_stride[lv];
-    }
-    c.nchunks = L->nchunks;
-    c.host_size = L->host_size;
-    return c;
-}
-
 struct ChunkPos {
     int lv, plane;
     int64_t idx, base;
@@ -124,11 +94,25 @@ __global__ void zero_words_kernel(uint64_t* p, int64_t n) {
 
 constexpr int kWaves = 4;  // chunks per workgroup, one per wave64
 
+// LDS of one wave: the zero-elimination bitmap, one step's surviving bytes
+// and the repetition-elimination bytes with the footer.  The surviving bytes
+// go to the chunk's staging slot step by step, so a wave needs 5.4 KiB of
+// LDS (an encoded chunk held whole would need 16 KiB and cap the encoder at
+// two waves per SIMD).
+constexpr int kTailBytes = 2048 + 256 + 32 + 4 + 2;  // RE1 + RE2 + RE3 kept bytes, B3, size
 struct __align__(16) WaveSlot {
-    uint16_t b0[kCS / 16];      // zero-elimination bitmap: 2 bytes per 16-byte segment
-    uint32_t out[kCS / 4 + 4];  // encoded bytes (+ slack read by the funnel-shifted store)
+    uint16_t b0[kCS / 16];           // zero-elimination bitmap: 2 bytes per 16-byte segment
+    uint8_t step[1024];              // surviving bytes of one 1 KiB step
+    uint8_t tail[(kTailBytes + 15) / 16 * 16];
+    int zs[16];                      // surviving bytes per step
 };
 static_assert(sizeof(WaveSlot) % 16 == 0, "slots must keep 16-byte alignment");
+
+// n bytes of LDS to global memory at any byte offset, consecutive lanes on
+// consecutive bytes
+__device__ __forceinline__ void wave_copy_bytes(const uint8_t* src, uint8_t* dst, int n, int lane) {
+    for (int i = lane; i < n; i += 64) dst[i] = src[i];
+}
 
 // One chunk per wave64.  Zero elimination streams the chunk in 16 steps of
 // 1 KiB (lane l owns 16-byte segment 64k+l, so every load is coalesced);
@@ -137,10 +121,10 @@ __device__ __forceinline__ void encode_chunk(const uint8_t* __restrict__ bp, con
                                              uint8_t* __restrict__ dst_stream, uint8_t* __restrict__ staging,
                                              int64_t id, WaveSlot& W) {
     const int lane = threadIdx.x & 63;
-    uint8_t* out = reinterpret_cast<uint8_t*>(W.out);
     const ChunkPos cp = chunk_pos(g, id);
     const int osize = cp.osize;
     const uint8_t* src = bp + cp.base;
+    uint8_t* dst = staging + id * kCS;
 
     // zero elimination; the whole chunk is requested before the first step
     // (16 KiB per wave, 64 registers per lane) so it costs one memory latency
@@ -156,26 +140,58 @@ __device__ __forceinline__ void encode_chunk(const uint8_t* __restrict__ bp, con
             if (b + 8 < osize) hi8[k] = *reinterpret_cast<const uint2*>(src + b + 8);
         }
     }
+    // bitmaps first: the total decides whether any byte is written (a chunk
+    // whose surviving bytes fill it is stored raw)
     int Z = 0;
 #pragma unroll
     for (int k = 0; k < kSteps; ++k) {
-        const int seg = 64 * k + lane;
-        const uint32_t w[4] = {lo8[k].x, lo8[k].y, hi8[k].x, hi8[k].y};
-        const uint32_t m = nz_bytes(w[0]) | (nz_bytes(w[1]) << 4) | (nz_bytes(w[2]) << 8) | (nz_bytes(w[3]) << 12);
-        const int cnt = __popc(m);
-        const int incl = wave_incl_scan(cnt);
-        int pos = Z + incl - cnt;
+        const uint32_t m = nz_bytes(lo8[k].x) | (nz_bytes(lo8[k].y) << 4) | (nz_bytes(hi8[k].x) << 8) |
+                           (nz_bytes(hi8[k].y) << 12);
+        W.b0[64 * k + lane] = (uint16_t)m;
+        int c = __popc(m);
 #pragma unroll
-        for (int i = 0; i < 16; ++i)
-            if ((m >> i) & 1u) out[pos++] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
-        W.b0[seg] = (uint16_t)m;
-        Z += __shfl(incl, 63, 64);
+        for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+        if (lane == 0) W.zs[k] = c;
+        Z += c;
     }
     bool fail = Z >= kAvail;
-    int wpos = Z;
+    wave_sync();
+    if (!fail) {
+        // the surviving bytes, step by step; a step's 16 bytes per lane are
+        // read again (from cache) rather than kept live in registers
+        int zpos = 0;
+#pragma unroll 1
+        for (int k = 0; k < kSteps; ++k) {
+            const int zs = W.zs[k];
+            if (zs == 0) continue;  // wave-uniform
+            const int b = 16 * (64 * k + lane);
+            uint32_t w[4] = {0u, 0u, 0u, 0u};
+            if (b < osize) {
+                const uint2 lo = *reinterpret_cast<const uint2*>(src + b);
+                w[0] = lo.x;
+                w[1] = lo.y;
+                if (b + 8 < osize) {
+                    const uint2 hi = *reinterpret_cast<const uint2*>(src + b + 8);
+                    w[2] = hi.x;
+                    w[3] = hi.y;
+                }
+            }
+            const uint32_t m = W.b0[64 * k + lane];
+            const int cnt = __popc(m);
+            int pos = wave_incl_scan(cnt) - cnt;
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                if ((m >> i) & 1u) W.step[pos++] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
+            wave_sync();
+            wave_copy_bytes(W.step, dst + zpos, zs, lane);
+            wave_sync();  // the next step reuses the buffer
+            zpos += zs;
+        }
+    }
+    int wpos = Z;  // stream position; RE bytes go to tail[wpos - Z]
     uint32_t b3 = 0;
     wave_sync();
-    if (Z != 0) {
+    if (Z != 0 && !fail) {  // (a chunk that already failed is raw whatever follows)
         // round 1: B0 (2048 bytes) -> B1 (256 bytes); lane l owns B0 bytes [32l, 32l+32)
         uint32_t v[8];
         const uint4* b0v = reinterpret_cast<const uint4*>(W.b0) + 2 * lane;
@@ -195,10 +211,10 @@ __device__ __forceinline__ void encode_chunk(const uint8_t* __restrict__ bp, con
         int R = __shfl(incl, 63, 64);
         fail = fail || wpos + R > kAvail;
         if (!fail) {
-            int pos = wpos + incl - cnt;
+            int pos = wpos - Z + incl - cnt;
 #pragma unroll
             for (int i = 0; i < 32; ++i)
-                if ((f1 >> i) & 1u) out[pos++] = (uint8_t)(v[i >> 2] >> (8 * (i & 3)));
+                if ((f1 >> i) & 1u) W.tail[pos++] = (uint8_t)(v[i >> 2] >> (8 * (i & 3)));
         }
         wpos += R;
         // round 2: B1 (256 bytes, lane l holds bytes [4l, 4l+4) = f1) -> B2 (32 bytes)
@@ -210,10 +226,10 @@ __device__ __forceinline__ void encode_chunk(const uint8_t* __restrict__ bp, con
         R = __shfl(incl, 63, 64);
         fail = fail || wpos + R > kAvail;
         if (!fail) {
-            int pos = wpos + incl - cnt;
+            int pos = wpos - Z + incl - cnt;
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-                if ((f2 >> i) & 1u) out[pos++] = (uint8_t)(f1 >> (8 * i));
+                if ((f2 >> i) & 1u) W.tail[pos++] = (uint8_t)(f1 >> (8 * i));
         }
         wpos += R;
         // round 3: B2 byte m = nibbles of lanes 2m, 2m+1 -> B3 (4 bytes)
@@ -225,7 +241,7 @@ __device__ __forceinline__ void encode_chunk(const uint8_t* __restrict__ bp, con
         b3 = (uint32_t)m3;
         R = __popcll(m3);
         fail = fail || wpos + R > kAvail;
-        if (!fail && keep) out[wpos + __popcll(m3 & ((1ull << lane) - 1))] = (uint8_t)b2;
+        if (!fail && keep) W.tail[wpos - Z + __popcll(m3 & ((1ull << lane) - 1))] = (uint8_t)b2;
         wpos += R;
         fail = fail || wpos >= kAvail - 4;
     }
@@ -235,20 +251,15 @@ __device__ __forceinline__ void encode_chunk(const uint8_t* __restrict__ bp, con
     if (!raw && lane == 0) {
         if (Z != 0) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) out[wpos + j] = (uint8_t)(b3 >> (8 * j));
+            for (int j = 0; j < 4; ++j) W.tail[wpos - Z + j] = (uint8_t)(b3 >> (8 * j));
         }
-        out[csize - 2] = (uint8_t)(osize & 0xFF);
-        out[csize - 1] = (uint8_t)((osize >> 8) & 0xFF);
+        W.tail[csize - 2 - Z] = (uint8_t)(osize & 0xFF);
+        W.tail[csize - 1 - Z] = (uint8_t)((osize >> 8) & 0xFF);
     }
-
-    // the encoded chunk goes to its 16 KiB staging slot (16-byte aligned);
-    // a raw chunk is copied later straight from the bit-planes
+    // RE bytes and footer after the zero-eliminated bytes in the staging
+    // slot; a raw chunk is copied later straight from the bit-planes
     wave_sync();
-    if (!raw) {
-        const uint4* sv = reinterpret_cast<const uint4*>(W.out);
-        uint4* dv = reinterpret_cast<uint4*>(staging + id * kCS);
-        for (int w = lane; w < (csize + 15) / 16; w += 64) dv[w] = sv[w];
-    }
+    if (!raw) wave_copy_bytes(W.tail, dst + Z, csize - Z, lane);
     if (lane == 0) reinterpret_cast<uint16_t*>(dst_stream + 8)[id] = (uint16_t)csize;
 }
 
@@ -270,8 +281,7 @@ __global__ __launch_bounds__(64 * kWaves) void batch_encode_kernel(const BatchFi
     if (gid >= total) return;
     const int f = find_field(prefix, 0, nf, gid);
     const BatchField& F = fields[f];
-    const ChunkGeom g = make_chunk_geom(&F.L);
-    encode_chunk(F.bp, g, F.out, F.staging, gid - prefix[f], slots[threadIdx.x >> 6]);
+    encode_chunk(F.bp, F.cg, F.out, F.staging, gid - prefix[f], slots[threadIdx.x >> 6]);
 }
 
 // Payload offsets by one workgroup scan of the size table, then the stream
@@ -317,8 +327,7 @@ __global__ __launch_bounds__(1024) void stream_scan_kernel(const uint16_t* __res
 // one workgroup per field of a batch
 __global__ __launch_bounds__(1024) void batch_scan_kernel(const BatchField* __restrict__ fields) {
     const BatchField& F = fields[blockIdx.x];
-    const ChunkGeom g = make_chunk_geom(&F.L);
-    stream_scan(reinterpret_cast<const uint16_t*>(F.out + 8), g, F.offsets, F.out, F.cum, F.outsize);
+    stream_scan(reinterpret_cast<const uint16_t*>(F.out + 8), F.cg, F.offsets, F.out, F.cum, F.outsize);
 }
 
 // One chunk per wave64: csize bytes from staging (or from the bit-planes for
@@ -363,8 +372,7 @@ __global__ __launch_bounds__(256) void batch_gather_kernel(const BatchField* __r
     if (gid >= total) return;
     const int f = find_field(prefix, 0, nf, gid);
     const BatchField& F = fields[f];
-    const ChunkGeom g = make_chunk_geom(&F.L);
-    gather_chunk(F.bp, F.staging, g, F.offsets, F.out, gid - prefix[f]);
+    gather_chunk(F.bp, F.staging, F.cg, F.offsets, F.out, gid - prefix[f]);
 }
 
 // compress_merge body: zero pad to 8 bytes, then flags, range, size table,
@@ -694,8 +702,7 @@ __global__ __launch_bounds__(1024) void retr_count_scan_kernel(const RetrJob* __
     int64_t sum = 0;
     for (int j = lo; j < hi; ++j) {
         const RetrJob& J = jobs[j];
-        const ChunkGeom g = make_chunk_geom(&J.L);
-        const Window w = window_ranges(g, J.state, J.state + 4);
+        const Window w = window_ranges(J.cg, J.state, J.state + 4);
         sum += w.count;
         if (J.bytes_read)
             *J.bytes_read += window_bytes(w, decode_ws(J).offsets, reinterpret_cast<const uint16_t*>(J.stream + 8));
@@ -713,7 +720,7 @@ __global__ __launch_bounds__(1024) void retr_count_scan_kernel(const RetrJob* __
     for (int j = lo; j < hi; ++j) {
         prefix[j] = run;
         const RetrJob& J = jobs[j];
-        run += window_ranges(make_chunk_geom(&J.L), J.state, J.state + 4).count;
+        run += window_ranges(J.cg, J.state, J.state + 4).count;
     }
     if (tid == 1023) prefix[nj] = run;
 }
@@ -727,9 +734,8 @@ __global__ __launch_bounds__(kThreads) void retr_decode_kernel(const RetrJob* __
         const int j = find_field(prefix, 0, nj, i);
         const RetrJob& J = jobs[j];
         const DecodeWs w = decode_ws(J);
-        const ChunkGeom g = make_chunk_geom(&J.L);
-        const Window win = window_ranges(g, J.state, J.state + 4);
-        decode_chunk(J.stream, g, w.offsets, J.bp, w.err, J.stream_len, window_chunk(win, i - prefix[j]), S);
+        const Window win = window_ranges(J.cg, J.state, J.state + 4);
+        decode_chunk(J.stream, J.cg, w.offsets, J.bp, w.err, J.stream_len, window_chunk(win, i - prefix[j]), S);
         __syncthreads();  // the next chunk reuses the LDS
     }
 }

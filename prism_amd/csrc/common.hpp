@@ -49,10 +49,77 @@ __host__ __device__ inline Geom make_geom(const pa_layout* L) {
     return g;
 }
 
+// ------------------------------------------------------- stage geometry --
+// Chunk grid of the codec: 16 KiB chunks of the host bit-plane layout, in
+// id order levels 3..0, plane, index (zbpc.cu:155-263).
+struct ChunkGeom {
+    int64_t chunk_start[5];  // first chunk id of levels 3,2,1,0 (index 0..3), total at [4]
+    int64_t cb[4];           // chunks per plane, by level
+    int64_t e[4];            // host stride, by level
+    int64_t hbase[4];        // host base, by level
+    int64_t dbase[4];        // device base, by level
+    int64_t dstride[4];      // device stride, by level
+    int64_t nchunks, host_size;
+};
+
+__host__ __device__ inline ChunkGeom make_chunk_geom(const pa_layout* L) {
+    ChunkGeom c;
+    int64_t id = 0;
+    for (int lv = 3, i = 0; lv >= 0; --lv, ++i) {
+        c.chunk_start[i] = id;
+        id += 32 * L->chunks_per_plane[lv];
+    }
+    c.chunk_start[4] = id;
+    for (int lv = 0; lv < 4; ++lv) {
+        c.cb[lv] = L->chunks_per_plane[lv];
+        c.e[lv] = L->host_stride[lv];
+        c.hbase[lv] = L->host_base[lv];
+        c.dbase[lv] = L->dev_base[lv];
+        c.dstride[lv] = L->dev_stride[lv];
+    }
+    c.nchunks = L->nchunks;
+    c.host_size = L->host_size;
+    return c;
+}
+
+// Level segments of the bit-plane shuffle (8-code groups; kShuffleGroups
+// groups per workgroup).
+constexpr int kShuffleGroups = 1024;
+struct Levels {
+    int64_t prefix[4];     // absolute start of each level's codes
+    int64_t real_cnt[4];   // real codes per level
+    int64_t seg_len[4];    // codes per level segment (pads and phantom tail included)
+    int64_t stride[4];     // s_l
+    int64_t base[4];       // A_l
+    int blk_start[5];      // first workgroup of each level (levels 3,2,1,0 in order)
+    int64_t dev_size, host_size;
+};
+
+__host__ __device__ inline Levels make_levels(const pa_layout* L) {
+    Levels v;
+    const int64_t cnt[4] = {L->n - L->cnt[0], L->cnt[0] - L->cnt[1], L->cnt[1] - L->cnt[2], L->cnt[2] - L->n4};
+    int b = 0;
+    for (int lv = 3, i = 0; lv >= 0; --lv, ++i) {
+        v.prefix[lv] = L->prefix[lv];
+        v.real_cnt[lv] = cnt[lv];
+        v.seg_len[lv] = L->dev_len[lv];
+        v.stride[lv] = L->dev_stride[lv];
+        v.base[lv] = L->dev_base[lv];
+        v.blk_start[i] = b;
+        b += (int)((L->dev_stride[lv] + kShuffleGroups - 1) / kShuffleGroups);
+    }
+    v.blk_start[4] = b;
+    v.dev_size = L->dev_size;
+    v.host_size = L->host_size;
+    return v;
+}
+
 // ------------------------------------------------------------------ batch --
 // One field of a batch: its input, its layout, and the device buffers every
 // stage of the compression reads and writes.  The batched kernels find the
-// field of a workgroup in a per-stage prefix of workgroup counts.
+// field of a workgroup in a per-stage prefix of workgroup counts.  The stage
+// geometry is computed on the host: built in a kernel, its arrays would be
+// indexed dynamically from private (scratch) memory.
 struct BatchField {
     const void* data;
     int dtype;     // PA_F32 / PA_F64
@@ -61,7 +128,9 @@ struct BatchField {
     double eb;     // relative or absolute bound
     double alpha;  // level scaling of the relative bound (alpha_of(rel eb); 1 in absolute mode)
     pa_layout L;
-    Geom g;  // make_geom(&L)
+    Geom g;         // make_geom(&L)
+    ChunkGeom cg;   // make_chunk_geom(&L)
+    Levels lv;      // make_levels(&L)
     // auto-tuner sampling (host-computed; tune_ok = 0 leaves all flags false)
     int tune_ok, tsx0, tsy0, tsz0, tnsx, tnsy, tnblocks;
     // code-array ranges the predictor does not write (zeroed)
@@ -88,7 +157,9 @@ struct BatchField {
 // new reconstruction goes.
 struct RetrJob {
     pa_layout L;
-    Geom g;  // make_geom(&L)
+    Geom g;        // make_geom(&L)
+    ChunkGeom cg;  // make_chunk_geom(&L)
+    Levels lv;     // make_levels(&L)
     int dtype;
     int search;            // strategy: 1 = search under target_cost, 0 = load every plane
     uint64_t target_cost;
