@@ -225,7 +225,7 @@ def batched_section(args):
     def plain():
         return [pipeline.compress(t, e, rel=True, progressive=True) for t, e in zip(data, ebs)]
 
-    b = pipeline.BatchCompressor(data, ebs, rel=True, progressive=True, graph=True)
+    b = pipeline.BatchCompressor(data, ebs, rel=True, progressive=True, graph=True, autotune=True)
     be = pipeline.BatchCompressor(data, ebs, rel=True, progressive=True, graph=False)
     ref = [c.stream.cpu() for c in plain()]
     got = b.run()
@@ -260,6 +260,7 @@ def batched_section(args):
             "batched_device_ms": t_dev * 1e3,
             "plain_GBps": total_bytes / t_plain / 1e9, "batched_GBps": total_bytes / t_graph / 1e9,
             "speedup_vs_plain": t_plain / t_graph, "launches_per_batch": b.launches,
+            "launch_params": b.params,
             "shapes_zyx": specs, "dtypes": ["f32" if t.dtype == torch.float32 else "f64" for t in data]}
 
 

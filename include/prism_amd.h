@@ -197,6 +197,16 @@ int pa_batch_plan_rebind(pa_batch_plan* plan, const pa_batch_field* fields, int 
 int pa_batch_run(pa_batch_plan* plan, void* stream);
 /* kernel launches one run issues (graph nodes when captured) */
 int pa_batch_plan_launches(const pa_batch_plan* plan);
+/* Launch-parameter autotuner: times every candidate of each tunable stage
+ * on the plan's own fields (reps timed launches each; synchronises
+ * `stream`), keeps the fastest and re-captures the graph.  Tuned: extrema
+ * load batches per workgroup {2, 4, 8, 16}, spline work order (contiguous
+ * per XCD or launch order), gather chunks per wave {1, 2, 4}.  Streams are
+ * unchanged by any choice. */
+int pa_batch_plan_autotune(pa_batch_plan* plan, int reps, void* stream);
+/* params3 = {extrema batches, XCD order, gather run}; timings9 (may be
+ * NULL) = the autotuner's ms per candidate in that order (4 + 2 + 3) */
+int pa_batch_plan_params(const pa_batch_plan* plan, int32_t* params3, float* timings9);
 int pa_batch_plan_destroy(pa_batch_plan* plan);
 
 /* ---- Batched progressive retrieval (no reference counterpart) ----

@@ -108,6 +108,8 @@ SIGNATURES = {
     "pa_batch_plan_rebind": (I32, [P, P, I32, P]),
     "pa_batch_run": (I32, [P, P]),
     "pa_batch_plan_launches": (I32, [P]),
+    "pa_batch_plan_autotune": (I32, [P, I32, P]),
+    "pa_batch_plan_params": (I32, [P, P, P]),
     "pa_batch_plan_destroy": (I32, [P]),
     "pa_retrieve_table_bytes": (SZ, [I32]),
     "pa_retrieve_workspace": (SZ, [I32]),

@@ -19,6 +19,9 @@ struct pa_batch_plan {
     std::vector<pa::BatchField> host;
     std::vector<int64_t> pre;  // prefix tables (host copy, source of the upload)
     pa::BatchDev dev{};
+    const int64_t* ext_tab[4] = {};  // extrema prefix per ext_batches candidate
+    int64_t ext_n[4] = {};
+    float tuned_ms[9] = {};          // autotuner timings: extrema x4, spline order x2, gather run x3
     uint8_t* table = nullptr;  // descriptors and prefixes inside the workspace
     int launches = 0;
     hipGraphExec_t exec = nullptr;
@@ -57,7 +60,7 @@ FieldSizes field_sizes(const BatchField& f) {
     return z;
 }
 
-constexpr int kPrefixes = 7;  // ext, tune, gap, tile, shuf, chunk, merge
+constexpr int kPrefixes = 10;  // ext (one per ext_batches candidate) x4, tune, gap, tile, shuf, chunk, merge
 
 size_t table_bytes(int nf) { return up((size_t)nf * sizeof(BatchField)) + up((size_t)kPrefixes * (nf + 1) * 8); }
 
@@ -166,7 +169,10 @@ int upload(pa_batch_plan* p, hipStream_t st) {
     for (int s = 0; s < kPrefixes; ++s) P[s] = pre.data() + (size_t)s * (nf + 1);
     for (int k = 0; k < nf; ++k) {
         const BatchField& f = p->host[k];
-        const int64_t cnt[kPrefixes] = {f.rel ? pa::batch_ext_blocks(&f.L, f.dtype) : 0,
+        const int64_t cnt[kPrefixes] = {f.rel ? pa::batch_ext_blocks(&f.L, f.dtype, pa::kExtChoices[0]) : 0,
+                                        f.rel ? pa::batch_ext_blocks(&f.L, f.dtype, pa::kExtChoices[1]) : 0,
+                                        f.rel ? pa::batch_ext_blocks(&f.L, f.dtype, pa::kExtChoices[2]) : 0,
+                                        f.rel ? pa::batch_ext_blocks(&f.L, f.dtype, pa::kExtChoices[3]) : 0,
                                         f.tnblocks,
                                         pa::batch_gap_blocks(f),
                                         pa::batch_tile_units(&f.L),
@@ -181,24 +187,31 @@ int upload(pa_batch_plan* p, hipStream_t st) {
     d.nf = nf;
     d.nf32 = p->nf32;
     const int64_t* dp = reinterpret_cast<const int64_t*>(p->table + desc);
-    d.p_ext = dp;
-    d.p_tune = dp + (nf + 1);
-    d.p_gap = dp + 2 * (nf + 1);
-    d.p_tile = dp + 3 * (nf + 1);
-    d.p_shuf = dp + 4 * (nf + 1);
-    d.p_chunk = dp + 5 * (nf + 1);
-    d.p_merge = dp + 6 * (nf + 1);
-    d.n_ext = P[0][nf];
-    d.n_tune = P[1][nf];
-    d.n_gap = P[2][nf];
-    d.n_tile = P[3][nf];
-    d.n_shuf = P[4][nf];
-    d.n_chunk = P[5][nf];
-    d.n_merge = P[6][nf];
-    d.n_tune32 = P[1][p->nf32];
-    d.n_tile32 = P[3][p->nf32];
+    for (int i = 0; i < 4; ++i) {
+        p->ext_tab[i] = dp + i * (nf + 1);
+        p->ext_n[i] = P[i][nf];
+    }
+    d.ext_batches = pa::kExtChoices[2];  // defaults until the autotuner runs
+    d.gather_run = 1;
+    d.xcd_order = 1;
+    d.p_ext = p->ext_tab[2];
+    d.n_ext = p->ext_n[2];
+    d.p_tune = dp + 4 * (nf + 1);
+    d.p_gap = dp + 5 * (nf + 1);
+    d.p_tile = dp + 6 * (nf + 1);
+    d.p_shuf = dp + 7 * (nf + 1);
+    d.p_chunk = dp + 8 * (nf + 1);
+    d.p_merge = dp + 9 * (nf + 1);
+    d.n_tune = P[4][nf];
+    d.n_gap = P[5][nf];
+    d.n_tile = P[6][nf];
+    d.n_shuf = P[7][nf];
+    d.n_chunk = P[8][nf];
+    d.n_merge = P[9][nf];
+    d.n_tune32 = P[4][p->nf32];
+    d.n_tile32 = P[6][p->nf32];
     const int64_t lim = 0x7fffffff;
-    if (d.n_ext > lim || d.n_tune > lim || d.n_gap > lim || d.n_tile > lim || d.n_shuf > lim ||
+    if (p->ext_n[0] > lim || d.n_tune > lim || d.n_gap > lim || d.n_tile > lim || d.n_shuf > lim ||
         (d.n_chunk + 3) / 4 > lim || d.n_merge > lim)
         return PA_ERR_ARG;
     if (hipMemcpyAsync(p->table, p->host.data(), (size_t)nf * sizeof(BatchField), hipMemcpyHostToDevice, st) !=
@@ -299,6 +312,75 @@ extern "C" int pa_batch_run(pa_batch_plan* p, void* stream) {
 }
 
 extern "C" int pa_batch_plan_launches(const pa_batch_plan* p) { return p ? p->launches : 0; }
+
+// Launch-parameter autotuner: times each candidate of every tunable stage
+// on the plan's own fields (the stages are idempotent once a full run has
+// left their inputs in place), keeps the fastest, and captures the graph
+// again with the choices.
+extern "C" int pa_batch_plan_autotune(pa_batch_plan* p, int reps, void* stream) {
+    if (!p || reps < 1) return PA_ERR_ARG;
+    hipStream_t st = pa::as_stream(stream);
+    int e = enqueue(p, st);
+    if (e != PA_OK) return e;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return PA_ERR_HIP;
+    int status = PA_OK;
+    auto time = [&](int (*launch)(const pa::BatchDev&, hipStream_t)) -> float {
+        if (status != PA_OK) return 0.f;
+        int rc = launch(p->dev, st);  // warm-up
+        if (rc == PA_OK && hipEventRecord(e0, st) != hipSuccess) rc = PA_ERR_HIP;
+        for (int r = 0; r < reps && rc == PA_OK; ++r) rc = launch(p->dev, st);
+        float ms = 0.f;
+        if (rc == PA_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = PA_ERR_HIP;
+        status = rc;
+        return ms / reps;
+    };
+    pa::BatchDev& d = p->dev;
+    int best = 2;
+    for (int i = 0; i < 4; ++i) {
+        d.ext_batches = pa::kExtChoices[i];
+        d.p_ext = p->ext_tab[i];
+        d.n_ext = p->ext_n[i];
+        p->tuned_ms[i] = time(pa::batch_extrema);
+        if (p->tuned_ms[i] < p->tuned_ms[best] || i == 0) best = i;
+    }
+    d.ext_batches = pa::kExtChoices[best];
+    d.p_ext = p->ext_tab[best];
+    d.n_ext = p->ext_n[best];
+    for (int x = 1; x >= 0; --x) {
+        d.xcd_order = x;
+        p->tuned_ms[4 + (1 - x)] = time(pa::batch_spline);
+    }
+    d.xcd_order = p->tuned_ms[4] <= p->tuned_ms[5] ? 1 : 0;
+    best = 0;
+    for (int i = 0; i < 3; ++i) {
+        d.gather_run = pa::kGatherChoices[i];
+        p->tuned_ms[6 + i] = time(pa::batch_gather);
+        if (p->tuned_ms[6 + i] < p->tuned_ms[6 + best]) best = i;
+    }
+    d.gather_run = pa::kGatherChoices[best];
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (status != PA_OK) return status;
+    if (p->exec) {
+        hipGraphExecDestroy(p->exec);
+        p->exec = nullptr;
+        return capture(p);
+    }
+    return PA_OK;
+}
+
+extern "C" int pa_batch_plan_params(const pa_batch_plan* p, int32_t* params3, float* timings9) {
+    if (!p || !params3) return PA_ERR_ARG;
+    params3[0] = p->dev.ext_batches;
+    params3[1] = p->dev.xcd_order;
+    params3[2] = p->dev.gather_run;
+    if (timings9)
+        for (int i = 0; i < 9; ++i) timings9[i] = p->tuned_ms[i];
+    return PA_OK;
+}
 
 extern "C" int pa_batch_plan_destroy(pa_batch_plan* p) {
     if (!p) return PA_OK;

@@ -24,10 +24,17 @@ struct BatchDev {
     // specialised by dtype
     int64_t n_ext, n_tune, n_gap, n_tile, n_shuf, n_chunk, n_merge;
     int64_t n_tune32, n_tile32;
+    // launch parameters (the plan's autotuner may change them)
+    int ext_batches;  // 2048-vector load batches per extrema workgroup (p_ext is built for it)
+    int gather_run;   // chunks per wave in the gather
+    int xcd_order;    // 1: spline work units dealt to XCDs in contiguous ranges; 0: launch order
 };
 
+constexpr int kExtChoices[4] = {2, 4, 8, 16};  // ext_batches candidates
+constexpr int kGatherChoices[3] = {1, 2, 4};   // gather_run candidates
+
 // per-field work counts of each stage (host)
-int64_t batch_ext_blocks(const pa_layout* L, int dtype);
+int64_t batch_ext_blocks(const pa_layout* L, int dtype, int ext_batches);
 int64_t batch_gap_blocks(const BatchField& f);
 int64_t batch_shuf_blocks(const pa_layout* L);
 int64_t batch_tile_units(const pa_layout* L);  // spline workgroups
@@ -58,5 +65,7 @@ int batch_extrema(const BatchDev& b, hipStream_t st);  // extrema.hip
 int batch_predict(const BatchDev& b, hipStream_t st);  // predictor.hip: eb, tuner, code gaps, spline
 int batch_shuffle(const BatchDev& b, hipStream_t st);  // bitplane.hip
 int batch_encode(const BatchDev& b, hipStream_t st);   // codec.hip: encode, scan, gather, trailer
+int batch_spline(const BatchDev& b, hipStream_t st);   // the spline launches of batch_predict
+int batch_gather(const BatchDev& b, hipStream_t st);   // the gather launch of batch_encode
 
 }  // namespace pa

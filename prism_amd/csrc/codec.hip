@@ -366,13 +366,18 @@ __global__ __launch_bounds__(256) void gather_kernel(const uint8_t* __restrict__
     gather_chunk(bp, staging, g, offsets, dst_stream, id);
 }
 
+// `run` consecutive chunks per wave
 __global__ __launch_bounds__(256) void batch_gather_kernel(const BatchField* __restrict__ fields,
-                                                           const int64_t* __restrict__ prefix, int nf, int64_t total) {
-    const int64_t gid = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (gid >= total) return;
-    const int f = find_field(prefix, 0, nf, gid);
-    const BatchField& F = fields[f];
-    gather_chunk(F.bp, F.staging, F.cg, F.offsets, F.out, gid - prefix[f]);
+                                                           const int64_t* __restrict__ prefix, int nf, int64_t total,
+                                                           int run) {
+    const int64_t gid0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * run;
+    for (int r = 0; r < run; ++r) {
+        const int64_t gid = gid0 + r;
+        if (gid >= total) return;
+        const int f = find_field(prefix, 0, nf, gid);
+        const BatchField& F = fields[f];
+        gather_chunk(F.bp, F.staging, F.cg, F.offsets, F.out, gid - prefix[f]);
+    }
 }
 
 // compress_merge body: zero pad to 8 bytes, then flags, range, size table,
@@ -861,12 +866,18 @@ int pa::batch_encode(const BatchDev& b, hipStream_t st) {
     }
     hipLaunchKernelGGL(batch_scan_kernel, dim3((unsigned)b.nf), dim3(1024), 0, st, b.fields);
     PA_LAUNCH_CHECK();
-    if (b.n_chunk > 0) {
-        hipLaunchKernelGGL(batch_gather_kernel, dim3((unsigned)((b.n_chunk + 3) / 4)), dim3(256), 0, st, b.fields,
-                           b.p_chunk, b.nf, b.n_chunk);
-        PA_LAUNCH_CHECK();
-    }
+    int e = batch_gather(b, st);
+    if (e != PA_OK) return e;
     hipLaunchKernelGGL(batch_merge_kernel, dim3((unsigned)b.n_merge), dim3(256), 0, st, b.fields, b.p_merge, b.nf);
+    PA_LAUNCH_CHECK();
+    return PA_OK;
+}
+
+int pa::batch_gather(const BatchDev& b, hipStream_t st) {
+    if (b.n_chunk <= 0) return PA_OK;
+    const int64_t per = 4 * (int64_t)b.gather_run;
+    hipLaunchKernelGGL(batch_gather_kernel, dim3((unsigned)((b.n_chunk + per - 1) / per)), dim3(256), 0, st, b.fields,
+                       b.p_chunk, b.nf, b.n_chunk, b.gather_run);
     PA_LAUNCH_CHECK();
     return PA_OK;
 }

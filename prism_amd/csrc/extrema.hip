@@ -102,8 +102,7 @@ __global__ void extrema_finish_kernel(const unsigned long long* keys, double* ou
 }
 
 // ----------------------------------------------------------------- batch --
-constexpr int64_t kExtBatch = 8 * 256;      // vectors per load batch (8 per lane in flight)
-constexpr int64_t kExtSpan = 8 * kExtBatch;  // vectors per batched extrema workgroup (256 KiB)
+constexpr int64_t kExtBatch = 8 * 256;  // vectors per load batch (8 per lane in flight)
 
 __global__ void batch_extrema_init_kernel(BatchField* fields, int nf) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -113,19 +112,19 @@ __global__ void batch_extrema_init_kernel(BatchField* fields, int nf) {
     keys[1] = 0ull;
 }
 
-// Workgroup b of the batch streams kExtSpan 16-byte vectors of its field in
+// Workgroup b of the batch streams `span` 16-byte vectors of its field in
 // batches of 8 loads per lane (the field's tail elements go to its last
 // workgroup); the running keys of each field live in its own descriptor.
 template <typename T>
-__device__ void extrema_span(const T* in, int64_t n, int64_t wg, unsigned long long* keys) {
+__device__ void extrema_span(const T* in, int64_t n, int64_t wg, int64_t span, unsigned long long* keys) {
     __shared__ double s_min[4], s_max[4];
     constexpr int V = 16 / sizeof(T);
     using Vec = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
     const int64_t nv = n / V;
     const Vec* vin = reinterpret_cast<const Vec*>(in);
     double mn = __builtin_inf(), mx = -__builtin_inf();
-    const int64_t v0 = wg * kExtSpan;
-    const int64_t v1 = v0 + kExtSpan < nv ? v0 + kExtSpan : nv;
+    const int64_t v0 = wg * span;
+    const int64_t v1 = v0 + span < nv ? v0 + span : nv;
     for (int64_t b = v0; b < v1; b += kExtBatch) {
         Vec v[8];
 #pragma unroll
@@ -146,7 +145,7 @@ __device__ void extrema_span(const T* in, int64_t n, int64_t wg, unsigned long l
             }
         }
     }
-    if (v0 + kExtSpan >= nv) {  // last workgroup of the field: scalar tail
+    if (v0 + span >= nv) {  // last workgroup of the field: scalar tail
         for (int64_t i = nv * V + threadIdx.x; i < n; i += 256) {
             mn = fmin(mn, (double)in[i]);
             mx = fmax(mx, (double)in[i]);
@@ -173,13 +172,14 @@ __device__ void extrema_span(const T* in, int64_t n, int64_t wg, unsigned long l
     }
 }
 
-__global__ __launch_bounds__(256) void batch_extrema_kernel(BatchField* fields, const int64_t* prefix, int nf) {
+__global__ __launch_bounds__(256) void batch_extrema_kernel(BatchField* fields, const int64_t* prefix, int nf,
+                                                            int64_t span) {
     const int f = find_field(prefix, 0, nf, blockIdx.x);
     const BatchField& F = fields[f];
     unsigned long long* keys = reinterpret_cast<unsigned long long*>(F.minmax + 2);
     const int64_t wg = blockIdx.x - prefix[f];
-    if (F.dtype == PA_F32) extrema_span<float>((const float*)F.data, F.L.n, wg, keys);
-    else extrema_span<double>((const double*)F.data, F.L.n, wg, keys);
+    if (F.dtype == PA_F32) extrema_span<float>((const float*)F.data, F.L.n, wg, span, keys);
+    else extrema_span<double>((const double*)F.data, F.L.n, wg, span, keys);
 }
 
 __global__ void batch_extrema_finish_kernel(BatchField* fields, int nf) {
@@ -221,9 +221,9 @@ extern "C" int pa_device_count(void) {
     return n;
 }
 
-int64_t pa::batch_ext_blocks(const pa_layout* L, int dtype) {
-    const int64_t nv = L->n / (16 / dtype);  // 16-byte vectors of the field
-    return nv / kExtSpan + 1;                // the last workgroup also takes the scalar tail
+int64_t pa::batch_ext_blocks(const pa_layout* L, int dtype, int ext_batches) {
+    const int64_t nv = L->n / (16 / dtype);      // 16-byte vectors of the field
+    return nv / (ext_batches * kExtBatch) + 1;  // the last workgroup also takes the scalar tail
 }
 
 int pa::batch_extrema(const BatchDev& b, hipStream_t st) {
@@ -231,7 +231,8 @@ int pa::batch_extrema(const BatchDev& b, hipStream_t st) {
     hipLaunchKernelGGL(batch_extrema_init_kernel, dim3(fb), dim3(256), 0, st, b.fields, b.nf);
     PA_LAUNCH_CHECK();
     if (b.n_ext > 0) {
-        hipLaunchKernelGGL(batch_extrema_kernel, dim3((unsigned)b.n_ext), dim3(256), 0, st, b.fields, b.p_ext, b.nf);
+        hipLaunchKernelGGL(batch_extrema_kernel, dim3((unsigned)b.n_ext), dim3(256), 0, st, b.fields, b.p_ext, b.nf,
+                           (int64_t)b.ext_batches * kExtBatch);
         PA_LAUNCH_CHECK();
     }
     hipLaunchKernelGGL(batch_extrema_finish_kernel, dim3(fb), dim3(256), 0, st, b.fields, b.nf);

@@ -1534,11 +1534,11 @@ __global__ void batch_gaps_kernel(const BatchField* __restrict__ fields, const i
 template <typename T>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void batch_spline_kernel(
     const BatchField* __restrict__ fields, const int64_t* __restrict__ prefix, int lo, int hi, int64_t base,
-    int64_t ntiles) {
+    int64_t ntiles, int xcd_order) {
     __shared__ T tile[kTileVals];
     __shared__ int32_t rowtab[1];
     const int64_t b = blockIdx.x, xcd = b % kXcds, per = ntiles / kXcds, extra = ntiles % kXcds;
-    const int64_t gt = base + xcd * per + (xcd < extra ? xcd : extra) + b / kXcds;
+    const int64_t gt = base + (xcd_order ? xcd * per + (xcd < extra ? xcd : extra) + b / kXcds : b);
     const int f = find_field(prefix, lo, hi, gt);
     const BatchField& F = fields[f];
     const SplineArgs<T> a{static_cast<const T*>(F.data), F.g, load_level_ebs<T>(F.ebinfo), F.flags,
@@ -1783,14 +1783,18 @@ int pa::batch_predict(const BatchDev& b, hipStream_t st) {
         hipLaunchKernelGGL(batch_gaps_kernel, dim3((unsigned)b.n_gap), dim3(256), 0, st, b.fields, b.p_gap, b.nf);
         PA_LAUNCH_CHECK();
     }
+    return batch_spline(b, st);
+}
+
+int pa::batch_spline(const BatchDev& b, hipStream_t st) {
     if (b.n_tile32 > 0) {
         hipLaunchKernelGGL(batch_spline_kernel<float>, dim3((unsigned)b.n_tile32), dim3(kThreads), 0, st, b.fields,
-                           b.p_tile, 0, b.nf32, (int64_t)0, b.n_tile32);
+                           b.p_tile, 0, b.nf32, (int64_t)0, b.n_tile32, b.xcd_order);
         PA_LAUNCH_CHECK();
     }
     if (b.n_tile - b.n_tile32 > 0) {
         hipLaunchKernelGGL(batch_spline_kernel<double>, dim3((unsigned)(b.n_tile - b.n_tile32)), dim3(kThreads), 0, st,
-                           b.fields, b.p_tile, b.nf32, b.nf, b.n_tile32, b.n_tile - b.n_tile32);
+                           b.fields, b.p_tile, b.nf32, b.nf, b.n_tile32, b.n_tile - b.n_tile32, b.xcd_order);
         PA_LAUNCH_CHECK();
     }
     return PA_OK;

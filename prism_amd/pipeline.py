@@ -210,7 +210,8 @@ class BatchCompressor:
     `fields`; run() accepts new tensors and bounds of the same kinds.  The
     returned streams are views of a buffer that the next run() reuses."""
 
-    def __init__(self, fields, ebs, rel: bool = True, progressive: bool = True, graph: bool = True):
+    def __init__(self, fields, ebs, rel: bool = True, progressive: bool = True, graph: bool = True,
+                 autotune: bool = False):
         if not fields:
             raise ValueError("empty batch")
         self.rel, self.progressive = rel, progressive
@@ -244,6 +245,17 @@ class BatchCompressor:
                                          _stream(), C.byref(plan)), "pa_batch_plan_create")
         self.plan = plan
         self.graph = graph
+        if autotune:
+            check(lib().pa_batch_plan_autotune(plan, 3, _stream()), "pa_batch_plan_autotune")
+
+    @property
+    def params(self) -> dict:
+        """Launch parameters in use and the autotuner's timings (ms)."""
+        p = (C.c_int32 * 3)()
+        t = (C.c_float * 9)()
+        check(lib().pa_batch_plan_params(self.plan, p, t), "pa_batch_plan_params")
+        return {"extrema_batches": p[0], "xcd_order": p[1], "gather_run": p[2],
+                "tuned_ms": {"extrema": list(t[0:4]), "spline_order": list(t[4:6]), "gather": list(t[6:9])}}
 
     def _descs(self, fields, ebs):
         if len(fields) != self.nf:

@@ -43,14 +43,19 @@ def assert_same(got, want, label):
         assert torch.equal(gs, w), f"{label}: field {i} differs at byte {int((gs != w).nonzero()[0])}"
 
 
-@pytest.mark.parametrize("graph", [False, True])
-def test_mixed_batch_equals_plain(pl, graph):
-    announce("batch mixed 12 fields, graph", graph)
+@pytest.mark.parametrize("graph,autotune", [(False, False), (True, False), (True, True)])
+def test_mixed_batch_equals_plain(pl, graph, autotune):
+    announce("batch mixed 12 fields, graph", graph, "autotune", autotune)
     data, ebs = mixed_fields(12, 300, max3d=80, max2d=300, max1d=2e5)
     want = plain_streams(pl, data, ebs)
-    b = pl.BatchCompressor(data, ebs, rel=True, progressive=True, graph=graph)
+    b = pl.BatchCompressor(data, ebs, rel=True, progressive=True, graph=graph, autotune=autotune)
     assert b.launches <= 15
-    assert_same(b.run(), want, f"graph={graph}")
+    assert_same(b.run(), want, f"graph={graph} autotune={autotune}")
+    p = b.params
+    assert p["extrema_batches"] in (2, 4, 8, 16) and p["xcd_order"] in (0, 1) and p["gather_run"] in (1, 2, 4)
+    if autotune:
+        assert all(t > 0 for t in p["tuned_ms"]["extrema"] + p["tuned_ms"]["gather"])
+        assert_same(b.run(), want, "replay after tuning")
 
 
 def test_edge_shapes_and_modes(pl):
