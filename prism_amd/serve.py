@@ -77,7 +77,9 @@ class Server:
     (stream, decoded bit-planes, codes, current reconstruction) plus the
     transient memory of admitted requests (new reconstructions, compression
     workspaces and outputs) until they complete.  max_inflight batches may be
-    queued on the stream at once; max_batch requests form one batch."""
+    queued on the stream at once; max_batch requests form one batch, chosen
+    among the first 4 x max_batch queued (a field's later rounds wait their
+    turn)."""
 
     def __init__(self, budget_bytes: int, max_batch: int = 64, max_inflight: int = 2, clock=time.perf_counter):
         self.budget = int(budget_bytes)
@@ -161,7 +163,8 @@ class Server:
         batch, need, seen = [], 0, set()
         room = self.budget - self.used
         keep = deque()
-        while self.queue and len(batch) < self.max_batch:
+        lookahead = 4 * self.max_batch  # queued requests examined per batch
+        while self.queue and len(batch) < self.max_batch and len(batch) + len(keep) < lookahead:
             r = self.queue.popleft()
             if r.kind == "retrieve" and r.field in seen:
                 keep.append(r)  # one round per field per batch; the next batch takes it
