@@ -80,6 +80,20 @@ int pa_spline_construct(const void* d_data, int dtype, const pa_layout* L, doubl
                         double rel_eb, const uint8_t* d_flags8, int32_t* d_qc, void* d_anchors,
                         void* stream);
 
+/* The host steps of the bound (eb = rel_eb * range, prism.cu:16-19; calc_eb,
+ * InterpolationPredictor.cu:1825-1839) on the device, so that a compression
+ * needs no host round trip before its last stage.  d_minmax: the result of
+ * pa_extrema for a relative bound (eb relative), NULL for an absolute eb.
+ * d_bounds (double[16]) = { absolute eb, range (1 for an absolute eb),
+ * eb_r[4], ebx2[4] (float64), eb_r[4] | ebx2[4] (float32, packed in 4
+ * doubles), status (0, or PA_ERR_ARG when the bound is not positive, e.g.
+ * a constant field: the stages then run on a placeholder bound of 1), 0 }.
+ * The values equal the host computation's. */
+int pa_level_bounds(const double* d_minmax, double eb, int dtype, double* d_bounds, void* stream);
+/* pa_spline_construct with the bounds of pa_level_bounds */
+int pa_spline_construct_dev(const void* d_data, int dtype, const pa_layout* L, const double* d_bounds,
+                            const uint8_t* d_flags8, int32_t* d_qc, void* d_anchors, void* stream);
+
 /* spline_reconstruct (InterpolationPredictor.cu:2372-2421).  d_maxlevel is a
  * device int (levels with level_id > *d_maxlevel are skipped) or NULL for 4. */
 int pa_spline_reconstruct(const int32_t* d_qc, const void* d_anchors, int dtype,

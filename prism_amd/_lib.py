@@ -88,6 +88,8 @@ SIGNATURES = {
     "pa_autotune_workspace": (SZ, [LP]),
     "pa_autotune": (I32, [P, I32, LP, P, P, P, SZ, P]),
     "pa_spline_construct": (I32, [P, I32, LP, D, D, P, P, P, P]),
+    "pa_level_bounds": (I32, [P, D, I32, P, P]),
+    "pa_spline_construct_dev": (I32, [P, I32, LP, P, P, P, P, P]),
     "pa_spline_reconstruct": (I32, [P, P, I32, LP, D, D, P, P, P, P]),
     "pa_spline_delta_reconstruct": (I32, [P, I32, LP, D, D, P, P, P, P, P]),
     "pa_bitshuffle": (I32, [P, LP, I32, P, P]),

@@ -755,6 +755,26 @@ struct LevelEb {
     T eb_r[4], ebx2[4];
 };
 
+// Level bounds from a device block laid out as {absolute eb, range,
+// eb_r[4] and ebx2[4] in float64, then eb_r[4] and ebx2[4] in float32}.
+template <typename T>
+__device__ __forceinline__ LevelEb<T> load_level_ebs(const double* e) {
+    LevelEb<T> le;
+    if constexpr (sizeof(T) == 4) {
+        const float* q = reinterpret_cast<const float*>(e + 10);
+        for (int i = 0; i < 4; ++i) {
+            le.eb_r[i] = q[i];
+            le.ebx2[i] = q[4 + i];
+        }
+    } else {
+        for (int i = 0; i < 4; ++i) {
+            le.eb_r[i] = e[2 + i];
+            le.ebx2[i] = e[6 + i];
+        }
+    }
+    return le;
+}
+
 // Tile plus halo into LDS: all global loads of a lane are issued before the
 // first LDS write so the tile costs one memory latency.
 template <typename T, bool IN>
@@ -783,6 +803,7 @@ struct SplineArgs {
     const T* data;
     Geom g;
     LevelEb<T> le;
+    const double* le_dev;  // when set, the level bounds come from this device block instead of `le`
     const uint8_t* flags8;
     const int32_t* d_maxlevel;
     int32_t* qc_out;
@@ -1059,10 +1080,17 @@ __device__ __forceinline__ void plane_tiles(const SplineArgs<T>& a, T* lds, int6
     }
 }
 
+template <typename T>
+__device__ __forceinline__ SplineArgs<T> resolve(const SplineArgs<T>& a) {
+    SplineArgs<T> r = a;
+    if (a.le_dev) r.le = load_level_ebs<T>(a.le_dev);
+    return r;
+}
+
 template <typename T, int M>
 __global__ __launch_bounds__(kThreads) void plane_kernel(SplineArgs<T> a) {
     __shared__ T lds[kPlaneWaves * kPlaneVals];
-    plane_tiles<T, M>(a, lds, blockIdx.x);
+    plane_tiles<T, M>(resolve(a), lds, blockIdx.x);
 }
 
 __host__ __device__ inline bool plane_field(const Geom& g) { return g.nz == 1 && g.ny > 1; }
@@ -1252,7 +1280,7 @@ __device__ __forceinline__ void line_tiles(const SplineArgs<T>& a, T* lds, int64
 template <typename T, int M>
 __global__ __launch_bounds__(kThreads) void line_kernel(SplineArgs<T> a) {
     __shared__ T lds[kLineSlots];
-    line_tiles<T, M>(a, lds, blockIdx.x);
+    line_tiles<T, M>(resolve(a), lds, blockIdx.x);
 }
 
 __host__ __device__ inline bool line_field(const Geom& g) { return g.ny == 1 && g.nz == 1; }
@@ -1276,8 +1304,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     const int64_t tix = xcd * per + (xcd < extra ? xcd : extra) + b / kXcds;
     const int bx = (int)(tix % g.gx), by = (int)((tix / g.gx) % g.gy), bz = (int)(tix / ((int64_t)g.gx * g.gy));
     const bool interior = bx * 16 + 16 < g.nx && by * 16 + 16 < g.ny && bz * 16 + 16 < g.nz;
-    if (interior) tile_run<T, M, true>(a, tile, rowtab, bx, by, bz);
-    else tile_run<T, M, false>(a, tile, rowtab, bx, by, bz);
+    const SplineArgs<T> r = resolve(a);
+    if (interior) tile_run<T, M, true>(r, tile, rowtab, bx, by, bz);
+    else tile_run<T, M, false>(r, tile, rowtab, bx, by, bz);
 }
 
 // Zero the code-array entries the predictor does not write: the anchor
@@ -1442,59 +1471,55 @@ LevelEb<T> level_ebs(double eb, double rel_eb) {
 // Per field: the absolute bound from the value range (REL) and its level
 // bounds; a bound that is not positive (a constant field in REL mode) marks
 // the field failed and the stages run on a placeholder bound of 1.
-__global__ void batch_eb_kernel(BatchField* fields, int nf) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nf) return;
-    BatchField& F = fields[f];
-    double range = 1.0, eb = F.eb;
-    if (F.rel) {
-        range = F.minmax[1] - F.minmax[0];
-        eb = F.eb * range;
+// e = {absolute eb, range, eb_r[4], ebx2[4] (float64), eb_r[4] | ebx2[4]
+// (float32)}; returns the status (PA_ERR_ARG for a bound that is not
+// positive, the stages then running on a placeholder bound of 1)
+__device__ int64_t fill_bounds(const double* minmax, double eb_in, int rel, int dtype, double alpha, double* e) {
+    double range = 1.0, eb = eb_in;
+    if (rel) {
+        range = minmax[1] - minmax[0];
+        eb = eb_in * range;
     }
-    uint64_t status = PA_OK;
+    int64_t status = PA_OK;
     if (!(eb > 0)) {
-        status = (uint64_t)(int64_t)PA_ERR_ARG;
+        status = PA_ERR_ARG;
         eb = 1.0;
     }
-    double* e = F.ebinfo;
     e[0] = eb;
     e[1] = range;
-    if (F.dtype == PA_F32) {
-        const LevelEb<float> le = level_ebs_alpha<float>(eb, F.alpha);
+    if (dtype == PA_F32) {
+        const LevelEb<float> le = level_ebs_alpha<float>(eb, alpha);
         float* q = reinterpret_cast<float*>(e + 10);
         for (int i = 0; i < 4; ++i) {
             q[i] = le.eb_r[i];
             q[4 + i] = le.ebx2[i];
         }
     } else {
-        const LevelEb<double> le = level_ebs_alpha<double>(eb, F.alpha);
+        const LevelEb<double> le = level_ebs_alpha<double>(eb, alpha);
         for (int i = 0; i < 4; ++i) {
             e[2 + i] = le.eb_r[i];
             e[6 + i] = le.ebx2[i];
         }
     }
-    F.result[1] = status;
-    F.result[2] = (uint64_t)__double_as_longlong(range);
-    F.result[3] = (uint64_t)__double_as_longlong(eb);
+    return status;
 }
 
-template <typename T>
-__device__ __forceinline__ LevelEb<T> load_level_ebs(const double* e) {
-    LevelEb<T> le;
-    if constexpr (sizeof(T) == 4) {
-        const float* q = reinterpret_cast<const float*>(e + 10);
-        for (int i = 0; i < 4; ++i) {
-            le.eb_r[i] = q[i];
-            le.ebx2[i] = q[4 + i];
-        }
-    } else {
-        for (int i = 0; i < 4; ++i) {
-            le.eb_r[i] = e[2 + i];
-            le.ebx2[i] = e[6 + i];
-        }
-    }
-    return le;
+__global__ void batch_eb_kernel(BatchField* fields, int nf) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    BatchField& F = fields[f];
+    const int64_t status = fill_bounds(F.minmax, F.eb, F.rel, F.dtype, F.alpha, F.ebinfo);
+    F.result[1] = (uint64_t)status;
+    F.result[2] = (uint64_t)__double_as_longlong(F.ebinfo[1]);
+    F.result[3] = (uint64_t)__double_as_longlong(F.ebinfo[0]);
 }
+
+__global__ void level_bounds_kernel(const double* minmax, double eb, int dtype, double alpha, double* bounds) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    bounds[14] = (double)fill_bounds(minmax, eb, minmax != nullptr, dtype, alpha, bounds);
+    bounds[15] = 0.0;
+}
+
 
 // tuner over the sample blocks of fields [lo, hi) (one dtype); grid.y = candidate
 template <typename T>
@@ -1541,7 +1566,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     const int64_t gt = base + (xcd_order ? xcd * per + (xcd < extra ? xcd : extra) + b / kXcds : b);
     const int f = find_field(prefix, lo, hi, gt);
     const BatchField& F = fields[f];
-    const SplineArgs<T> a{static_cast<const T*>(F.data), F.g, load_level_ebs<T>(F.ebinfo), F.flags,
+    const SplineArgs<T> a{static_cast<const T*>(F.data), F.g, load_level_ebs<T>(F.ebinfo), nullptr, F.flags,
                           nullptr, F.qc, nullptr, static_cast<T*>(F.anchors), nullptr, nullptr, nullptr};
     const Geom& g = a.g;
     const int64_t tix = gt - prefix[f];
@@ -1578,7 +1603,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         le.eb_r[i] = sizeof(T) == 4 ? (T)J.le32[i] : (T)J.le64[i];
         le.ebx2[i] = sizeof(T) == 4 ? (T)J.le32[4 + i] : (T)J.le64[4 + i];
     }
-    const SplineArgs<T> a{nullptr, J.g, le, J.flags, J.state + 8, nullptr, J.qc, nullptr,
+    const SplineArgs<T> a{nullptr, J.g, le, nullptr, J.flags, J.state + 8, nullptr, J.qc, nullptr,
                           J.first ? static_cast<const T*>(J.anchors) : nullptr,
                           J.first ? nullptr : static_cast<const T*>(J.old), static_cast<T*>(J.out)};
     const Geom& g = a.g;
@@ -1621,12 +1646,12 @@ int gap_blocks(const Gaps& gaps) {
 template <typename T, int M>
 int launch_spline(const pa_layout* L, const T* data, double eb, double rel_eb, const uint8_t* flags8,
                   const int32_t* d_maxlevel, int32_t* qc_out, const int32_t* qc_in, T* anchors,
-                  const T* anchors_in, const T* old, T* out, hipStream_t st) {
+                  const T* anchors_in, const T* old, T* out, hipStream_t st, const double* le_dev = nullptr) {
     Geom g = make_geom(L);
     const int64_t ntiles = (int64_t)g.gx * g.gy * g.gz;
     if (ntiles > 0x7fffffff) return PA_ERR_ARG;
-    LevelEb<T> le = level_ebs<T>(eb, rel_eb);
-    const SplineArgs<T> args{data, g, le, flags8, d_maxlevel, qc_out, qc_in, anchors, anchors_in, old, out};
+    LevelEb<T> le = le_dev ? LevelEb<T>{} : level_ebs<T>(eb, rel_eb);
+    const SplineArgs<T> args{data, g, le, le_dev, flags8, d_maxlevel, qc_out, qc_in, anchors, anchors_in, old, out};
     if (line_field(g))
         hipLaunchKernelGGL((line_kernel<T, M>), dim3((unsigned)((ntiles + kLineTiles - 1) / kLineTiles)),
                            dim3(kThreads), 0, st, args);
@@ -1709,6 +1734,29 @@ extern "C" int pa_spline_construct(const void* d_data, int dtype, const pa_layou
                                               nullptr, (float*)d_anchors, nullptr, nullptr, nullptr, st);
     return launch_spline<double, COMPRESS>(L, (const double*)d_data, eb, rel_eb, d_flags8, nullptr, d_qc,
                                            nullptr, (double*)d_anchors, nullptr, nullptr, nullptr, st);
+}
+
+extern "C" int pa_level_bounds(const double* d_minmax, double eb, int dtype, double* d_bounds, void* stream) {
+    if (!d_bounds || !valid_dtype(dtype) || !(eb > 0)) return PA_ERR_ARG;
+    const double alpha = alpha_of(d_minmax ? eb : 0.0);
+    hipLaunchKernelGGL(level_bounds_kernel, dim3(1), dim3(64), 0, as_stream(stream), d_minmax, eb, dtype, alpha,
+                       d_bounds);
+    PA_LAUNCH_CHECK();
+    return PA_OK;
+}
+
+extern "C" int pa_spline_construct_dev(const void* d_data, int dtype, const pa_layout* L, const double* d_bounds,
+                                       const uint8_t* d_flags8, int32_t* d_qc, void* d_anchors, void* stream) {
+    if (!L || !d_data || !d_bounds || !d_flags8 || !d_qc || !d_anchors || !valid_dtype(dtype)) return PA_ERR_ARG;
+    hipStream_t st = as_stream(stream);
+    const Gaps gaps = make_gaps(L);
+    hipLaunchKernelGGL(zero_gaps_kernel, dim3(gap_blocks(gaps)), dim3(256), 0, st, d_qc, gaps);
+    PA_LAUNCH_CHECK();
+    if (dtype == PA_F32)
+        return launch_spline<float, COMPRESS>(L, (const float*)d_data, 0.0, 0.0, d_flags8, nullptr, d_qc, nullptr,
+                                              (float*)d_anchors, nullptr, nullptr, nullptr, st, d_bounds);
+    return launch_spline<double, COMPRESS>(L, (const double*)d_data, 0.0, 0.0, d_flags8, nullptr, d_qc, nullptr,
+                                           (double*)d_anchors, nullptr, nullptr, nullptr, st, d_bounds);
 }
 
 extern "C" int pa_spline_reconstruct(const int32_t* d_qc, const void* d_anchors, int dtype, const pa_layout* L,

@@ -140,61 +140,64 @@ def quality(original: torch.Tensor, reconstruction: torch.Tensor) -> dict:
 
 def compress(data: torch.Tensor, eb: float, rel: bool = True, progressive: bool = True,
              return_parts: bool = False, timer=None):
-    """Compresses a contiguous device field (T[nz][ny][nx])."""
+    """Compresses a contiguous device field (T[nz][ny][nx]).
+
+    The bound is resolved on the device (pa_level_bounds), so every stage is
+    queued before the host waits once, at the end, for the stream length
+    (the reference reads the range and the size table back in between)."""
     tm = timer or _NO_TIMER
     if not data.is_cuda or data.dtype not in DTYPES:
         raise ValueError("data must be a float32/float64 CUDA tensor")
+    if not eb > 0:
+        raise ValueError("error bound must be positive")
     data = data.contiguous()
     nx, ny, nz = shape_xyz(data)
     L = layout(nx, ny, nz)
     dt = DTYPES[data.dtype]
     st = _stream()
-    rng = 1.0
-    rng_dev = None
+    dev = data.device
+    bounds = torch.empty(16, dtype=torch.float64, device=dev)
     mm = None
     if rel:
         with tm("extrema"):
             mm = extrema(data)
-        rng_dev = (mm[1:2] - mm[0:1]).contiguous()
-    # the tuner needs no bound: it is queued before the host waits for the range
-    flags = torch.empty(8, dtype=torch.uint8, device=data.device)
-    scores = torch.empty(15, dtype=torch.float64, device=data.device)
+    check(lib().pa_level_bounds(_p(mm), float(eb), dt, _p(bounds), st), "pa_level_bounds")
+    flags = torch.empty(8, dtype=torch.uint8, device=dev)
+    scores = torch.empty(15, dtype=torch.float64, device=dev)
     tws = _empty(max(8, lib().pa_autotune_workspace(C.byref(L))))
     with tm("autotune"):
         check(lib().pa_autotune(_p(data), dt, C.byref(L), _p(scores), _p(flags), _p(tws), tws.numel(), st),
               "pa_autotune")
-    if rel:
-        lo, hi = mm[:2].tolist()
-        rng = hi - lo
-        eb_abs, rel_eb = eb * rng, eb
-    else:
-        eb_abs, rel_eb = eb, 0.0
-    if not eb_abs > 0:
-        raise ValueError("error bound must be positive (constant field in REL mode?)")
-    qc = torch.empty(L.code_len, dtype=torch.int32, device=data.device)
-    anchors = torch.empty(L.n4, dtype=data.dtype, device=data.device)
+    qc = torch.empty(L.code_len, dtype=torch.int32, device=dev)
+    anchors = torch.empty(L.n4, dtype=data.dtype, device=dev)
     with tm("predict"):
-        check(lib().pa_spline_construct(_p(data), dt, C.byref(L), eb_abs, rel_eb, _p(flags), _p(qc), _p(anchors),
-                                        st), "pa_spline_construct")
+        check(lib().pa_spline_construct_dev(_p(data), dt, C.byref(L), _p(bounds), _p(flags), _p(qc), _p(anchors),
+                                            st), "pa_spline_construct_dev")
     bp = _empty(L.host_size)
     mode = PA_MODE_NB if progressive else PA_MODE_SM
     with tm("shuffle"):
         check(lib().pa_bitshuffle(_p(qc), C.byref(L), mode, _p(bp), st), "pa_bitshuffle")
     bound = lib().pa_compress_bound(C.byref(L), dt)
     out = _empty(bound)
-    cum = torch.empty(128, dtype=torch.int64, device=data.device)
-    sizes = torch.empty(2, dtype=torch.int64, device=data.device)
+    cum = torch.empty(128, dtype=torch.int64, device=dev)
+    sizes = torch.empty(2, dtype=torch.int64, device=dev)
     ews = _empty(lib().pa_encode_workspace(C.byref(L)))
     with tm("encode"):
         check(lib().pa_lossless_encode(_p(bp), C.byref(L), _p(out), _p(cum), _p(sizes), _p(ews), ews.numel(), st),
               "pa_lossless_encode")
     total_t = sizes[1:2]
     with tm("merge"):
-        check(lib().pa_compress_merge(_p(out), C.byref(L), dt, _p(sizes), _p(flags), _p(rng_dev), _p(cum),
+        check(lib().pa_compress_merge(_p(out), C.byref(L), dt, _p(sizes), _p(flags), _p(bounds[1:2]), _p(cum),
                                       _p(anchors), _p(total_t), st), "pa_compress_merge")
-    total = int(total_t.item())
+    # the one host read: stream length, status, range
+    bounds[15:16].copy_(total_t.view(torch.float64))
+    vals = bounds.cpu()
+    if vals[14].item() != 0:
+        raise ValueError("error bound must be positive (constant field in REL mode?)")
+    total = int(vals[15:16].view(torch.int64).item())
+    rng = float(vals[1].item()) if rel else 1.0
     c = Compressed(stream=out[:total], shape=(nx, ny, nz), dtype=data.dtype, eb_input=eb, rel=rel,
-                   progressive=progressive, range=rng, flags=flags.tolist())
+                   progressive=progressive, range=rng, flags=[])
     if return_parts:
         return c, {"qc": qc, "anchors": anchors, "bp": bp, "cum": cum, "flags": flags, "scores": scores,
                    "layout": L}
