@@ -121,6 +121,21 @@ int pa_inverse_bitshuffle_window(const uint8_t* d_bp, const pa_layout* L, int mo
                                  const int32_t* d_begin, const int32_t* d_end, int32_t* d_qc,
                                  void* stream);
 
+/* Narrow hand-off between the inverse shuffle and the reconstruction of a
+ * progressive round (no reference counterpart): when every level that reads
+ * planes starts at plane 17 or later, the delta codes fit int16 and are
+ * written as int16 at the same indices, halving the code traffic of both
+ * stages; *d_narrow (device int32) records the choice and the _narrow
+ * reconstructions read it.  Results equal the int32 path's. */
+int pa_inverse_bitshuffle_window_narrow(const uint8_t* d_bp, const pa_layout* L, const int32_t* d_begin,
+                                        const int32_t* d_end, void* d_qc, int32_t* d_narrow, void* stream);
+int pa_spline_reconstruct_narrow(const void* d_qc, const int32_t* d_narrow, const void* d_anchors, int dtype,
+                                 const pa_layout* L, double eb, double rel_eb, const uint8_t* d_flags8,
+                                 const int32_t* d_maxlevel, void* d_out, void* stream);
+int pa_spline_delta_reconstruct_narrow(const void* d_qc, const int32_t* d_narrow, int dtype, const pa_layout* L,
+                                       double eb, double rel_eb, const uint8_t* d_flags8,
+                                       const int32_t* d_maxlevel, const void* d_old, void* d_out, void* stream);
+
 /* lossless_encode (zbpc.cu:476-520, kernel zbpc_encode 155-263): writes
  * header, size table and payloads to d_out (capacity >= pa_compress_bound),
  * the 4x32 cumulative size table to d_cum and the unpadded stream length to
@@ -239,7 +254,7 @@ typedef struct pa_retrieve_job {
     double eb;        /* absolute bound of the stream */
     double rel_eb;    /* relative bound of the stream (0 in absolute mode) */
     double target;    /* absolute target error of this round */
-    int32_t* state;   /* device int32[9]: begin[4], end[4], maxlevel (zeros before round 0) */
+    int32_t* state;   /* device int32[10]: begin[4], end[4], maxlevel, narrow flag (zeros before round 0) */
     uint8_t* bp;      /* device, layout host_size bytes, kept between rounds */
     int32_t* qc;      /* device, code_len int32, kept between rounds */
     void* dws;        /* device, pa_decode_workspace bytes, kept between rounds */

@@ -111,10 +111,21 @@ __global__ __launch_bounds__(kThreads) void batch_shuffle_kernel(const BatchFiel
 
 // Inverse; WINDOW restricts each level to planes [begin[lr], end[lr]) (plus
 // the sign plane in SM mode, bitplane.cu:467-471) and reads only those.
+// Whether a window's codes fit int16: every level that reads planes starts
+// at plane 17 or later, so its negabinary digits lie in bits 0..14 and the
+// values in [-10922, 21845].
+__device__ __forceinline__ bool window_narrow(const int32_t* begin, const int32_t* end) {
+    bool narrow = true;
+#pragma unroll
+    for (int lr = 0; lr < 4; ++lr) narrow = narrow && (begin[lr] >= end[lr] || begin[lr] >= 17);
+    return narrow;
+}
+
 template <int MODE, bool WINDOW>
 __device__ __forceinline__ void unshuffle_block(const uint8_t* __restrict__ bp, const Levels& v,
                                                 const int32_t* __restrict__ begin, const int32_t* __restrict__ end,
-                                                int32_t* __restrict__ qc, int blk, int32_t* stage) {
+                                                int32_t* __restrict__ qc, int blk, int32_t* stage,
+                                                bool narrow = false) {
     int first;
     const int lv = block_level(v, blk, &first);
     const int64_t g0 = (int64_t)(blk - first) * kGroupsPerBlock;
@@ -145,11 +156,32 @@ __device__ __forceinline__ void unshuffle_block(const uint8_t* __restrict__ bp, 
     __syncthreads();
     const int64_t c0 = 8 * g0;
     const int64_t seg = v.seg_len[lv];
+    if (narrow) {
+        int16_t* dst = reinterpret_cast<int16_t*>(qc) + v.prefix[lv];
+        for (int i = threadIdx.x; i < kCodesPerBlock; i += kThreads) {
+            const int64_t r = c0 + i;
+            if (r < seg) dst[r] = (int16_t)stage[(i >> 5) * kRow + (i & 31)];
+        }
+        return;
+    }
     int32_t* dst = qc + v.prefix[lv];
     for (int i = threadIdx.x; i < kCodesPerBlock; i += kThreads) {
         const int64_t r = c0 + i;
         if (r < seg) dst[r] = stage[(i >> 5) * kRow + (i & 31)];
     }
+}
+
+// NB window inverse that writes int16 codes when the window allows it and
+// records the choice in *narrow for the reconstruction
+__global__ __launch_bounds__(kThreads) void unshuffle_narrow_kernel(const uint8_t* __restrict__ bp, Levels v,
+                                                                    const int32_t* __restrict__ begin,
+                                                                    const int32_t* __restrict__ end,
+                                                                    int32_t* __restrict__ qc,
+                                                                    int32_t* __restrict__ narrow_out) {
+    __shared__ int32_t stage[kThreads * kRow];
+    const bool narrow = window_narrow(begin, end);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *narrow_out = narrow;
+    unshuffle_block<PA_MODE_NB, true>(bp, v, begin, end, qc, blockIdx.x, stage, narrow);
 }
 
 template <int MODE, bool WINDOW>
@@ -167,7 +199,10 @@ __global__ __launch_bounds__(kThreads) void retr_unshuffle_kernel(const RetrJob*
     __shared__ int32_t stage[kThreads * kRow];
     const int j = find_field(prefix, 0, nj, blockIdx.x);
     const RetrJob& J = jobs[j];
-    unshuffle_block<PA_MODE_NB, true>(J.bp, J.lv, J.state, J.state + 4, J.qc, (int)(blockIdx.x - prefix[j]), stage);
+    const bool narrow = window_narrow(J.state, J.state + 4);
+    const int blk = (int)(blockIdx.x - prefix[j]);
+    if (blk == 0 && threadIdx.x == 0) J.state[9] = narrow;
+    unshuffle_block<PA_MODE_NB, true>(J.bp, J.lv, J.state, J.state + 4, J.qc, blk, stage, narrow);
 }
 
 }  // namespace
@@ -234,6 +269,17 @@ int pa::retr_unshuffle(const RetrDev& r, hipStream_t st) {
     if (r.n_unshuf <= 0) return PA_OK;
     hipLaunchKernelGGL(retr_unshuffle_kernel, dim3((unsigned)r.n_unshuf), dim3(kThreads), 0, st, r.jobs, r.p_unshuf,
                        r.nj);
+    PA_LAUNCH_CHECK();
+    return PA_OK;
+}
+
+extern "C" int pa_inverse_bitshuffle_window_narrow(const uint8_t* d_bp, const pa_layout* L, const int32_t* d_begin,
+                                                   const int32_t* d_end, void* d_qc, int32_t* d_narrow,
+                                                   void* stream) {
+    if (!L || !d_qc || !d_bp || !d_begin || !d_end || !d_narrow) return PA_ERR_ARG;
+    Levels v = make_levels(L);
+    hipLaunchKernelGGL(unshuffle_narrow_kernel, dim3(v.blk_start[4]), dim3(kThreads), 0, as_stream(stream), d_bp, v,
+                       d_begin, d_end, static_cast<int32_t*>(d_qc), d_narrow);
     PA_LAUNCH_CHECK();
     return PA_OK;
 }

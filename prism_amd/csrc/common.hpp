@@ -172,7 +172,7 @@ struct RetrJob {
     const uint64_t* cum;   // trailer: cumulative payload sizes [4][32]
     const uint8_t* flags;  // trailer: flags8
     const void* anchors;   // trailer: anchors
-    int32_t* state;        // begin[4], end[4], maxlevel
+    int32_t* state;        // begin[4], end[4], maxlevel, narrow (int16 codes this round)
     unsigned long long* keys;  // strategy workgroup keys (33 x 33)
     uint8_t* bp;           // decoded bit-planes (host layout), kept between rounds
     int32_t* qc;           // codes, code_len

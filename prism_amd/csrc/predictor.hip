@@ -39,6 +39,7 @@ struct Ctx {
     T eb_r, ebx2;     // per-level values of the current level
     int32_t* qc_out;
     const int32_t* qc_in;
+    bool q16;         // qc_in holds int16 codes (narrow hand-off of a progressive round)
     bool natural;     // tuner candidates 9..11
     double acc;       // tuner partial sum of this lane
     // the tile is not the last one along axis A (0 = x, 1 = y, 2 = z)
@@ -52,6 +53,11 @@ struct Ctx {
         return IN || v < (A == 0 ? g.nx : (A == 1 ? g.ny : g.nz));
     }
 };
+
+// code i of the hand-off buffer (int32, or int16 when q16)
+__device__ __forceinline__ int32_t code_at(const int32_t* q, bool q16, uint32_t i) {
+    return q16 ? (int32_t)reinterpret_cast<const int16_t*>(q)[i] : q[i];
+}
 
 template <typename T>
 __device__ __forceinline__ T cubic(T a, T b, T c, T d, bool natural) {
@@ -687,7 +693,7 @@ __device__ __forceinline__ void fill0_load(const Ctx<T, IN>& t, const int32_t* t
         const bool full = ((y | z) & 1) != 0;
         v[it] = 0;
         if (load && r < kRows0 && xin && (full || (x & 1)) && t.template inf<1>(t.oy + y) && t.template inf<2>(t.oz + z))
-            v[it] = t.qc_in[(uint32_t)(tab[r] + (full ? t.ox + x : (t.ox >> 1) + (x >> 1)))];
+            v[it] = code_at(t.qc_in, t.q16, (uint32_t)(tab[r] + (full ? t.ox + x : (t.ox >> 1) + (x >> 1))));
     }
 #pragma unroll
     for (int it = 0; it < kFill0EdgeIters; ++it) {  // x = 16: level 0 where y or z is odd
@@ -696,7 +702,7 @@ __device__ __forceinline__ void fill0_load(const Ctx<T, IN>& t, const int32_t* t
         e[it] = 0;
         if (load && r < kRows0 && ((y | z) & 1) && t.template inf<0>(t.ox + 16) && t.template inf<1>(t.oy + y) &&
             t.template inf<2>(t.oz + z))
-            e[it] = t.qc_in[(uint32_t)(tab[r] + t.ox + 16)];
+            e[it] = code_at(t.qc_in, t.q16, (uint32_t)(tab[r] + t.ox + 16));
     }
 }
 
@@ -733,7 +739,8 @@ __device__ __forceinline__ void fill_load(const Ctx<T, IN>& t, const int32_t* ta
         v[it] = 0;
         if (load && mine && t.template inf<0>(t.ox + i * F::U) && t.template inf<1>(t.oy + j * F::U) &&
             t.template inf<2>(t.oz + k * F::U))
-            v[it] = t.qc_in[(uint32_t)(tab[row_off<L>() + row] + (full ? (t.ox >> L) + i : (t.ox >> (L + 1)) + (i >> 1)))];
+            v[it] = code_at(t.qc_in, t.q16,
+                            (uint32_t)(tab[row_off<L>() + row] + (full ? (t.ox >> L) + i : (t.ox >> (L + 1)) + (i >> 1))));
     }
 }
 
@@ -804,6 +811,8 @@ struct SplineArgs {
     Geom g;
     LevelEb<T> le;
     const double* le_dev;  // when set, the level bounds come from this device block instead of `le`
+    const int32_t* narrow;  // when set and non-zero on the device, qc_in holds int16 codes
+    bool q16;
     const uint8_t* flags8;
     const int32_t* d_maxlevel;
     int32_t* qc_out;
@@ -816,7 +825,7 @@ struct SplineArgs {
 
 // One tile: compress (load, predict, quantise, store codes) or decompress
 // (fill the code slots, reconstruct, add old, store).
-template <typename T, int M, bool IN>
+template <typename T, int M, bool IN, bool Q16 = false>
 __device__ __forceinline__ void tile_run(const SplineArgs<T>& a, T* tile, int32_t* rowtab, int bx, int by, int bz) {
     const Geom& g = a.g;
     Ctx<T, IN> t;
@@ -830,6 +839,7 @@ __device__ __forceinline__ void tile_run(const SplineArgs<T>& a, T* tile, int32_
     t.g = g;
     t.qc_out = a.qc_out;
     t.qc_in = a.qc_in;
+    t.q16 = Q16;  // resolved per launch: one code width per kernel body
     t.natural = false;
     t.acc = 0.0;
     uint8_t fl[8];
@@ -1005,6 +1015,7 @@ __device__ __forceinline__ void plane_tiles(const SplineArgs<T>& a, T* lds, int6
     t.g = g;
     t.qc_out = a.qc_out;
     t.qc_in = a.qc_in;
+    t.q16 = a.q16;
     t.natural = false;
     t.acc = 0.0;
     uint8_t fl[8];
@@ -1050,7 +1061,7 @@ __device__ __forceinline__ void plane_tiles(const SplineArgs<T>& a, T* lds, int6
                         case 2: idx = lm_index<2>(g, gx, gy, 0); break;
                         default: idx = lm_index<3>(g, gx, gy, 0); break;
                     }
-                    q[it] = t.qc_in[idx];
+                    q[it] = code_at(t.qc_in, t.q16, (uint32_t)idx);
                 }
             }
         }
@@ -1084,6 +1095,7 @@ template <typename T>
 __device__ __forceinline__ SplineArgs<T> resolve(const SplineArgs<T>& a) {
     SplineArgs<T> r = a;
     if (a.le_dev) r.le = load_level_ebs<T>(a.le_dev);
+    if (a.narrow) r.q16 = *a.narrow != 0;
     return r;
 }
 
@@ -1149,7 +1161,7 @@ __device__ __forceinline__ void line_codes_out(const Geom& g, int64_t unit, cons
 }
 
 template <int L>
-__device__ __forceinline__ void line_codes_load(const Geom& g, int64_t unit, bool load, const int32_t* qc,
+__device__ __forceinline__ void line_codes_load(const Geom& g, int64_t unit, bool load, const int32_t* qc, bool q16,
                                                 int32_t (&r)[8 >> L]) {
     constexpr int U = 1 << L, N = 8 >> L;
     const int lane = lane_id();
@@ -1157,7 +1169,7 @@ __device__ __forceinline__ void line_codes_load(const Geom& g, int64_t unit, boo
 #pragma unroll
     for (int it = 0; it < N; ++it) {
         const int64_t m = m0 + lane + it * kThreads;
-        r[it] = (load && (2 * m + 1) * U < g.nx) ? qc[g.prefix[L] + m] : 0;
+        r[it] = (load && (2 * m + 1) * U < g.nx) ? code_at(qc, q16, (uint32_t)(g.prefix[L] + m)) : 0;
     }
 }
 
@@ -1238,10 +1250,10 @@ __device__ __forceinline__ void line_tiles(const SplineArgs<T>& a, T* lds, int64
             }
         }
         int32_t r3[1], r2[2], r1[4], r0[8];
-        line_codes_load<3>(g, unit, 3 <= maxlevel, a.qc_in, r3);
-        line_codes_load<2>(g, unit, 2 <= maxlevel, a.qc_in, r2);
-        line_codes_load<1>(g, unit, 1 <= maxlevel, a.qc_in, r1);
-        line_codes_load<0>(g, unit, 0 <= maxlevel, a.qc_in, r0);
+        line_codes_load<3>(g, unit, 3 <= maxlevel, a.qc_in, a.q16, r3);
+        line_codes_load<2>(g, unit, 2 <= maxlevel, a.qc_in, a.q16, r2);
+        line_codes_load<1>(g, unit, 1 <= maxlevel, a.qc_in, a.q16, r1);
+        line_codes_load<0>(g, unit, 0 <= maxlevel, a.qc_in, a.q16, r0);
         const T a0 = (a.anchors_in && active) ? a.anchors_in[tile] : (T)0;
         const T a16 = (a.anchors_in && tile + 1 < g.gx) ? a.anchors_in[tile + 1] : (T)0;
         line_codes_stage<3>(r3, ci);
@@ -1305,6 +1317,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     const int bx = (int)(tix % g.gx), by = (int)((tix / g.gx) % g.gy), bz = (int)(tix / ((int64_t)g.gx * g.gy));
     const bool interior = bx * 16 + 16 < g.nx && by * 16 + 16 < g.ny && bz * 16 + 16 < g.nz;
     const SplineArgs<T> r = resolve(a);
+    if constexpr (M == DECOMPRESS) {
+        if (r.q16) {
+            if (interior) tile_run<T, M, true, true>(r, tile, rowtab, bx, by, bz);
+            else tile_run<T, M, false, true>(r, tile, rowtab, bx, by, bz);
+            return;
+        }
+    }
     if (interior) tile_run<T, M, true>(r, tile, rowtab, bx, by, bz);
     else tile_run<T, M, false>(r, tile, rowtab, bx, by, bz);
 }
@@ -1341,6 +1360,7 @@ __device__ __forceinline__ void tune_block(const T* __restrict__ data, const Geo
     t.g = g;
     t.qc_out = nullptr;
     t.qc_in = nullptr;
+    t.q16 = false;
     t.acc = 0.0;
     load_tile(t, data);
     __syncthreads();
@@ -1566,8 +1586,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     const int64_t gt = base + (xcd_order ? xcd * per + (xcd < extra ? xcd : extra) + b / kXcds : b);
     const int f = find_field(prefix, lo, hi, gt);
     const BatchField& F = fields[f];
-    const SplineArgs<T> a{static_cast<const T*>(F.data), F.g, load_level_ebs<T>(F.ebinfo), nullptr, F.flags,
-                          nullptr, F.qc, nullptr, static_cast<T*>(F.anchors), nullptr, nullptr, nullptr};
+    const SplineArgs<T> a{static_cast<const T*>(F.data), F.g, load_level_ebs<T>(F.ebinfo), nullptr, nullptr, false,
+                          F.flags, nullptr, F.qc, nullptr, static_cast<T*>(F.anchors), nullptr, nullptr, nullptr};
     const Geom& g = a.g;
     const int64_t tix = gt - prefix[f];
     if (line_field(g)) {
@@ -1603,7 +1623,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         le.eb_r[i] = sizeof(T) == 4 ? (T)J.le32[i] : (T)J.le64[i];
         le.ebx2[i] = sizeof(T) == 4 ? (T)J.le32[4 + i] : (T)J.le64[4 + i];
     }
-    const SplineArgs<T> a{nullptr, J.g, le, nullptr, J.flags, J.state + 8, nullptr, J.qc, nullptr,
+    const SplineArgs<T> a{nullptr, J.g, le, nullptr, nullptr, J.state[9] != 0, J.flags, J.state + 8, nullptr, J.qc, nullptr,
                           J.first ? static_cast<const T*>(J.anchors) : nullptr,
                           J.first ? nullptr : static_cast<const T*>(J.old), static_cast<T*>(J.out)};
     const Geom& g = a.g;
@@ -1618,6 +1638,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     }
     const int bx = (int)(tix % g.gx), by = (int)((tix / g.gx) % g.gy), bz = (int)(tix / ((int64_t)g.gx * g.gy));
     const bool interior = bx * 16 + 16 < g.nx && by * 16 + 16 < g.ny && bz * 16 + 16 < g.nz;
+    if (a.q16) {
+        if (interior) tile_run<T, DECOMPRESS, true, true>(a, tile, rowtab, bx, by, bz);
+        else tile_run<T, DECOMPRESS, false, true>(a, tile, rowtab, bx, by, bz);
+        return;
+    }
     if (interior) tile_run<T, DECOMPRESS, true>(a, tile, rowtab, bx, by, bz);
     else tile_run<T, DECOMPRESS, false>(a, tile, rowtab, bx, by, bz);
 }
@@ -1646,12 +1671,14 @@ int gap_blocks(const Gaps& gaps) {
 template <typename T, int M>
 int launch_spline(const pa_layout* L, const T* data, double eb, double rel_eb, const uint8_t* flags8,
                   const int32_t* d_maxlevel, int32_t* qc_out, const int32_t* qc_in, T* anchors,
-                  const T* anchors_in, const T* old, T* out, hipStream_t st, const double* le_dev = nullptr) {
+                  const T* anchors_in, const T* old, T* out, hipStream_t st, const double* le_dev = nullptr,
+                  const int32_t* narrow = nullptr) {
     Geom g = make_geom(L);
     const int64_t ntiles = (int64_t)g.gx * g.gy * g.gz;
     if (ntiles > 0x7fffffff) return PA_ERR_ARG;
     LevelEb<T> le = le_dev ? LevelEb<T>{} : level_ebs<T>(eb, rel_eb);
-    const SplineArgs<T> args{data, g, le, le_dev, flags8, d_maxlevel, qc_out, qc_in, anchors, anchors_in, old, out};
+    const SplineArgs<T> args{data, g, le, le_dev, narrow, false, flags8, d_maxlevel, qc_out, qc_in, anchors, anchors_in,
+                             old, out};
     if (line_field(g))
         hipLaunchKernelGGL((line_kernel<T, M>), dim3((unsigned)((ntiles + kLineTiles - 1) / kLineTiles)),
                            dim3(kThreads), 0, st, args);
@@ -1769,6 +1796,35 @@ extern "C" int pa_spline_reconstruct(const int32_t* d_qc, const void* d_anchors,
                                                 nullptr, (const float*)d_anchors, nullptr, (float*)d_out, st);
     return launch_spline<double, DECOMPRESS>(L, nullptr, eb, rel_eb, d_flags8, d_maxlevel, nullptr, d_qc, nullptr,
                                              (const double*)d_anchors, nullptr, (double*)d_out, st);
+}
+
+extern "C" int pa_spline_reconstruct_narrow(const void* d_qc, const int32_t* d_narrow, const void* d_anchors, int dtype,
+                                            const pa_layout* L, double eb, double rel_eb, const uint8_t* d_flags8,
+                                            const int32_t* d_maxlevel, void* d_out, void* stream) {
+    if (!L || !d_qc || !d_narrow || !d_anchors || !d_flags8 || !d_out || !valid_dtype(dtype) || !(eb > 0))
+        return PA_ERR_ARG;
+    hipStream_t st = as_stream(stream);
+    const int32_t* q = static_cast<const int32_t*>(d_qc);
+    if (dtype == PA_F32)
+        return launch_spline<float, DECOMPRESS>(L, nullptr, eb, rel_eb, d_flags8, d_maxlevel, nullptr, q, nullptr,
+                                                (const float*)d_anchors, nullptr, (float*)d_out, st, nullptr, d_narrow);
+    return launch_spline<double, DECOMPRESS>(L, nullptr, eb, rel_eb, d_flags8, d_maxlevel, nullptr, q, nullptr,
+                                             (const double*)d_anchors, nullptr, (double*)d_out, st, nullptr, d_narrow);
+}
+
+extern "C" int pa_spline_delta_reconstruct_narrow(const void* d_qc, const int32_t* d_narrow, int dtype,
+                                                  const pa_layout* L, double eb, double rel_eb,
+                                                  const uint8_t* d_flags8, const int32_t* d_maxlevel,
+                                                  const void* d_old, void* d_out, void* stream) {
+    if (!L || !d_qc || !d_narrow || !d_flags8 || !d_old || !d_out || !valid_dtype(dtype) || !(eb > 0))
+        return PA_ERR_ARG;
+    hipStream_t st = as_stream(stream);
+    const int32_t* q = static_cast<const int32_t*>(d_qc);
+    if (dtype == PA_F32)
+        return launch_spline<float, DECOMPRESS>(L, nullptr, eb, rel_eb, d_flags8, d_maxlevel, nullptr, q, nullptr,
+                                                nullptr, (const float*)d_old, (float*)d_out, st, nullptr, d_narrow);
+    return launch_spline<double, DECOMPRESS>(L, nullptr, eb, rel_eb, d_flags8, d_maxlevel, nullptr, q, nullptr,
+                                             nullptr, (const double*)d_old, (double*)d_out, st, nullptr, d_narrow);
 }
 
 extern "C" int pa_spline_delta_reconstruct(const int32_t* d_qc, int dtype, const pa_layout* L, double eb,

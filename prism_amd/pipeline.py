@@ -396,8 +396,9 @@ class Retriever:
         self.range = stream_range(c)
         self.eb_abs = c.eb_input * self.range if c.rel else c.eb_input
         self.rel_eb = c.eb_input if c.rel else 0.0
-        self.state = torch.zeros(16, dtype=torch.int32, device=dev)  # begin[4] end[4] maxlevel
+        self.state = torch.zeros(16, dtype=torch.int32, device=dev)  # begin[4] end[4] maxlevel narrow
         self.begin, self.end, self.maxlevel = self.state[0:4], self.state[4:8], self.state[8:9]
+        self.narrow = self.state[9:10]  # codes of this round handed over as int16
         self.bp = _empty(self.L.host_size)
         self.qc = torch.empty(self.L.code_len, dtype=torch.int32, device=dev)
         self.dws = _empty(lib().pa_decode_workspace(C.byref(self.L)))
@@ -421,8 +422,9 @@ class Retriever:
                                               _p(self.bp), _p(self.bytes_read), int(self.rounds > 0),
                                               _p(self.dws), self.dws.numel(), st), "pa_lossless_decode_window")
         with tm("unshuffle"):
-            check(lib().pa_inverse_bitshuffle_window(_p(self.bp), C.byref(L), PA_MODE_NB, _p(self.begin),
-                                                     _p(self.end), _p(self.qc), st), "pa_inverse_bitshuffle_window")
+            check(lib().pa_inverse_bitshuffle_window_narrow(_p(self.bp), C.byref(L), _p(self.begin), _p(self.end),
+                                                            _p(self.qc), _p(self.narrow), st),
+                  "pa_inverse_bitshuffle_window_narrow")
         nx, ny, nz = self.c.shape
         new = torch.empty((nz, ny, nx), dtype=self.c.dtype, device=self.qc.device)
         with tm("reconstruct"):
@@ -453,13 +455,14 @@ class Retriever:
     def _reconstruct(self, new, st):
         L = self.L
         if self.out is None:
-            check(lib().pa_spline_reconstruct(_p(self.qc), _p(self.anchors), self.dt, C.byref(L), self.eb_abs,
-                                              self.rel_eb, _p(self.flags), _p(self.maxlevel), _p(new), st),
-                  "pa_spline_reconstruct")
+            check(lib().pa_spline_reconstruct_narrow(_p(self.qc), _p(self.narrow), _p(self.anchors), self.dt,
+                                                     C.byref(L), self.eb_abs, self.rel_eb, _p(self.flags),
+                                                     _p(self.maxlevel), _p(new), st), "pa_spline_reconstruct_narrow")
         else:
-            check(lib().pa_spline_delta_reconstruct(_p(self.qc), self.dt, C.byref(L), self.eb_abs, self.rel_eb,
-                                                    _p(self.flags), _p(self.maxlevel), _p(self.out), _p(new),
-                                                    st), "pa_spline_delta_reconstruct")
+            check(lib().pa_spline_delta_reconstruct_narrow(_p(self.qc), _p(self.narrow), self.dt, C.byref(L),
+                                                           self.eb_abs, self.rel_eb, _p(self.flags),
+                                                           _p(self.maxlevel), _p(self.out), _p(new), st),
+                  "pa_spline_delta_reconstruct_narrow")
 
 
 class RetrievalBatch:

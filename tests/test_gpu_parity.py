@@ -321,3 +321,44 @@ def test_quality_statistics(pa, dtype):
             mse = sse / n
             assert abs(q["psnr_db"] - (20 * np.log10(rg) - 10 * np.log10(mse))) < 1e-9
             assert abs(q["nrmse"] - np.sqrt(mse) / rg) <= 1e-12 * np.sqrt(mse) / rg
+
+
+def test_narrow_handoff_equals_int32(pa, oracle):
+    """Progressive rounds through the int16 hand-off (every level reading
+    planes >= 17) and through the int32 stage calls give the same bits; the
+    first round (planes from 0) stays int32."""
+    _lib, pipe = pa
+    shape = (40, 50, 60)
+    data = smooth(shape, np.float64, seed=3)
+    c = pipe.compress(dev(data), 1e-6, rel=True, progressive=True)
+    r = pipe.Retriever(c)  # narrow path
+    L = r.L
+    flags_seen = []
+    st32 = torch.zeros(16, dtype=torch.int32, device="cuda")
+    bp32 = torch.empty(L.host_size, dtype=torch.uint8, device="cuda")
+    qc32 = torch.empty(L.code_len, dtype=torch.int32, device="cuda")
+    dws = torch.empty(_lib.lib().pa_decode_workspace(C.byref(L)), dtype=torch.uint8, device="cuda")
+    sws = torch.empty(_lib.lib().pa_strategy_workspace(), dtype=torch.uint8, device="cuda")
+    old = None
+    for k, t in enumerate((1e-2, 1e-3, 1e-4, 1e-5, 1e-6)):
+        announce("narrow hand-off round", k, t)
+        got = r.retrieve(t)
+        flags_seen.append(int(r.narrow.item()))
+        lib = _lib.lib()
+        _lib.check(lib.pa_find_strategy(P(r.cum), P(st32[0:4]), P(st32[4:8]), P(st32[8:9]), r.eb_abs, t * r.range,
+                                        P(sws), sws.numel(), S()))
+        _lib.check(lib.pa_lossless_decode_window(P(c.stream), c.nbytes, C.byref(L), P(st32[0:4]), P(st32[4:8]),
+                                                 P(bp32), None, int(k > 0), P(dws), dws.numel(), S()))
+        _lib.check(lib.pa_inverse_bitshuffle_window(P(bp32), C.byref(L), _lib.PA_MODE_NB, P(st32[0:4]), P(st32[4:8]),
+                                                    P(qc32), S()))
+        new = torch.empty(shape, dtype=torch.float64, device="cuda")
+        if old is None:
+            _lib.check(lib.pa_spline_reconstruct(P(qc32), P(r.anchors), _lib.PA_F64, C.byref(L), r.eb_abs, r.rel_eb,
+                                                 P(r.flags), P(st32[8:9]), P(new), S()))
+        else:
+            _lib.check(lib.pa_spline_delta_reconstruct(P(qc32), _lib.PA_F64, C.byref(L), r.eb_abs, r.rel_eb,
+                                                       P(r.flags), P(st32[8:9]), P(old), P(new), S()))
+        old = new
+        assert torch.equal(got.view(torch.uint8), new.view(torch.uint8)), k
+        assert torch.equal(r.state[0:9], st32[0:9])
+    assert flags_seen[0] == 0 and 1 in flags_seen[1:], flags_seen
