@@ -132,7 +132,8 @@ def main():
         lat = (time.perf_counter() - a) * 1e3
         q = pipeline.quality(field, out)
         nread = int(r.bytes_read.item())
-        levels.append({"target": t, "bound_abs": t * c.range, "linf": q["linf"],
+        narrow = int(r.narrow.item())
+        levels.append({"target": t, "bound_abs": t * c.range, "linf": q["linf"], "int16_codes": bool(narrow),
                        "linf_over_bound": q["linf"] / (t * c.range), "psnr_db": q["psnr_db"], "nrmse": q["nrmse"],
                        "payload_bytes_read": nread - prev, "ms": lat,
                        "begin": r.begin.tolist(), "end": r.end.tolist()})
@@ -141,7 +142,7 @@ def main():
     # roofline of the dominant kernel: per-launch algorithmic bytes over the
     # launch's own event-timed duration, averaged over the launches
     nrounds = len(args.targets)
-    codes = 4 * (L.n - L.n4)
+    codes = 4 * (L.n - L.n4)  # int32 codes written by the predictor and read by the shuffle
 
     def planes_bytes(lv_info):
         return sum((lv_info["end"][3 - lv] - lv_info["begin"][3 - lv]) * L.dev_stride[lv] for lv in range(4))
@@ -158,10 +159,11 @@ def main():
         rnd = levels[k % nrounds]
         if name == "decode":
             return rnd["payload_bytes_read"] + planes_bytes(rnd)
+        code_b = 2 if rnd["int16_codes"] else 4  # the round's code hand-off width
         if name == "unshuffle":
-            return planes_bytes(rnd) + 4 * L.code_len
+            return planes_bytes(rnd) + code_b * L.code_len
         if name == "reconstruct":
-            return codes + L.n * s + (L.n * s if k % nrounds else L.n4 * s)
+            return code_b * (L.n - L.n4) + L.n * s + (L.n * s if k % nrounds else L.n4 * s)
         return 0
 
     passes = max(1, min(args.steps, 5))
