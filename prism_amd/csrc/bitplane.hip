@@ -22,6 +22,7 @@ constexpr int kGroupsPerBlock = kShuffleGroups;       // 1024 groups
 static_assert(kShuffleGroups == 4 * kThreads, "one group quartet per lane");
 constexpr int kCodesPerBlock = 8 * kGroupsPerBlock;   // 8192 codes
 constexpr int kRow = 33;                              // padded LDS row (32 codes)
+constexpr int kStageWords = kThreads * kRow + 8;      // inverse: up to 3 codes of alignment shift
 
 // In-register transpose of a 32x32 bit matrix: afterwards bit c of a[k]
 // equals bit k of the original a[c] (bit 0 = least significant).
@@ -121,6 +122,43 @@ __device__ __forceinline__ bool window_narrow(const int32_t* begin, const int32_
     return narrow;
 }
 
+// Stage slot of stage position d with one pad word per ROW words, so the
+// lanes' writes of one instruction fall on distinct banks.
+template <int ROW>
+__device__ __forceinline__ int padded(int d) { return d + d / ROW; }
+
+// Codes leave the stage as 16-byte stores: stage position d holds the code
+// at destination index (block start - sh + d), sh the distance of the block
+// start from the previous 16-byte boundary, so the vectors of 16 / E codes
+// (E bytes each) are aligned; partial vectors at the block ends (and at the
+// end of the level segment) are written code by code.
+template <typename E, int ROW>
+__device__ __forceinline__ void store_codes(const int32_t* stage, E* abase, int sh, int nvalid) {
+    constexpr int V = 16 / (int)sizeof(E);  // codes per vector
+    constexpr int W = V * (int)sizeof(E) / 4;  // 4 stage words per vector
+    const int nv = (nvalid + sh + V - 1) / V;
+    for (int vi = threadIdx.x; vi < nv; vi += kThreads) {
+        const int lo = V * vi, hi = lo + V;
+        if (lo >= sh && hi <= sh + nvalid) {
+            const int p = padded<ROW>(W * vi);  // the 4 words share one row
+            uint4 w;
+            w.x = (uint32_t)stage[p];
+            w.y = (uint32_t)stage[p + 1];
+            w.z = (uint32_t)stage[p + 2];
+            w.w = (uint32_t)stage[p + 3];
+            *reinterpret_cast<uint4*>(abase + lo) = w;
+        } else {
+            const int a = lo > sh ? lo : sh, b = hi < sh + nvalid ? hi : sh + nvalid;
+            for (int d = a; d < b; ++d) {
+                if (sizeof(E) == 2)
+                    abase[d] = (E)reinterpret_cast<const uint16_t*>(stage)[2 * padded<ROW>(d >> 1) + (d & 1)];
+                else
+                    abase[d] = (E)stage[padded<ROW>(d)];
+            }
+        }
+    }
+}
+
 template <int MODE, bool WINDOW>
 __device__ __forceinline__ void unshuffle_block(const uint8_t* __restrict__ bp, const Levels& v,
                                                 const int32_t* __restrict__ begin, const int32_t* __restrict__ end,
@@ -140,35 +178,57 @@ __device__ __forceinline__ void unshuffle_block(const uint8_t* __restrict__ bp, 
         }
     }
     const int64_t g = g0 + 4 * threadIdx.x;
-    if (g < v.stride[lv]) {
-        uint32_t a[32];
+    const bool has = g < v.stride[lv];
+    uint32_t a[32];
+    if (has) {
         const uint8_t* row = bp + v.base[lv] + g;
 #pragma unroll
         for (int k = 0; k < 32; ++k)
             a[31 - k] = (planes >> k) & 1u ? *reinterpret_cast<const uint32_t*>(row + k * v.stride[lv]) : 0u;
         transpose32(a);
+    }
+    const int64_t c0 = 8 * g0;
+    const int64_t seg = v.seg_len[lv];
+    const int nvalid = (int)(seg - c0 < kCodesPerBlock ? seg - c0 : kCodesPerBlock);  // <= 0: nothing to write
+    if (narrow) {  // int16 codes, 16 words per stage row
+        int16_t* base = reinterpret_cast<int16_t*>(qc) + v.prefix[lv] + c0;
+        const int sh = (int)((reinterpret_cast<uintptr_t>(base) >> 1) & 7);
+        if (has) {
+            const int d0 = 32 * threadIdx.x + sh;
+            if ((sh & 1) == 0) {  // code pairs share a stage word
+#pragma unroll
+                for (int q = 0; q < 32; q += 2) {
+                    const uint32_t lo = (uint16_t)(MODE == PA_MODE_NB ? nb_inv(a[slot(q)]) : sm_inv(a[slot(q)]));
+                    const uint32_t hi =
+                        (uint16_t)(MODE == PA_MODE_NB ? nb_inv(a[slot(q + 1)]) : sm_inv(a[slot(q + 1)]));
+                    stage[padded<16>((d0 + q) >> 1)] = (int32_t)(lo | (hi << 16));
+                }
+            } else {
+                uint16_t* s16 = reinterpret_cast<uint16_t*>(stage);
+#pragma unroll
+                for (int q = 0; q < 32; ++q) {
+                    const int d = d0 + q;
+                    s16[2 * padded<16>(d >> 1) + (d & 1)] =
+                        (uint16_t)(MODE == PA_MODE_NB ? nb_inv(a[slot(q)]) : sm_inv(a[slot(q)]));
+                }
+            }
+        }
+        __syncthreads();
+        if (nvalid > 0) store_codes<int16_t, 16>(stage, base - sh, sh, nvalid);
+        return;
+    }
+    int32_t* base = qc + v.prefix[lv] + c0;
+    const int sh = (int)((reinterpret_cast<uintptr_t>(base) >> 2) & 3);
+    if (has) {
+        const int d0 = 32 * threadIdx.x + sh;
 #pragma unroll
         for (int q = 0; q < 32; ++q) {
             const uint32_t w = a[slot(q)];
-            stage[threadIdx.x * kRow + q] = MODE == PA_MODE_NB ? nb_inv(w) : sm_inv(w);
+            stage[padded<32>(d0 + q)] = MODE == PA_MODE_NB ? nb_inv(w) : sm_inv(w);
         }
     }
     __syncthreads();
-    const int64_t c0 = 8 * g0;
-    const int64_t seg = v.seg_len[lv];
-    if (narrow) {
-        int16_t* dst = reinterpret_cast<int16_t*>(qc) + v.prefix[lv];
-        for (int i = threadIdx.x; i < kCodesPerBlock; i += kThreads) {
-            const int64_t r = c0 + i;
-            if (r < seg) dst[r] = (int16_t)stage[(i >> 5) * kRow + (i & 31)];
-        }
-        return;
-    }
-    int32_t* dst = qc + v.prefix[lv];
-    for (int i = threadIdx.x; i < kCodesPerBlock; i += kThreads) {
-        const int64_t r = c0 + i;
-        if (r < seg) dst[r] = stage[(i >> 5) * kRow + (i & 31)];
-    }
+    if (nvalid > 0) store_codes<int32_t, 32>(stage, base - sh, sh, nvalid);
 }
 
 // NB window inverse that writes int16 codes when the window allows it and
@@ -178,7 +238,7 @@ __global__ __launch_bounds__(kThreads) void unshuffle_narrow_kernel(const uint8_
                                                                     const int32_t* __restrict__ end,
                                                                     int32_t* __restrict__ qc,
                                                                     int32_t* __restrict__ narrow_out) {
-    __shared__ int32_t stage[kThreads * kRow];
+    __shared__ int32_t stage[kStageWords];
     const bool narrow = window_narrow(begin, end);
     if (blockIdx.x == 0 && threadIdx.x == 0) *narrow_out = narrow;
     unshuffle_block<PA_MODE_NB, true>(bp, v, begin, end, qc, blockIdx.x, stage, narrow);
@@ -189,14 +249,14 @@ __global__ __launch_bounds__(kThreads) void unshuffle_kernel(const uint8_t* __re
                                                              const int32_t* __restrict__ begin,
                                                              const int32_t* __restrict__ end,
                                                              int32_t* __restrict__ qc) {
-    __shared__ int32_t stage[kThreads * kRow];
+    __shared__ int32_t stage[kStageWords];
     unshuffle_block<MODE, WINDOW>(bp, v, begin, end, qc, blockIdx.x, stage);
 }
 
 // window inverse (NB) of every job of a batch of retrieval rounds
 __global__ __launch_bounds__(kThreads) void retr_unshuffle_kernel(const RetrJob* __restrict__ jobs,
                                                                   const int64_t* __restrict__ prefix, int nj) {
-    __shared__ int32_t stage[kThreads * kRow];
+    __shared__ int32_t stage[kStageWords];
     const int j = find_field(prefix, 0, nj, blockIdx.x);
     const RetrJob& J = jobs[j];
     const bool narrow = window_narrow(J.state, J.state + 4);

@@ -10,6 +10,7 @@ value is needed (the value range of a REL-mode field, the stream length).
 from __future__ import annotations
 
 import ctypes as C
+import threading
 from dataclasses import dataclass, field
 
 import torch
@@ -33,8 +34,25 @@ def _dev():
 
 
 def _empty(nbytes: int, dtype=torch.uint8):
-    item = torch.empty(0, dtype=dtype).element_size()
+    item = dtype.itemsize
     return torch.empty(max(1, (int(nbytes) + item - 1) // item), dtype=dtype, device=_dev())
+
+
+_TLS = threading.local()
+
+
+def _host_read(t: torch.Tensor) -> torch.Tensor:
+    """Copies a small device tensor to the host through a pinned per-thread
+    buffer and waits for the current stream; returns a fresh CPU tensor."""
+    nb = t.numel() * t.element_size()
+    buf = getattr(_TLS, "pinned", None)
+    if buf is None or buf.numel() < nb:
+        buf = torch.empty(max(nb, 256), dtype=torch.uint8, pin_memory=True)
+        _TLS.pinned = buf
+    h = buf[:nb].view(t.dtype)
+    h.copy_(t.reshape(-1), non_blocking=True)
+    torch.cuda.current_stream().synchronize()
+    return h.clone()
 
 
 def shape_xyz(t: torch.Tensor):
@@ -185,13 +203,11 @@ def compress(data: torch.Tensor, eb: float, rel: bool = True, progressive: bool 
     with tm("encode"):
         check(lib().pa_lossless_encode(_p(bp), C.byref(L), _p(out), _p(cum), _p(sizes), _p(ews), ews.numel(), st),
               "pa_lossless_encode")
-    total_t = sizes[1:2]
-    with tm("merge"):
+    with tm("merge"):  # the stream length goes to bounds[15] (int64 bits)
         check(lib().pa_compress_merge(_p(out), C.byref(L), dt, _p(sizes), _p(flags), _p(bounds[1:2]), _p(cum),
-                                      _p(anchors), _p(total_t), st), "pa_compress_merge")
+                                      _p(anchors), _p(bounds[15:16]), st), "pa_compress_merge")
     # the one host read: stream length, status, range
-    bounds[15:16].copy_(total_t.view(torch.float64))
-    vals = bounds.cpu()
+    vals = _host_read(bounds)
     if vals[14].item() != 0:
         raise ValueError("error bound must be positive (constant field in REL mode?)")
     total = int(vals[15:16].view(torch.int64).item())
@@ -393,38 +409,44 @@ class Retriever:
         self.dt = DTYPES[c.dtype]
         dev = c.stream.device
         self.t0, self.flags, self.cum, self.anchors = _trailer(c, self.L)
-        self.range = stream_range(c)
+        # the range compress() returns is the one the trailer records
+        self.range = c.range if c.range is not None else stream_range(c)
         self.eb_abs = c.eb_input * self.range if c.rel else c.eb_input
         self.rel_eb = c.eb_input if c.rel else 0.0
-        self.state = torch.zeros(16, dtype=torch.int32, device=dev)  # begin[4] end[4] maxlevel narrow
+        small = torch.zeros(18, dtype=torch.int32, device=dev)  # one fill for state and the byte counter
+        self.state = small[0:16]  # begin[4] end[4] maxlevel narrow
         self.begin, self.end, self.maxlevel = self.state[0:4], self.state[4:8], self.state[8:9]
         self.narrow = self.state[9:10]  # codes of this round handed over as int16
         self.bp = _empty(self.L.host_size)
         self.qc = torch.empty(self.L.code_len, dtype=torch.int32, device=dev)
         self.dws = _empty(lib().pa_decode_workspace(C.byref(self.L)))
         self.sws = _empty(lib().pa_strategy_workspace())
-        self.bytes_read = torch.zeros(1, dtype=torch.int64, device=dev)
+        self.bytes_read = small[16:18].view(torch.int64)
         self.out = None
         self.rounds = 0
+        # argument values of the per-round calls
+        self._a = dict(cum=_p(self.cum), begin=_p(self.begin), end=_p(self.end), maxlevel=_p(self.maxlevel),
+                       sws=_p(self.sws), stream=_p(c.stream), bp=_p(self.bp), bytes_read=_p(self.bytes_read),
+                       dws=_p(self.dws), qc=_p(self.qc), narrow=_p(self.narrow), anchors=_p(self.anchors),
+                       flags=_p(self.flags), L=C.byref(self.L))
 
     def retrieve(self, target: float, timer=None) -> torch.Tensor:
         """Refines to the target error bound (relative in REL mode)."""
         tm = timer or _NO_TIMER
         st = _stream()
-        L = self.L
+        a = self._a
+        lb = lib()
         target_abs = target * self.range if self.c.rel else target
         with tm("strategy"):
-            check(lib().pa_find_strategy(_p(self.cum), _p(self.begin), _p(self.end), _p(self.maxlevel),
-                                         self.eb_abs, target_abs, _p(self.sws), self.sws.numel(), st),
-                  "pa_find_strategy")
+            check(lb.pa_find_strategy(a["cum"], a["begin"], a["end"], a["maxlevel"], self.eb_abs, target_abs,
+                                      a["sws"], self.sws.numel(), st), "pa_find_strategy")
         with tm("decode"):
-            check(lib().pa_lossless_decode_window(_p(self.c.stream), self.c.nbytes, C.byref(L), _p(self.begin), _p(self.end),
-                                              _p(self.bp), _p(self.bytes_read), int(self.rounds > 0),
-                                              _p(self.dws), self.dws.numel(), st), "pa_lossless_decode_window")
+            check(lb.pa_lossless_decode_window(a["stream"], self.c.nbytes, a["L"], a["begin"], a["end"], a["bp"],
+                                               a["bytes_read"], int(self.rounds > 0), a["dws"], self.dws.numel(), st),
+                  "pa_lossless_decode_window")
         with tm("unshuffle"):
-            check(lib().pa_inverse_bitshuffle_window_narrow(_p(self.bp), C.byref(L), _p(self.begin), _p(self.end),
-                                                            _p(self.qc), _p(self.narrow), st),
-                  "pa_inverse_bitshuffle_window_narrow")
+            check(lb.pa_inverse_bitshuffle_window_narrow(a["bp"], a["L"], a["begin"], a["end"], a["qc"], a["narrow"],
+                                                         st), "pa_inverse_bitshuffle_window_narrow")
         nx, ny, nz = self.c.shape
         new = torch.empty((nz, ny, nx), dtype=self.c.dtype, device=self.qc.device)
         with tm("reconstruct"):
@@ -453,16 +475,15 @@ class Retriever:
         self.rounds += 1
 
     def _reconstruct(self, new, st):
-        L = self.L
+        a = self._a
         if self.out is None:
-            check(lib().pa_spline_reconstruct_narrow(_p(self.qc), _p(self.narrow), _p(self.anchors), self.dt,
-                                                     C.byref(L), self.eb_abs, self.rel_eb, _p(self.flags),
-                                                     _p(self.maxlevel), _p(new), st), "pa_spline_reconstruct_narrow")
+            check(lib().pa_spline_reconstruct_narrow(a["qc"], a["narrow"], a["anchors"], self.dt, a["L"], self.eb_abs,
+                                                     self.rel_eb, a["flags"], a["maxlevel"], _p(new), st),
+                  "pa_spline_reconstruct_narrow")
         else:
-            check(lib().pa_spline_delta_reconstruct_narrow(_p(self.qc), _p(self.narrow), self.dt, C.byref(L),
-                                                           self.eb_abs, self.rel_eb, _p(self.flags),
-                                                           _p(self.maxlevel), _p(self.out), _p(new), st),
-                  "pa_spline_delta_reconstruct_narrow")
+            check(lib().pa_spline_delta_reconstruct_narrow(a["qc"], a["narrow"], self.dt, a["L"], self.eb_abs,
+                                                           self.rel_eb, a["flags"], a["maxlevel"], _p(self.out),
+                                                           _p(new), st), "pa_spline_delta_reconstruct_narrow")
 
 
 class RetrievalBatch:
