@@ -87,11 +87,6 @@ __device__ __forceinline__ uint32_t nz_bytes(uint32_t w) {
 // keep the ordering of LDS accesses of one wave across phases
 __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); __builtin_amdgcn_wave_barrier(); }
 
-__global__ void zero_words_kernel(uint64_t* p, int64_t n) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = 0;
-}
-
 constexpr int kWaves = 4;  // chunks per workgroup, one per wave64
 
 // LDS of one wave: the zero-elimination bitmap, one step's surviving bytes
@@ -102,16 +97,17 @@ constexpr int kWaves = 4;  // chunks per workgroup, one per wave64
 constexpr int kTailBytes = 2048 + 256 + 32 + 4 + 2;  // RE1 + RE2 + RE3 kept bytes, B3, size
 struct __align__(16) WaveSlot {
     uint16_t b0[kCS / 16];           // zero-elimination bitmap: 2 bytes per 16-byte segment
-    uint8_t step[1024];              // surviving bytes of one 1 KiB step
-    uint8_t tail[(kTailBytes + 15) / 16 * 16];
+    uint8_t step[1024 + 16];         // carried bytes (< 4) + surviving bytes of one 1 KiB step
+    uint8_t tail[(kTailBytes + 8 + 15) / 16 * 16];  // carried bytes + RE bytes and footer
     int zs[16];                      // surviving bytes per step
 };
 static_assert(sizeof(WaveSlot) % 16 == 0, "slots must keep 16-byte alignment");
 
-// n bytes of LDS to global memory at any byte offset, consecutive lanes on
-// consecutive bytes
-__device__ __forceinline__ void wave_copy_bytes(const uint8_t* src, uint8_t* dst, int n, int lane) {
-    for (int i = lane; i < n; i += 64) dst[i] = src[i];
+// n words of LDS to global memory, consecutive lanes on consecutive words
+__device__ __forceinline__ void wave_copy_words(const uint8_t* src, uint8_t* dst, int n, int lane) {
+    const uint32_t* s = reinterpret_cast<const uint32_t*>(src);
+    uint32_t* d = reinterpret_cast<uint32_t*>(dst);
+    for (int i = lane; i < n; i += 64) d[i] = s[i];
 }
 
 // One chunk per wave64.  Zero elimination streams the chunk in 16 steps of
@@ -158,12 +154,17 @@ __device__ __forceinline__ void encode_chunk(const uint8_t* __restrict__ bp, con
     wave_sync();
     if (!fail) {
         // the surviving bytes, step by step; a step's 16 bytes per lane are
-        // read again (from cache) rather than kept live in registers
+        // read again (from cache) rather than kept live in registers.  The
+        // staging slot is written in whole 4-byte words, each once: a step's
+        // bytes follow the r < 4 bytes carried from the previous step's last
+        // partial word, and its own last partial word is carried on (to the
+        // tail after the last step).
         int zpos = 0;
 #pragma unroll 1
         for (int k = 0; k < kSteps; ++k) {
             const int zs = W.zs[k];
             if (zs == 0) continue;  // wave-uniform
+            const int r = zpos & 3;
             const int b = 16 * (64 * k + lane);
             uint32_t w[4] = {0u, 0u, 0u, 0u};
             if (b < osize) {
@@ -178,17 +179,24 @@ __device__ __forceinline__ void encode_chunk(const uint8_t* __restrict__ bp, con
             }
             const uint32_t m = W.b0[64 * k + lane];
             const int cnt = __popc(m);
-            int pos = wave_incl_scan(cnt) - cnt;
+            int pos = r + wave_incl_scan(cnt) - cnt;
 #pragma unroll
             for (int i = 0; i < 16; ++i)
                 if ((m >> i) & 1u) W.step[pos++] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
             wave_sync();
-            wave_copy_bytes(W.step, dst + zpos, zs, lane);
+            const int nfull = (r + zs) >> 2;
+            wave_copy_words(W.step, dst + zpos - r, nfull, lane);
+            wave_sync();  // the copy has read the buffer
+            if (lane == 0) reinterpret_cast<uint32_t*>(W.step)[0] = reinterpret_cast<const uint32_t*>(W.step)[nfull];
             wave_sync();  // the next step reuses the buffer
             zpos += zs;
         }
     }
-    int wpos = Z;  // stream position; RE bytes go to tail[wpos - Z]
+    // RE bytes go to tail[zr + wpos - Z], after the zr carried bytes of the
+    // last partial word of the zero-eliminated bytes
+    const int zr = Z & 3;
+    if (lane == 0) reinterpret_cast<uint32_t*>(W.tail)[0] = reinterpret_cast<const uint32_t*>(W.step)[0];
+    int wpos = Z;  // stream position
     uint32_t b3 = 0;
     wave_sync();
     if (Z != 0 && !fail) {  // (a chunk that already failed is raw whatever follows)
@@ -211,7 +219,7 @@ __device__ __forceinline__ void encode_chunk(const uint8_t* __restrict__ bp, con
         int R = __shfl(incl, 63, 64);
         fail = fail || wpos + R > kAvail;
         if (!fail) {
-            int pos = wpos - Z + incl - cnt;
+            int pos = zr + wpos - Z + incl - cnt;
 #pragma unroll
             for (int i = 0; i < 32; ++i)
                 if ((f1 >> i) & 1u) W.tail[pos++] = (uint8_t)(v[i >> 2] >> (8 * (i & 3)));
@@ -226,7 +234,7 @@ __device__ __forceinline__ void encode_chunk(const uint8_t* __restrict__ bp, con
         R = __shfl(incl, 63, 64);
         fail = fail || wpos + R > kAvail;
         if (!fail) {
-            int pos = wpos - Z + incl - cnt;
+            int pos = zr + wpos - Z + incl - cnt;
 #pragma unroll
             for (int i = 0; i < 4; ++i)
                 if ((f2 >> i) & 1u) W.tail[pos++] = (uint8_t)(f1 >> (8 * i));
@@ -241,7 +249,7 @@ __device__ __forceinline__ void encode_chunk(const uint8_t* __restrict__ bp, con
         b3 = (uint32_t)m3;
         R = __popcll(m3);
         fail = fail || wpos + R > kAvail;
-        if (!fail && keep) W.tail[wpos - Z + __popcll(m3 & ((1ull << lane) - 1))] = (uint8_t)b2;
+        if (!fail && keep) W.tail[zr + wpos - Z + __popcll(m3 & ((1ull << lane) - 1))] = (uint8_t)b2;
         wpos += R;
         fail = fail || wpos >= kAvail - 4;
     }
@@ -251,15 +259,16 @@ __device__ __forceinline__ void encode_chunk(const uint8_t* __restrict__ bp, con
     if (!raw && lane == 0) {
         if (Z != 0) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) W.tail[wpos - Z + j] = (uint8_t)(b3 >> (8 * j));
+            for (int j = 0; j < 4; ++j) W.tail[zr + wpos - Z + j] = (uint8_t)(b3 >> (8 * j));
         }
-        W.tail[csize - 2 - Z] = (uint8_t)(osize & 0xFF);
-        W.tail[csize - 1 - Z] = (uint8_t)((osize >> 8) & 0xFF);
+        W.tail[zr + csize - 2 - Z] = (uint8_t)(osize & 0xFF);
+        W.tail[zr + csize - 1 - Z] = (uint8_t)((osize >> 8) & 0xFF);
     }
     // RE bytes and footer after the zero-eliminated bytes in the staging
-    // slot; a raw chunk is copied later straight from the bit-planes
+    // slot (the last word may run up to 3 bytes past csize < kCS, inside the
+    // slot); a raw chunk is copied later straight from the bit-planes
     wave_sync();
-    if (!raw) wave_copy_bytes(W.tail, dst + Z, csize - Z, lane);
+    if (!raw) wave_copy_words(W.tail, dst + Z - zr, (zr + csize - Z + 3) >> 2, lane);
     if (lane == 0) reinterpret_cast<uint16_t*>(dst_stream + 8)[id] = (uint16_t)csize;
 }
 
@@ -448,8 +457,10 @@ __device__ __forceinline__ void size_scan(const uint16_t* __restrict__ sizes, in
     }
 }
 
+// also clears the decoder's error words (the 64-byte slot before the offsets)
 __global__ __launch_bounds__(1024) void size_scan_kernel(const uint16_t* __restrict__ sizes, int64_t n,
-                                                         int64_t* __restrict__ offsets) {
+                                                         int64_t* __restrict__ offsets, uint64_t* __restrict__ err) {
+    if (threadIdx.x < 8) err[threadIdx.x] = 0ull;
     size_scan(sizes, n, offsets);
 }
 
@@ -520,12 +531,34 @@ __device__ __forceinline__ uint64_t window_bytes(const Window& w, const int64_t*
 // Decoder: one workgroup per chunk (chunks are independent once the
 // offsets are known).  The windowed form walks the selected id ranges.
 struct DecodeLds {
-    uint32_t c[kCS / 4 + 2];
+    uint32_t c[kCS / 4 + 4];  // payload; the last words are read past its end and discarded
     uint64_t b0[256];
+    uint64_t sel[256];  // byte-permute selectors of each 8-bit zero mask
+    int e0[256];        // first payload byte of each lane's 64 output bytes
     uint8_t b1[256];
     uint8_t b2[32];
     int scan[8];
 };
+
+// Selector of mask m for v_perm_b32 over 8 payload bytes: output byte j is
+// payload byte popc(m & (2^j - 1)) when bit j of m is set, else 0 (selector
+// 0x0C).  Written once per workgroup; read after the chunk's barriers.
+__device__ __forceinline__ void init_selectors(DecodeLds& S) {
+    const uint32_t m = threadIdx.x;
+    uint64_t sel = 0;
+    int c = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) sel |= (uint64_t)((m >> j) & 1u ? c++ : 0x0C) << (8 * j);
+    S.sel[m] = sel;
+}
+
+// 8 payload bytes from byte position b of the LDS words
+__device__ __forceinline__ void bytes8(const uint32_t* w, int b, uint32_t& lo, uint32_t& hi) {
+    const int q = b >> 2, o = b & 3;
+    const uint32_t w0 = w[q], w1 = w[q + 1], w2 = w[q + 2];
+    lo = __builtin_amdgcn_alignbyte(w1, w0, o);
+    hi = __builtin_amdgcn_alignbyte(w2, w1, o);
+}
 
 __device__ void decode_chunk(const uint8_t* __restrict__ in, const ChunkGeom& g, const int64_t* __restrict__ offsets,
                              uint8_t* __restrict__ bp, int32_t* __restrict__ err, int64_t in_len, int64_t id,
@@ -576,10 +609,9 @@ __device__ void decode_chunk(const uint8_t* __restrict__ in, const ChunkGeom& g,
     const uint8_t* cb = reinterpret_cast<const uint8_t*>(s_c) + shift;
     if (csize == osize) {  // stored raw
         for (int i = tid; i < nwords; i += kThreads) {
-            uint64_t v = 0;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) v |= (uint64_t)cb[8 * i + j] << (8 * j);
-            dst[i] = v;
+            uint32_t lo, hi;
+            bytes8(s_c, shift + 8 * i, lo, hi);
+            dst[i] = (uint64_t)hi << 32 | lo;
         }
         return;
     }
@@ -635,19 +667,29 @@ __device__ void decode_chunk(const uint8_t* __restrict__ in, const ChunkGeom& g,
     const int e0 = block_scan(__popcll(f0), s_scan, &n0);
     bad = bad || d3 < 0 || d2 < 0 || d1 < 0 || n0 != d1;
     if (bad && tid == 0) atomicOr(err, 4);
-    int c = e0;
+    // Output word wi holds bytes 8 sub .. 8 sub + 7 of lane t = wi / 8's 64
+    // (sub = wi mod 8); its nonzero bytes are the next popc(mask) payload
+    // bytes from e0[t] plus the set bits of f0[t] below byte sub.  Lanes
+    // take consecutive words, so stores are contiguous and neighbouring
+    // lanes read neighbouring payload bytes.
+    S.e0[tid] = e0;
+    __syncthreads();
 #pragma unroll
-    for (int w = 0; w < 8; ++w) {
+    for (int k = 0; k < 8; ++k) {
+        const int wi = tid + k * kThreads;
+        if (wi >= nwords) break;
+        const int t = wi >> 3, sub = wi & 7;
+        const uint64_t f = s_b0[t];
+        const uint32_t m = (uint32_t)(f >> (8 * sub)) & 0xFFu;
         uint64_t v = 0;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            if ((f0 >> (8 * w + j)) & 1ull) {
-                v |= (uint64_t)rd(c) << (8 * j);
-                ++c;
-            }
+        if (m != 0 && !bad) {  // consistent sizes keep every read inside the payload (n0 = d1 < csize)
+            uint32_t lo, hi;
+            bytes8(s_c, shift + S.e0[t] + __popcll(f & ((1ull << (8 * sub)) - 1ull)), lo, hi);
+            const uint64_t sel = S.sel[m];
+            v = (uint64_t)__builtin_amdgcn_perm(hi, lo, (uint32_t)(sel >> 32)) << 32 |
+                __builtin_amdgcn_perm(hi, lo, (uint32_t)sel);
         }
-        const int wi = tid * 8 + w;
-        if (wi < nwords) dst[wi] = bad ? 0ull : v;
+        dst[wi] = v;
     }
 }
 
@@ -660,6 +702,7 @@ __global__ __launch_bounds__(kThreads) void decode_kernel(const uint8_t* __restr
                                                           uint64_t* __restrict__ bytes_read,
                                                           int32_t* __restrict__ err, int64_t in_len) {
     __shared__ DecodeLds S;
+    init_selectors(S);
     if (!WINDOW) {
         decode_chunk(in, g, offsets, bp, err, in_len, blockIdx.x, S);
         return;
@@ -734,6 +777,7 @@ __global__ __launch_bounds__(1024) void retr_count_scan_kernel(const RetrJob* __
 __global__ __launch_bounds__(kThreads) void retr_decode_kernel(const RetrJob* __restrict__ jobs, int nj,
                                                                const int64_t* __restrict__ prefix) {
     __shared__ DecodeLds S;
+    init_selectors(S);
     const int64_t total = prefix[nj];
     for (int64_t i = blockIdx.x; i < total; i += gridDim.x) {
         const int j = find_field(prefix, 0, nj, i);
@@ -819,10 +863,9 @@ static int decode_common(const uint8_t* d_in, int64_t in_len, const pa_layout* L
     int32_t* err = static_cast<int32_t*>(d_ws);
     int64_t* offsets = reinterpret_cast<int64_t*>(static_cast<uint8_t*>(d_ws) + 64);
     if (!offsets_ready) {
-        hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<uint64_t*>(d_ws), 8);
-        PA_LAUNCH_CHECK();
         hipLaunchKernelGGL(size_scan_kernel, dim3(1), dim3(1024), 0, st,
-                           reinterpret_cast<const uint16_t*>(d_in + 8), L->nchunks, offsets);
+                           reinterpret_cast<const uint16_t*>(d_in + 8), L->nchunks, offsets,
+                           reinterpret_cast<uint64_t*>(d_ws));
         PA_LAUNCH_CHECK();
     }
     if (window) {
