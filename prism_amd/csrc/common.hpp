@@ -197,13 +197,16 @@ __device__ __forceinline__ int find_field(const int64_t* prefix, int lo, int hi,
 // Level-major position of a point of level LEVEL (< 4): its rank among the
 // level's points in raster order, after the region start (compaction
 // formula, InterpolationPredictor.cu:2070-2083).
+// The code array is shorter than 2^31 (layout creation rejects longer
+// ones), so the index is evaluated in 32-bit unsigned arithmetic: every
+// partial sum is below 2^32 and the result below 2^31.
 template <int LEVEL>
-__device__ __forceinline__ int64_t lm_index(const Geom& g, int gx, int gy, int gz) {
-    const int64_t a = gx >> LEVEL, b = gy >> LEVEL, c = gz >> LEVEL;
-    int64_t idx = a + b * g.lx[LEVEL] + c * g.lxy[LEVEL] + g.prefix[LEVEL] -
-                  ((c + 1) >> 1) * g.lxy[LEVEL + 1];
+__device__ __forceinline__ uint32_t lm_index(const Geom& g, int gx, int gy, int gz) {
+    const uint32_t a = (uint32_t)gx >> LEVEL, b = (uint32_t)gy >> LEVEL, c = (uint32_t)gz >> LEVEL;
+    uint32_t idx = a + b * (uint32_t)g.lx[LEVEL] + c * (uint32_t)g.lxy[LEVEL] + (uint32_t)g.prefix[LEVEL] -
+                   ((c + 1) >> 1) * (uint32_t)g.lxy[LEVEL + 1];
     if ((c & 1) == 0) {
-        idx -= ((b + 1) >> 1) * g.lx[LEVEL + 1];
+        idx -= ((b + 1) >> 1) * (uint32_t)g.lx[LEVEL + 1];
         if ((b & 1) == 0) idx -= (a + 1) >> 1;
     }
     return idx;

@@ -468,11 +468,11 @@ __device__ __forceinline__ T apply_stencil(int k, T t0, T t1, T t2, T t3) {
 }
 
 struct LineLane {
-    int base;      // LDS index of the line's c = 0 point
-    bool on;       // line inside the tile's sweep and the field
-    bool own;      // line inside the 16^3 tile proper (codes are stored)
-    int64_t idx0;  // level-major index of point j = 0
-    int64_t st;    // level-major stride between consecutive points
+    int base;       // LDS index of the line's c = 0 point
+    bool on;        // line inside the tile's sweep and the field
+    bool own;       // line inside the 16^3 tile proper (codes are stored)
+    uint32_t idx0;  // level-major index of point j = 0
+    uint32_t st;    // level-major stride between consecutive points
 };
 
 template <typename T, int U, int K, bool EXCL, bool IN>
@@ -500,7 +500,7 @@ __device__ __forceinline__ LineLane line_lane(const Ctx<T, IN>& t, int it) {
     if constexpr (S::N >= 2) {
         // the index is affine along the line: the varying coordinate keeps
         // its parity and its halves grow by one per point
-        const int64_t i1 = S::A == 0 ? lm_index<LEVEL>(t.g, gx + 2 * U, gy, gz)
+        const uint32_t i1 = S::A == 0 ? lm_index<LEVEL>(t.g, gx + 2 * U, gy, gz)
                                      : (S::A == 1 ? lm_index<LEVEL>(t.g, gx, gy + 2 * U, gz)
                                                   : lm_index<LEVEL>(t.g, gx, gy, gz + 2 * U));
         r.st = i1 - r.idx0;
@@ -550,7 +550,7 @@ constexpr int kHollow0Iters = 8;
 struct ColumnLane {
     int x, y, z0;
     bool on;
-    int64_t idx0, dz;
+    uint32_t idx0, dz;
 };
 
 template <typename T, bool IN>
@@ -562,7 +562,7 @@ __device__ __forceinline__ ColumnLane column_lane(const Ctx<T, IN>& t) {
     c.z0 = l >> 7;
     c.on = t.template inf<0>(t.ox + c.x) && t.template inf<1>(t.oy + c.y);
     c.idx0 = lm_index<0>(t.g, t.ox + c.x, t.oy + c.y, t.oz + c.z0);
-    c.dz = 2 * t.g.lxy[0] - t.g.lxy[1];  // z -> z + 2 keeps the parity of z
+    c.dz = (uint32_t)(2 * t.g.lxy[0] - t.g.lxy[1]);  // z -> z + 2 keeps the parity of z
     return c;
 }
 
@@ -646,10 +646,12 @@ constexpr int row_off() {
 
 template <int L>
 __device__ __forceinline__ int32_t row_base(const Geom& g, int gy, int gz) {
-    const int64_t b = gy >> L, c = gz >> L;
-    int64_t r = b * g.lx[L] + c * g.lxy[L] + g.prefix[L] - ((c + 1) >> 1) * g.lxy[L + 1];
-    if ((c & 1) == 0) r -= ((b + 1) >> 1) * g.lx[L + 1];
-    return (int32_t)r;  // < code_len < 2^31
+    // 32-bit unsigned arithmetic, exact below code_len < 2^31 (as lm_index)
+    const uint32_t b = (uint32_t)gy >> L, c = (uint32_t)gz >> L;
+    uint32_t r = b * (uint32_t)g.lx[L] + c * (uint32_t)g.lxy[L] + (uint32_t)g.prefix[L] -
+                 ((c + 1) >> 1) * (uint32_t)g.lxy[L + 1];
+    if ((c & 1) == 0) r -= ((b + 1) >> 1) * (uint32_t)g.lx[L + 1];
+    return (int32_t)r;
 }
 
 template <typename T, int L, bool IN>
@@ -680,21 +682,56 @@ struct FillShape {
     static constexpr int ITERS = (TOTAL + kThreads - 1) / kThreads;
 };
 
+// Row r0 + 16 it of the 17 x 17 rows of a tile (r0 = lane / 16 < 16):
+// (y, z) = (r0 - it, it), or (r0 - it + 17, it - 1) when r0 < it (16 is
+// -1 mod 17), and (16, 16) for the last row (it = 18, r0 = 0 only).
+struct RowYZ {
+    int y, z;
+    bool valid;
+};
+template <int IT>
+__device__ __forceinline__ RowYZ row_yz(int r0) {
+    RowYZ w;
+    if constexpr (IT == kFill0Iters - 1) {
+        w.y = 16;
+        w.z = 16;
+        w.valid = r0 == 0;
+    } else {
+        const bool wrap = r0 < IT;
+        w.y = r0 - IT + (wrap ? kTS : 0);
+        w.z = IT - (wrap ? 1 : 0);
+        w.valid = true;
+    }
+    return w;
+}
+
+template <typename T, int IT, bool IN>
+__device__ __forceinline__ void fill0_one(const Ctx<T, IN>& t, const int32_t* tab, bool load, int r0, int x, bool xin,
+                                          uint32_t off_full, uint32_t off_half, int32_t& v) {
+    const RowYZ w = row_yz<IT>(r0);
+    const bool full = ((w.y | w.z) & 1) != 0;
+    v = 0;
+    if (load && w.valid && xin && (full || (x & 1)) && t.template inf<1>(t.oy + w.y) && t.template inf<2>(t.oz + w.z))
+        v = code_at(t.qc_in, t.q16, (uint32_t)tab[r0 + 16 * IT] + (full ? off_full : off_half));
+}
+
+template <typename T, int IT, bool IN>
+__device__ __forceinline__ void fill0_iter(const Ctx<T, IN>& t, const int32_t* tab, bool load, int r0, int x,
+                                           bool xin, uint32_t off_full, uint32_t off_half,
+                                           int32_t (&v)[kFill0Iters]) {
+    if constexpr (IT < kFill0Iters) {
+        fill0_one<T, IT>(t, tab, load, r0, x, xin, off_full, off_half, v[IT]);
+        fill0_iter<T, IT + 1>(t, tab, load, r0, x, xin, off_full, off_half, v);
+    }
+}
+
 template <typename T, bool IN>
 __device__ __forceinline__ void fill0_load(const Ctx<T, IN>& t, const int32_t* tab, bool load,
                                            int32_t (&v)[kFill0Iters], int32_t (&e)[kFill0EdgeIters]) {
     const int lane = lane_id();
-    const int x = lane & 15;
+    const int x = lane & 15, r0 = lane >> 4;
     const bool xin = t.template inf<0>(t.ox + x);
-#pragma unroll
-    for (int it = 0; it < kFill0Iters; ++it) {
-        const int r = (lane >> 4) + 16 * it;
-        const int y = r % kTS, z = r / kTS;
-        const bool full = ((y | z) & 1) != 0;
-        v[it] = 0;
-        if (load && r < kRows0 && xin && (full || (x & 1)) && t.template inf<1>(t.oy + y) && t.template inf<2>(t.oz + z))
-            v[it] = code_at(t.qc_in, t.q16, (uint32_t)(tab[r] + (full ? t.ox + x : (t.ox >> 1) + (x >> 1))));
-    }
+    fill0_iter<T, 0>(t, tab, load, r0, x, xin, (uint32_t)(t.ox + x), (uint32_t)((t.ox >> 1) + (x >> 1)), v);
 #pragma unroll
     for (int it = 0; it < kFill0EdgeIters; ++it) {  // x = 16: level 0 where y or z is odd
         const int r = lane + kThreads * it;
@@ -702,7 +739,16 @@ __device__ __forceinline__ void fill0_load(const Ctx<T, IN>& t, const int32_t* t
         e[it] = 0;
         if (load && r < kRows0 && ((y | z) & 1) && t.template inf<0>(t.ox + 16) && t.template inf<1>(t.oy + y) &&
             t.template inf<2>(t.oz + z))
-            e[it] = code_at(t.qc_in, t.q16, (uint32_t)(tab[r] + t.ox + 16));
+            e[it] = code_at(t.qc_in, t.q16, (uint32_t)tab[r] + (uint32_t)(t.ox + 16));
+    }
+}
+
+template <typename T, int IT>
+__device__ __forceinline__ void fill0_put(T* s, int r0, int x, const int32_t (&v)[kFill0Iters]) {
+    if constexpr (IT < kFill0Iters) {
+        const RowYZ w = row_yz<IT>(r0);
+        if (w.valid && (((w.y | w.z) & 1) || (x & 1))) s[(r0 + 16 * IT) * kTS + x] = code_slot(v[IT], T());
+        fill0_put<T, IT + 1>(s, r0, x, v);
     }
 }
 
@@ -710,13 +756,8 @@ template <typename T>
 __device__ __forceinline__ void fill0_store(T* s, const int32_t (&v)[kFill0Iters],
                                             const int32_t (&e)[kFill0EdgeIters]) {
     const int lane = lane_id();
-    const int x = lane & 15;
-#pragma unroll
-    for (int it = 0; it < kFill0Iters; ++it) {
-        const int r = (lane >> 4) + 16 * it;
-        const int y = r % kTS, z = r / kTS;
-        if (r < kRows0 && (((y | z) & 1) || (x & 1))) s[r * kTS + x] = code_slot(v[it], T());
-    }
+    const int x = lane & 15, r0 = lane >> 4;
+    fill0_put<T, 0>(s, r0, x, v);
 #pragma unroll
     for (int it = 0; it < kFill0EdgeIters; ++it) {
         const int r = lane + kThreads * it;
@@ -783,25 +824,49 @@ __device__ __forceinline__ LevelEb<T> load_level_ebs(const double* e) {
 }
 
 // Tile plus halo into LDS: all global loads of a lane are issued before the
-// first LDS write so the tile costs one memory latency.
+// first LDS write so the tile costs one memory latency.  Lane l takes
+// x = l mod 16 of the rows r = l / 16 + 16 it (row r = 17 z + y); since
+// 16 = 17 - 1, row r0 + 16 it (it < 18) is (y, z) = (r0 - it, it) when
+// r0 >= it and (r0 - it + 17, it - 1) otherwise, so row positions and field
+// offsets are a compile-time step plus one select per row; the last row
+// (288, taken by r0 = 0 at it = 18) is (16, 16).  The x = 16 column is a
+// second, short pass.  Field offsets below 2^31 (the layout's limit) are
+// 32-bit.
 template <typename T, bool IN>
 __device__ __forceinline__ void load_tile(const Ctx<T, IN>& t, const T* data) {
-    constexpr int IT = (kTileVals + kThreads - 1) / kThreads;
-    T v[IT];
     const int lane = lane_id();
+    const int x = lane & 15, r0 = lane >> 4;
+    const uint32_t nx = (uint32_t)t.g.nx, nxy = nx * (uint32_t)t.g.ny;
+    const bool xin = t.template inf<0>(t.ox + x);
+    const T* base = data + ((uint32_t)t.oz * nxy + (uint32_t)t.oy * nx + (uint32_t)t.ox);
+    const uint32_t lane_off = (uint32_t)r0 * nx + (uint32_t)x;
+    const uint32_t step = nxy - nx, wrap_off = 17 * nx - nxy;  // per row step; back by 17 rows of y
+    T v[kFill0Iters], e[kFill0EdgeIters];
 #pragma unroll
-    for (int it = 0; it < IT; ++it) {
-        const int i = lane + it * kThreads;
-        const int x = i % kTS, y = (i / kTS) % kTS, z = i / (kTS * kTS);
-        const int gx = t.ox + x, gy = t.oy + y, gz = t.oz + z;
+    for (int it = 0; it < kFill0Iters; ++it) {
+        const bool last = it == kFill0Iters - 1, wrap = r0 < it;
+        const int y = last ? 16 : r0 - it + (wrap ? kTS : 0), z = last ? 16 : it - (wrap ? 1 : 0);
+        const uint32_t off =
+            last ? 16 * nx + 16 * nxy + (uint32_t)x : lane_off + (uint32_t)it * step + (wrap ? wrap_off : 0u);
         v[it] = (T)0;
-        if (i < kTileVals && t.template inf<0>(gx) && t.template inf<1>(gy) && t.template inf<2>(gz))
-            v[it] = data[(int64_t)gx + (int64_t)gy * t.g.nx + (int64_t)gz * t.g.nx * t.g.ny];
+        if ((it < kFill0Iters - 1 || r0 == 0) && xin && t.template inf<1>(t.oy + y) && t.template inf<2>(t.oz + z))
+            v[it] = base[off];
     }
 #pragma unroll
-    for (int it = 0; it < IT; ++it) {
-        const int i = lane + it * kThreads;
-        if (i < kTileVals) t.s[i] = v[it];
+    for (int it = 0; it < kFill0EdgeIters; ++it) {  // x = 16
+        const int r = lane + kThreads * it;
+        const int y = r % kTS, z = r / kTS;
+        e[it] = (T)0;
+        if (r < kRows0 && t.template inf<0>(t.ox + 16) && t.template inf<1>(t.oy + y) && t.template inf<2>(t.oz + z))
+            e[it] = base[(uint32_t)y * nx + (uint32_t)z * nxy + 16u];
+    }
+#pragma unroll
+    for (int it = 0; it < kFill0Iters; ++it)
+        if (it < kFill0Iters - 1 || r0 == 0) t.s[(r0 + 16 * it) * kTS + x] = v[it];
+#pragma unroll
+    for (int it = 0; it < kFill0EdgeIters; ++it) {
+        const int r = lane + kThreads * it;
+        if (r < kRows0) t.s[r * kTS + 16] = e[it];
     }
 }
 
@@ -900,7 +965,11 @@ __device__ __forceinline__ void tile_run(const SplineArgs<T>& a, T* tile, int32_
     if (3 <= maxlevel) { t.eb_r = le.eb_r[3]; t.ebx2 = le.ebx2[3]; level<T, 8, M>(t, fl[3], fl[7]); }
     if (2 <= maxlevel) { t.eb_r = le.eb_r[2]; t.ebx2 = le.ebx2[2]; level<T, 4, M>(t, fl[2], fl[6]); }
     if (1 <= maxlevel) { t.eb_r = le.eb_r[1]; t.ebx2 = le.ebx2[1]; level<T, 2, M>(t, fl[1], fl[5]); }
-    if (0 <= maxlevel) { t.eb_r = le.eb_r[0]; t.ebx2 = le.ebx2[0]; level<T, 1, M>(t, fl[0], fl[4]); }
+    if (0 <= maxlevel) {
+        t.eb_r = le.eb_r[0];
+        t.ebx2 = le.ebx2[0];
+        level<T, 1, M>(t, fl[0], fl[4]);
+    }
 
     if constexpr (M == DECOMPRESS) {  // lane: (x, y); iteration: z
         const int lane = lane_id();
@@ -1054,14 +1123,14 @@ __device__ __forceinline__ void plane_tiles(const SplineArgs<T>& a, T* lds, int6
                 if (lv == 4) {
                     if (a.anchors_in) anc[it] = a.anchors_in[(gx >> 4) + (int64_t)(gy >> 4) * g.gx];
                 } else if (lv <= maxlevel) {
-                    int64_t idx;
+                    uint32_t idx;
                     switch (lv) {
                         case 0: idx = lm_index<0>(g, gx, gy, 0); break;
                         case 1: idx = lm_index<1>(g, gx, gy, 0); break;
                         case 2: idx = lm_index<2>(g, gx, gy, 0); break;
                         default: idx = lm_index<3>(g, gx, gy, 0); break;
                     }
-                    q[it] = code_at(t.qc_in, t.q16, (uint32_t)idx);
+                    q[it] = code_at(t.qc_in, t.q16, idx);
                 }
             }
         }
