@@ -203,8 +203,10 @@ extern "C" int pa_extrema(const void* d_data, int dtype, int64_t n, double* d_mi
     unsigned long long* keys = reinterpret_cast<unsigned long long*>(d_minmax + 2);
     hipLaunchKernelGGL(extrema_init_kernel, dim3(1), dim3(1), 0, st, keys);
     PA_LAUNCH_CHECK();
+    // 1024 workgroups (4 per CU) streaming contiguous ranges read faster on
+    // MI355X than 2048 or 4096 (78 us against 90 us for 302 MB)
     const int64_t work = (n + 255) / 256;
-    const int blocks = (int)(work < 2048 ? work : 2048);
+    const int blocks = (int)(work < 1024 ? work : 1024);
     if (dtype == PA_F32)
         hipLaunchKernelGGL(extrema_kernel<float>, dim3(blocks), dim3(256), 0, st, (const float*)d_data, n, keys);
     else
