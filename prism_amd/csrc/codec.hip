@@ -159,14 +159,9 @@ __device__ __forceinline__ void encode_chunk(const uint8_t* __restrict__ bp, con
         // bytes follow the r < 4 bytes carried from the previous step's last
         // partial word, and its own last partial word is carried on (to the
         // tail after the last step).
-        int zpos = 0;
-#pragma unroll 1
-        for (int k = 0; k < kSteps; ++k) {
-            const int zs = W.zs[k];
-            if (zs == 0) continue;  // wave-uniform
-            const int r = zpos & 3;
+        auto load_step = [&](int k, uint32_t (&w)[4]) {
             const int b = 16 * (64 * k + lane);
-            uint32_t w[4] = {0u, 0u, 0u, 0u};
+            w[0] = w[1] = w[2] = w[3] = 0u;
             if (b < osize) {
                 const uint2 lo = *reinterpret_cast<const uint2*>(src + b);
                 w[0] = lo.x;
@@ -177,6 +172,17 @@ __device__ __forceinline__ void encode_chunk(const uint8_t* __restrict__ bp, con
                     w[3] = hi.y;
                 }
             }
+        };
+        int zpos = 0;
+        uint32_t nx[4];
+        load_step(0, nx);
+#pragma unroll 1
+        for (int k = 0; k < kSteps; ++k) {
+            uint32_t w[4] = {nx[0], nx[1], nx[2], nx[3]};
+            if (k + 1 < kSteps) load_step(k + 1, nx);  // the next step's bytes load during this one
+            const int zs = W.zs[k];
+            if (zs == 0) continue;  // wave-uniform
+            const int r = zpos & 3;
             const uint32_t m = W.b0[64 * k + lane];
             const int cnt = __popc(m);
             int pos = r + wave_incl_scan(cnt) - cnt;
@@ -358,10 +364,22 @@ __device__ __forceinline__ void gather_chunk(const uint8_t* __restrict__ bp, con
     if (lane < head) d[lane] = sb[lane];
     const int nw = (csize - head) >> 2;
     uint32_t* dw = reinterpret_cast<uint32_t*>(d + head);
-    for (int w = lane; w < nw; w += 64) {
-        const uint32_t lo = sw[w];
-        const uint32_t hi = w + 1 < swords ? sw[w + 1] : 0u;
-        dw[w] = head == 0 ? lo : (lo >> (8 * head)) | (hi << (32 - 8 * head));
+    // batches of 8 words per lane: all 16 loads of a batch are in flight
+    // before the first store (a chunk is up to 4096 words)
+    constexpr int kB = 8;
+    for (int w0 = lane; w0 < nw; w0 += 64 * kB) {
+        uint32_t lo[kB], hi[kB];
+#pragma unroll
+        for (int k = 0; k < kB; ++k) {
+            const int w = w0 + 64 * k;
+            lo[k] = w < nw ? sw[w] : 0u;
+            hi[k] = w < nw && w + 1 < swords ? sw[w + 1] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < kB; ++k) {
+            const int w = w0 + 64 * k;
+            if (w < nw) dw[w] = head == 0 ? lo[k] : (lo[k] >> (8 * head)) | (hi[k] << (32 - 8 * head));
+        }
     }
     const int tail0 = head + 4 * nw;
     if (lane < csize - tail0) d[tail0 + lane] = sb[tail0 + lane];
