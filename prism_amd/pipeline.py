@@ -13,16 +13,34 @@ import ctypes as C
 import threading
 from dataclasses import dataclass, field
 
+import numpy as np
 import torch
 
 from . import _lib
-from ._lib import PA_F32, PA_F64, PA_MODE_NB, PA_MODE_SM, check, layout, lib
+from ._lib import PA_F32, PA_F64, PA_MODE_NB, PA_MODE_SM, check, lib
+from ._lib import layout as _new_layout
 
 DTYPES = {torch.float32: PA_F32, torch.float64: PA_F64}
 
 
 def _stream():
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+_LAYOUTS: dict = {}
+
+
+def layout(nx: int, ny: int = 1, nz: int = 1):
+    """The pa_layout of a shape, built once per shape and shared (every
+    entry point takes it as const)."""
+    key = (nx, ny, nz)
+    L = _LAYOUTS.get(key)
+    if L is None:
+        L = _new_layout(nx, ny, nz)
+        if len(_LAYOUTS) > 256:
+            _LAYOUTS.clear()
+        _LAYOUTS[key] = L
+    return L
 
 
 def _p(t):
@@ -41,9 +59,9 @@ def _empty(nbytes: int, dtype=torch.uint8):
 _TLS = threading.local()
 
 
-def _host_read(t: torch.Tensor) -> torch.Tensor:
+def _host_read(t: torch.Tensor) -> np.ndarray:
     """Copies a small device tensor to the host through a pinned per-thread
-    buffer and waits for the current stream; returns a fresh CPU tensor."""
+    buffer and waits for the current stream; returns a fresh numpy copy."""
     nb = t.numel() * t.element_size()
     buf = getattr(_TLS, "pinned", None)
     if buf is None or buf.numel() < nb:
@@ -52,7 +70,7 @@ def _host_read(t: torch.Tensor) -> torch.Tensor:
     h = buf[:nb].view(t.dtype)
     h.copy_(t.reshape(-1), non_blocking=True)
     torch.cuda.current_stream().synchronize()
-    return h.clone()
+    return h.numpy().copy()
 
 
 def shape_xyz(t: torch.Tensor):
@@ -169,16 +187,16 @@ def compress(data: torch.Tensor, eb: float, rel: bool = True, progressive: bool 
     if not eb > 0:
         raise ValueError("error bound must be positive")
     data = data.contiguous()
+    mm = None
+    if rel:  # first, so the device starts while the host prepares the rest
+        with tm("extrema"):
+            mm = extrema(data)
     nx, ny, nz = shape_xyz(data)
     L = layout(nx, ny, nz)
     dt = DTYPES[data.dtype]
     st = _stream()
     dev = data.device
     bounds = torch.empty(16, dtype=torch.float64, device=dev)
-    mm = None
-    if rel:
-        with tm("extrema"):
-            mm = extrema(data)
     check(lib().pa_level_bounds(_p(mm), float(eb), dt, _p(bounds), st), "pa_level_bounds")
     flags = torch.empty(8, dtype=torch.uint8, device=dev)
     scores = torch.empty(15, dtype=torch.float64, device=dev)
@@ -208,10 +226,10 @@ def compress(data: torch.Tensor, eb: float, rel: bool = True, progressive: bool 
                                       _p(anchors), _p(bounds[15:16]), st), "pa_compress_merge")
     # the one host read: stream length, status, range
     vals = _host_read(bounds)
-    if vals[14].item() != 0:
+    if vals[14] != 0:
         raise ValueError("error bound must be positive (constant field in REL mode?)")
-    total = int(vals[15:16].view(torch.int64).item())
-    rng = float(vals[1].item()) if rel else 1.0
+    total = int(vals.view(np.int64)[15])
+    rng = float(vals[1]) if rel else 1.0
     c = Compressed(stream=out[:total], shape=(nx, ny, nz), dtype=data.dtype, eb_input=eb, rel=rel,
                    progressive=progressive, range=rng, flags=[])
     if return_parts:
