@@ -888,10 +888,9 @@ struct SplineArgs {
     T* out;
 };
 
-// One tile: compress (load, predict, quantise, store codes) or decompress
-// (fill the code slots, reconstruct, add old, store).
-template <typename T, int M, bool IN, bool Q16 = false>
-__device__ __forceinline__ void tile_run(const SplineArgs<T>& a, T* tile, int32_t* rowtab, int bx, int by, int bz) {
+// Per-tile context of tile (bx, by, bz).
+template <typename T, bool IN>
+__device__ __forceinline__ Ctx<T, IN> tile_ctx(const SplineArgs<T>& a, T* tile, int bx, int by, int bz, bool q16) {
     const Geom& g = a.g;
     Ctx<T, IN> t;
     t.s = tile;
@@ -904,9 +903,108 @@ __device__ __forceinline__ void tile_run(const SplineArgs<T>& a, T* tile, int32_
     t.g = g;
     t.qc_out = a.qc_out;
     t.qc_in = a.qc_in;
-    t.q16 = Q16;  // resolved per launch: one code width per kernel body
+    t.q16 = q16;
     t.natural = false;
     t.acc = 0.0;
+    return t;
+}
+
+// ------------------------------------------------------- decompression --
+// A tile's codes and corner anchors in registers, between their loads and
+// their LDS slots.
+template <typename T>
+struct TileCodes {
+    int32_t v3[FillShape<3>::ITERS], v2[FillShape<2>::ITERS], v1[FillShape<1>::ITERS], v0[kFill0Iters],
+        e0[kFill0EdgeIters];
+    T anc;
+    int aslot;
+};
+
+// Issues the loads of every code of the tile (levels up to maxlevel) from the
+// row table of the tile in LDS, and of its eight corner anchors
+// (x_reset_scratch_data 1051-1061).
+template <typename T, bool IN>
+__device__ __forceinline__ void codes_load(const Ctx<T, IN>& t, const int32_t* rowtab, int maxlevel,
+                                           const T* anchors_in, int bx, int by, int bz, TileCodes<T>& c) {
+    const Geom& g = t.g;
+    fill_load<T, 3>(t, rowtab, 3 <= maxlevel, c.v3);
+    fill_load<T, 2>(t, rowtab, 2 <= maxlevel, c.v2);
+    fill_load<T, 1>(t, rowtab, 1 <= maxlevel, c.v1);
+    fill0_load<T>(t, rowtab, 0 <= maxlevel, c.v0, c.e0);
+    c.anc = (T)0;
+    c.aslot = -1;
+    if (threadIdx.x < 8) {
+        const int cx = threadIdx.x & 1, cy = (threadIdx.x >> 1) & 1, cz = threadIdx.x >> 2;
+        const int ax = bx + cx, ay = by + cy, az = bz + cz;
+        c.aslot = sidx(16 * cx, 16 * cy, 16 * cz);
+        if (anchors_in && ax < g.gx && ay < g.gy && az < g.gz)
+            c.anc = anchors_in[ax + (int64_t)ay * g.gx + (int64_t)az * g.gx * g.gy];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void codes_store(T* tile, const TileCodes<T>& c) {
+    fill_store<T, 3>(tile, c.v3);
+    fill_store<T, 2>(tile, c.v2);
+    fill_store<T, 1>(tile, c.v1);
+    fill0_store<T>(tile, c.v0, c.e0);
+    if (c.aslot >= 0) tile[c.aslot] = c.anc;
+}
+
+// The previous reconstruction of the tile (delta form); lane: (x, y) =
+// (l mod 16, l / 16), iteration: z.
+constexpr int kOutIters = 4096 / kThreads;
+template <typename T, bool IN>
+__device__ __forceinline__ void prev_load(const Ctx<T, IN>& t, const T* old, T (&prev)[kOutIters]) {
+    const Geom& g = t.g;
+    const int lane = lane_id();
+    const int gx = t.ox + (lane & 15), gy = t.oy + (lane >> 4);
+    const bool in_xy = t.template inf<0>(gx) && t.template inf<1>(gy);
+    const int64_t nxy = (int64_t)g.nx * g.ny;
+    const T* src = old + (int64_t)gx + (int64_t)gy * g.nx + (int64_t)t.oz * nxy;
+#pragma unroll
+    for (int z = 0; z < kOutIters; ++z) {
+        prev[z] = (T)0;
+        if (in_xy && t.template inf<2>(t.oz + z)) prev[z] = src[z * nxy];
+    }
+}
+
+// The reconstructed tile (plus the previous one in the delta form) to `out`.
+template <typename T, bool IN>
+__device__ __forceinline__ void tile_out(const Ctx<T, IN>& t, bool delta, const T (&prev)[kOutIters], T* out) {
+    const Geom& g = t.g;
+    const int lane = lane_id();
+    const int x = lane & 15, y = lane >> 4;
+    const int gx = t.ox + x, gy = t.oy + y;
+    const bool in_xy = t.template inf<0>(gx) && t.template inf<1>(gy);
+    const int64_t nxy = (int64_t)g.nx * g.ny;
+    T* dst = out + (int64_t)gx + (int64_t)gy * g.nx + (int64_t)t.oz * nxy;
+#pragma unroll
+    for (int z = 0; z < kOutIters; ++z) {
+        if (in_xy && t.template inf<2>(t.oz + z)) {
+            const T v = t.s[sidx(x, y, z)];
+            // written once and read only by a later round: no cache allocation
+            __builtin_nontemporal_store(delta ? v + prev[z] : v, &dst[z * nxy]);
+        }
+    }
+}
+
+// The four levels of a tile, coarsest first.
+template <typename T, int M, bool IN>
+__device__ __forceinline__ void tile_levels(Ctx<T, IN>& t, const LevelEb<T>& le, const uint8_t (&fl)[8], int maxlevel) {
+    if (3 <= maxlevel) { t.eb_r = le.eb_r[3]; t.ebx2 = le.ebx2[3]; level<T, 8, M>(t, fl[3], fl[7]); }
+    if (2 <= maxlevel) { t.eb_r = le.eb_r[2]; t.ebx2 = le.ebx2[2]; level<T, 4, M>(t, fl[2], fl[6]); }
+    if (1 <= maxlevel) { t.eb_r = le.eb_r[1]; t.ebx2 = le.ebx2[1]; level<T, 2, M>(t, fl[1], fl[5]); }
+    if (0 <= maxlevel) { t.eb_r = le.eb_r[0]; t.ebx2 = le.ebx2[0]; level<T, 1, M>(t, fl[0], fl[4]); }
+}
+
+// One tile: compress (load, predict, quantise, store codes) or decompress
+// (fill the code slots, reconstruct, add old, store).
+template <typename T, int M, bool IN, bool Q16 = false>
+__device__ __forceinline__ void tile_run(const SplineArgs<T>& a, T* tile, int32_t* rowtab, int bx, int by, int bz) {
+    const Geom& g = a.g;
+    // q16 is resolved per launch: one code width per kernel body
+    Ctx<T, IN> t = tile_ctx<T, IN>(a, tile, bx, by, bz, Q16);
     uint8_t fl[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) fl[i] = a.flags8[i];
@@ -914,21 +1012,9 @@ __device__ __forceinline__ void tile_run(const SplineArgs<T>& a, T* tile, int32_
 
     // decompression: the previous reconstruction (delta form) and every
     // code of the tile are requested before any is needed
-    constexpr int OIT = 4096 / kThreads;
-    T prev[OIT];
+    T prev[kOutIters];
     if constexpr (M == DECOMPRESS) {
-        if (a.old) {  // lane: (x, y) = (l mod 16, l / 16); iteration: z
-            const int lane = lane_id();
-            const int gx = t.ox + (lane & 15), gy = t.oy + (lane >> 4);
-            const bool in_xy = t.template inf<0>(gx) && t.template inf<1>(gy);
-            const int64_t nxy = (int64_t)g.nx * g.ny;
-            const T* src = a.old + (int64_t)gx + (int64_t)gy * g.nx + (int64_t)t.oz * nxy;
-#pragma unroll
-            for (int z = 0; z < OIT; ++z) {
-                prev[z] = (T)0;
-                if (in_xy && t.template inf<2>(t.oz + z)) prev[z] = src[z * nxy];
-            }
-        }
+        if (a.old) prev_load(t, a.old, prev);
     }
 
     if constexpr (M == COMPRESS) {
@@ -938,55 +1024,15 @@ __device__ __forceinline__ void tile_run(const SplineArgs<T>& a, T* tile, int32_
     } else {
         fill_rows(t, rowtab);
         lds_barrier();
-        int32_t v3[FillShape<3>::ITERS], v2[FillShape<2>::ITERS], v1[FillShape<1>::ITERS], v0[kFill0Iters],
-            e0[kFill0EdgeIters];
-        fill_load<T, 3>(t, rowtab, 3 <= maxlevel, v3);
-        fill_load<T, 2>(t, rowtab, 2 <= maxlevel, v2);
-        fill_load<T, 1>(t, rowtab, 1 <= maxlevel, v1);
-        fill0_load<T>(t, rowtab, 0 <= maxlevel, v0, e0);
-        T anc = (T)0;
-        int aslot = -1;
-        if (threadIdx.x < 8) {  // x_reset_scratch_data 1051-1061: the eight corner anchors
-            const int cx = threadIdx.x & 1, cy = (threadIdx.x >> 1) & 1, cz = threadIdx.x >> 2;
-            const int ax = bx + cx, ay = by + cy, az = bz + cz;
-            aslot = sidx(16 * cx, 16 * cy, 16 * cz);
-            if (a.anchors_in && ax < g.gx && ay < g.gy && az < g.gz)
-                anc = a.anchors_in[ax + (int64_t)ay * g.gx + (int64_t)az * g.gx * g.gy];
-        }
-        fill_store<T, 3>(tile, v3);
-        fill_store<T, 2>(tile, v2);
-        fill_store<T, 1>(tile, v1);
-        fill0_store<T>(tile, v0, e0);
-        if (aslot >= 0) tile[aslot] = anc;
+        TileCodes<T> c;
+        codes_load(t, rowtab, maxlevel, a.anchors_in, bx, by, bz, c);
+        codes_store(tile, c);
         __syncthreads();
     }
 
-    const LevelEb<T>& le = a.le;
-    if (3 <= maxlevel) { t.eb_r = le.eb_r[3]; t.ebx2 = le.ebx2[3]; level<T, 8, M>(t, fl[3], fl[7]); }
-    if (2 <= maxlevel) { t.eb_r = le.eb_r[2]; t.ebx2 = le.ebx2[2]; level<T, 4, M>(t, fl[2], fl[6]); }
-    if (1 <= maxlevel) { t.eb_r = le.eb_r[1]; t.ebx2 = le.ebx2[1]; level<T, 2, M>(t, fl[1], fl[5]); }
-    if (0 <= maxlevel) {
-        t.eb_r = le.eb_r[0];
-        t.ebx2 = le.ebx2[0];
-        level<T, 1, M>(t, fl[0], fl[4]);
-    }
+    tile_levels<T, M>(t, a.le, fl, maxlevel);
 
-    if constexpr (M == DECOMPRESS) {  // lane: (x, y); iteration: z
-        const int lane = lane_id();
-        const int x = lane & 15, y = lane >> 4;
-        const int gx = t.ox + x, gy = t.oy + y;
-        const bool in_xy = t.template inf<0>(gx) && t.template inf<1>(gy);
-        const int64_t nxy = (int64_t)g.nx * g.ny;
-        T* dst = a.out + (int64_t)gx + (int64_t)gy * g.nx + (int64_t)t.oz * nxy;
-#pragma unroll
-        for (int z = 0; z < OIT; ++z) {
-            if (in_xy && t.template inf<2>(t.oz + z)) {
-                const T v = tile[sidx(x, y, z)];
-                // written once and read only by a later round: no cache allocation
-                __builtin_nontemporal_store(a.old ? v + prev[z] : v, &dst[z * nxy]);
-            }
-        }
-    }
+    if constexpr (M == DECOMPRESS) tile_out(t, a.old != nullptr, prev, a.out);
 }
 
 // ------------------------------------------------------------ 2-D fields --
