@@ -380,14 +380,31 @@ __device__ __forceinline__ double code_slot(int32_t q, double) {
 __device__ __forceinline__ int32_t slot_code(float v) { return __float_as_int(v); }
 __device__ __forceinline__ int32_t slot_code(double v) { return (int32_t)(uint32_t)__double_as_longlong(v); }
 
+// Where a compression stage puts its codes: straight to the level-major
+// array (levels 1-3), into registers with their LDS slots (the first two
+// stages of level 0, whose values later stages still read), or into the
+// point's own LDS slot (the last stage of level 0: no later stage reads its
+// values).  Level 0's codes then leave the tile as whole rows.
+enum Sink { TO_GLOBAL, TO_REGS, TO_SLOT };
+constexpr int kRegCodes = 8;  // points per lane of a level-0 stage (line or point form)
+struct RegCodes {
+    int32_t code[kRegCodes];
+    int32_t slot[kRegCodes];  // -1: no own point
+};
+
 // ----------------------------------------------------------- point form --
 // Lane l visits points p = l + 256 * it of the stage (tuner, and the
 // multi-dimensional stages).
-template <typename T, int U, int K, int M, bool IN>
-__device__ __forceinline__ void stage(Ctx<T, IN>& t, bool inclusive) {
+template <typename T, int U, int K, int M, bool IN, int SINK = TO_GLOBAL>
+__device__ __forceinline__ void stage(Ctx<T, IN>& t, bool inclusive, RegCodes* rc = nullptr) {
     constexpr int TOTAL = stage_total<U, K>();
     constexpr int LEVEL = level_of<U>();
+    static_assert(SINK != TO_REGS || stage_iters<U, K>() <= kRegCodes, "register codes: at most 8 points per lane");
     const int lane = lane_id();
+    if constexpr (SINK == TO_REGS) {
+#pragma unroll
+        for (int k = 0; k < kRegCodes; ++k) rc->slot[k] = -1;
+    }
 #pragma unroll
     for (int it = 0; it < stage_iters<U, K>(); ++it) {
         const int p = lane + it * kThreads;
@@ -402,7 +419,15 @@ __device__ __forceinline__ void stage(Ctx<T, IN>& t, bool inclusive) {
             t.acc += (double)(e < (T)0 ? -e : e);
         } else if constexpr (M == COMPRESS) {
             const int32_t qv = quantize(t, v, pred);
-            if (x < 16 && y < 16 && z < 16) t.qc_out[lm_index<LEVEL>(t.g, t.ox + x, t.oy + y, t.oz + z)] = qv;
+            const bool own = x < 16 && y < 16 && z < 16;
+            if constexpr (SINK == TO_GLOBAL) {
+                if (own) t.qc_out[lm_index<LEVEL>(t.g, t.ox + x, t.oy + y, t.oz + z)] = qv;
+            } else if constexpr (SINK == TO_REGS) {
+                rc->code[it] = qv;
+                rc->slot[it] = own ? sidx(x, y, z) : -1;
+            } else {
+                v = code_slot(qv, T());
+            }
         } else {
             v = pred + (T)slot_code(v) * t.ebx2;
         }
@@ -508,10 +533,15 @@ __device__ __forceinline__ LineLane line_lane(const Ctx<T, IN>& t, int it) {
     return r;
 }
 
-template <typename T, int U, int K, bool EXCL, int M, bool IN>
-__device__ __forceinline__ void line_stage(Ctx<T, IN>& t) {
+template <typename T, int U, int K, bool EXCL, int M, bool IN, int SINK = TO_GLOBAL>
+__device__ __forceinline__ void line_stage(Ctx<T, IN>& t, RegCodes* rc = nullptr) {
     using S = LineShape<U, K, EXCL>;
     constexpr int N = S::N;
+    static_assert(SINK != TO_REGS || S::ITERS * N <= kRegCodes, "register codes: at most 8 points per lane");
+    if constexpr (SINK == TO_REGS) {
+#pragma unroll
+        for (int k = 0; k < kRegCodes; ++k) rc->slot[k] = -1;
+    }
     const int o = S::A == 0 ? t.ox : (S::A == 1 ? t.oy : t.oz);
     const int nax = S::A == 0 ? t.g.nx : (S::A == 1 ? t.g.ny : t.g.nz);
     const bool c1 = t.template c1<S::A>();
@@ -527,63 +557,18 @@ __device__ __forceinline__ void line_stage(Ctx<T, IN>& t) {
             if (!t.template inf<S::A>(o + (2 * j + 1) * U)) break;  // beyond the field; so are later points
             const T pred = apply_stencil(stencil_of<U>(j, N, c1, o, nax), k[j > 0 ? j - 1 : 0], k[j], k[j + 1],
                                          k[j + 2 <= N ? j + 2 : 0]);
-            T& v = t.s[ln.base + (2 * j + 1) * U * S::STA];
+            const int slot = ln.base + (2 * j + 1) * U * S::STA;
+            T& v = t.s[slot];
             if constexpr (M == COMPRESS) {
                 const int32_t qv = quantize(t, v, pred);
-                if (ln.own) t.qc_out[ln.idx0 + j * ln.st] = qv;
-            } else {
-                v = pred + (T)slot_code(v) * t.ebx2;
-            }
-        }
-    }
-    lds_barrier();
-}
-
-// ---------------------------------------------------------- column form --
-// The last stage of level 0 in z-y-x order predicts along x at x = 1, 3,
-// .., 15 for y, z in 0..15.  Lane l takes x = 2(l mod 8) + 1,
-// y = (l / 8) mod 16 and z = l / 128 + 2 it: eight lanes cover the odd
-// points of a row, whose codes are adjacent, and each lane keeps one
-// stencil and one code stride.
-constexpr int kHollow0Iters = 8;
-
-struct ColumnLane {
-    int x, y, z0;
-    bool on;
-    uint32_t idx0, dz;
-};
-
-template <typename T, bool IN>
-__device__ __forceinline__ ColumnLane column_lane(const Ctx<T, IN>& t) {
-    ColumnLane c;
-    const int l = lane_id();
-    c.x = 2 * (l & 7) + 1;
-    c.y = (l >> 3) & 15;
-    c.z0 = l >> 7;
-    c.on = t.template inf<0>(t.ox + c.x) && t.template inf<1>(t.oy + c.y);
-    c.idx0 = lm_index<0>(t.g, t.ox + c.x, t.oy + c.y, t.oz + c.z0);
-    c.dz = (uint32_t)(2 * t.g.lxy[0] - t.g.lxy[1]);  // z -> z + 2 keeps the parity of z
-    return c;
-}
-
-template <typename T, int M, bool IN>
-__device__ __forceinline__ void hollow0_stage(Ctx<T, IN>& t) {
-    const ColumnLane c = column_lane(t);
-    if (c.on) {
-        const int j = c.x >> 1;
-        const int k = stencil_of<1>(j, 8, t.template c1<0>(), t.ox, t.g.nx);
-        const int i0 = j > 0 ? c.x - 3 : 0, i3 = c.x + 3 <= 16 ? c.x + 3 : 0;
-#pragma unroll
-        for (int it = 0; it < kHollow0Iters; ++it) {
-            const int z = c.z0 + 2 * it;
-            if (!t.template inf<2>(t.oz + z)) break;
-            const int base = sidx(0, c.y, z);
-            const T pred = apply_stencil(k, t.s[base + i0], t.s[base + c.x - 1], t.s[base + c.x + 1],
-                                         t.s[base + i3]);
-            T& v = t.s[base + c.x];
-            if constexpr (M == COMPRESS) {
-                const int32_t qv = quantize(t, v, pred);
-                t.qc_out[c.idx0 + it * c.dz] = qv;
+                if constexpr (SINK == TO_GLOBAL) {
+                    if (ln.own) t.qc_out[ln.idx0 + j * ln.st] = qv;
+                } else if constexpr (SINK == TO_REGS) {
+                    rc->code[it * N + j] = qv;
+                    rc->slot[it * N + j] = ln.own ? slot : -1;
+                } else {
+                    v = code_slot(qv, T());
+                }
             } else {
                 v = pred + (T)slot_code(v) * t.ebx2;
             }
@@ -595,9 +580,30 @@ __device__ __forceinline__ void hollow0_stage(Ctx<T, IN>& t) {
 // ---------------------------------------------------------------- levels --
 // The three stages of one level; outside tuning the last stage of level 0
 // uses the exclusive border (InterpolationPredictor.cu:1940-2025).
+template <typename T, bool IN>
+__device__ __forceinline__ void level0_codes_out(Ctx<T, IN>& t, const RegCodes& a, const RegCodes& b);
+
 template <typename T, int U, int M, bool IN>
 __device__ __forceinline__ void level(Ctx<T, IN>& t, bool md, bool rev) {
     constexpr bool EX = M != TUNE && U == 1;
+    if constexpr (M == COMPRESS && U == 1) {
+        RegCodes a, b;
+        if (md) {
+            stage<T, 1, LINE, M, IN, TO_REGS>(t, true, &a);
+            stage<T, 1, FACE, M, IN, TO_REGS>(t, true, &b);
+            stage<T, 1, CUBE, M, IN, TO_SLOT>(t, false);
+        } else if (rev) {
+            line_stage<T, 1, HOLLOW_R, false, M, IN, TO_REGS>(t, &a);
+            line_stage<T, 1, YELLOW_R, false, M, IN, TO_REGS>(t, &b);
+            line_stage<T, 1, BLUE_R, true, M, IN, TO_SLOT>(t);
+        } else {
+            line_stage<T, 1, BLUE, false, M, IN, TO_REGS>(t, &a);
+            line_stage<T, 1, YELLOW, false, M, IN, TO_REGS>(t, &b);
+            line_stage<T, 1, HOLLOW, true, M, IN, TO_SLOT>(t);
+        }
+        level0_codes_out(t, a, b);
+        return;
+    }
     if (md) {
         stage<T, U, LINE, M>(t, true);
         stage<T, U, FACE, M>(t, true);
@@ -619,11 +625,7 @@ __device__ __forceinline__ void level(Ctx<T, IN>& t, bool md, bool rev) {
     } else {
         line_stage<T, U, BLUE, false, M>(t);
         line_stage<T, U, YELLOW, false, M>(t);
-        // compression stores codes, which coalesce in the column form;
-        // decompression reads them from LDS, where x-lines with one stencil
-        // per point position are cheaper
-        if constexpr (EX && M == COMPRESS) hollow0_stage<T, M>(t);
-        else line_stage<T, U, HOLLOW, EX, M>(t);
+        line_stage<T, U, HOLLOW, EX, M>(t);
     }
 }
 
@@ -652,6 +654,37 @@ __device__ __forceinline__ int32_t row_base(const Geom& g, int gy, int gz) {
                  ((c + 1) >> 1) * (uint32_t)g.lxy[L + 1];
     if ((c & 1) == 0) r -= ((b + 1) >> 1) * (uint32_t)g.lx[L + 1];
     return (int32_t)r;
+}
+
+// Level 0's codes of the 16^3 tile, all in their LDS slots once the
+// register-held ones join them, written out row by row: lane (x, y) =
+// (l mod 16, l / 16) takes z = 0..15; a row (y, z) with y or z odd holds
+// every x (16 adjacent codes), the others the odd x (8 adjacent codes).
+// Row bases advance by 2 lxy0 - lxy1 from z to z + 2.
+template <typename T, bool IN>
+__device__ __forceinline__ void level0_codes_out(Ctx<T, IN>& t, const RegCodes& a, const RegCodes& b) {
+#pragma unroll
+    for (int k = 0; k < kRegCodes; ++k) {
+        if (a.slot[k] >= 0) t.s[a.slot[k]] = code_slot(a.code[k], T());
+        if (b.slot[k] >= 0) t.s[b.slot[k]] = code_slot(b.code[k], T());
+    }
+    lds_barrier();
+    const int lane = lane_id();
+    const int x = lane & 15, y = lane >> 4;
+    const int gx = t.ox + x, gy = t.oy + y;
+    if (!(t.template inf<0>(gx) && t.template inf<1>(gy))) return;
+    const uint32_t step2 = (uint32_t)(2 * t.g.lxy[0] - t.g.lxy[1]);
+    const uint32_t base_e = (uint32_t)row_base<0>(t.g, gy, t.oz), base_o = (uint32_t)row_base<0>(t.g, gy, t.oz + 1);
+    const uint32_t off_full = (uint32_t)gx, off_half = (uint32_t)((t.ox >> 1) + (x >> 1));
+#pragma unroll
+    for (int z = 0; z < 16; ++z) {
+        if (!t.template inf<2>(t.oz + z)) break;
+        const bool full = ((y | z) & 1) != 0;
+        if (full || (x & 1)) {
+            const uint32_t base = ((z & 1) ? base_o : base_e) + (uint32_t)(z >> 1) * step2;
+            t.qc_out[base + (full ? off_full : off_half)] = slot_code(t.s[sidx(x, y, z)]);
+        }
+    }
 }
 
 template <typename T, int L, bool IN>
