@@ -302,34 +302,81 @@ __global__ __launch_bounds__(64 * kWaves) void batch_encode_kernel(const BatchFi
 // Payload offsets by one workgroup scan of the size table, then the stream
 // header, the stream length and the cumulative size table by (level, plane)
 // (zbpc.cu:176-181, 231-262).
-__device__ __forceinline__ void stream_scan(const uint16_t* __restrict__ sizes, const ChunkGeom& g,
-                                            int64_t* __restrict__ offsets, uint8_t* __restrict__ dst_stream,
-                                            uint64_t* __restrict__ cum, uint64_t* __restrict__ outsize) {
-    __shared__ int64_t s_w[16];
+// One workgroup of 1024 lanes: lane t takes the run of `per` chunks from
+// t * per.  Up to kScanLds sizes are first staged through LDS with
+// coalesced loads, so the runs read LDS rather than scattered 2-byte words.
+constexpr int kScanLds = 16384;
+
+struct ScanLds {
+    int64_t w[16];
+    int64_t base[1024];  // exclusive offset of each lane's run
+    uint16_t sz[kScanLds];
+};
+
+// exclusive offsets of the payloads; returns the lane's run start and fills
+// S.base; the sizes come from LDS when staged
+__device__ __forceinline__ const uint16_t* scan_runs(const uint16_t* __restrict__ sizes, int64_t n, ScanLds& S,
+                                                     int64_t* __restrict__ offsets, int64_t* total) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t n = g.nchunks, per = (n + 1023) / 1024;
+    const bool staged = n <= kScanLds;
+    if (staged) {
+        for (int64_t i = tid; i < n; i += 1024) S.sz[i] = sizes[i];
+        __syncthreads();
+    }
+    const uint16_t* sz = staged ? S.sz : sizes;
+    const int64_t per = (n + 1023) / 1024;
     const int64_t lo = tid * per, hi = lo + per < n ? lo + per : n;
     int64_t sum = 0;
-    for (int64_t i = lo; i < hi; ++i) sum += sizes[i];
+    for (int64_t i = lo; i < hi; ++i) sum += sz[i];
     int64_t x = sum;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
         const int64_t y = __shfl_up(x, d, 64);
         if (lane >= d) x += y;
     }
-    if (lane == 63) s_w[wave] = x;
+    if (lane == 63) S.w[wave] = x;
     __syncthreads();
-    int64_t run = x - sum;
-    for (int w = 0; w < wave; ++w) run += s_w[w];
+    int64_t run = x - sum, tot = 0;
+    for (int w = 0; w < 16; ++w) {
+        run += w < wave ? S.w[w] : 0;
+        tot += S.w[w];
+    }
+    S.base[tid] = run;
     for (int64_t i = lo; i < hi; ++i) {
         offsets[i] = run;
-        run += sizes[i];
-        const ChunkPos cp = chunk_pos(g, i);
-        if (cp.last_of_plane) cum[(3 - cp.lv) * 32 + cp.plane] = (uint64_t)run;
-        if (i == n - 1) {
-            *reinterpret_cast<uint64_t*>(dst_stream) = (uint64_t)g.host_size;
-            *outsize = (uint64_t)(8 + 2 * n + run);
-        }
+        run += sz[i];
+    }
+    *total = tot;
+    __syncthreads();
+    return sz;
+}
+
+// inclusive offset of chunk id (after it), from the run bases
+__device__ __forceinline__ int64_t scan_inclusive(const uint16_t* sz, int64_t n, const ScanLds& S, int64_t id) {
+    const int64_t per = (n + 1023) / 1024;
+    const int64_t t = id / per;
+    int64_t r = S.base[t];
+    for (int64_t i = t * per; i <= id; ++i) r += sz[i];
+    return r;
+}
+
+__device__ __forceinline__ void stream_scan(const uint16_t* __restrict__ sizes, const ChunkGeom& g,
+                                            int64_t* __restrict__ offsets, uint8_t* __restrict__ dst_stream,
+                                            uint64_t* __restrict__ cum, uint64_t* __restrict__ outsize) {
+    __shared__ ScanLds S;
+    const int64_t n = g.nchunks;
+    int64_t total;
+    const uint16_t* sz = scan_runs(sizes, n, S, offsets, &total);
+    // cum[lr][plane]: the payload end of the plane's last chunk (every level
+    // has 32 planes of cb >= 1 chunks)
+    if (threadIdx.x < 128) {
+        const int lr = threadIdx.x >> 5, plane = threadIdx.x & 31;
+        const int64_t id = g.chunk_start[lr] + (int64_t)(plane + 1) * g.cb[3 - lr] - 1;
+        cum[lr * 32 + plane] = (uint64_t)scan_inclusive(sz, n, S, id);
+    }
+    if (threadIdx.x == 0) {
+        *reinterpret_cast<uint64_t*>(dst_stream) = (uint64_t)g.host_size;
+        *outsize = (uint64_t)(8 + 2 * n + total);
     }
 }
 
@@ -452,27 +499,9 @@ __global__ void batch_merge_kernel(const BatchField* __restrict__ fields, const 
 
 // exclusive offsets of the payloads from the size table; one workgroup
 __device__ __forceinline__ void size_scan(const uint16_t* __restrict__ sizes, int64_t n, int64_t* __restrict__ offsets) {
-    __shared__ int64_t s_w[16];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t per = (n + 1023) / 1024;
-    const int64_t lo = tid * per, hi = lo + per < n ? lo + per : n;
-    int64_t sum = 0;
-    for (int64_t i = lo; i < hi; ++i) sum += sizes[i];
-    int64_t x = sum;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int64_t y = __shfl_up(x, d, 64);
-        if (lane >= d) x += y;
-    }
-    if (lane == 63) s_w[wave] = x;
-    __syncthreads();
-    int64_t before = 0;
-    for (int w = 0; w < wave; ++w) before += s_w[w];
-    int64_t run = before + x - sum;
-    for (int64_t i = lo; i < hi; ++i) {
-        offsets[i] = run;
-        run += sizes[i];
-    }
+    __shared__ ScanLds S;
+    int64_t total;
+    scan_runs(sizes, n, S, offsets, &total);
 }
 
 // also clears the decoder's error words (the 64-byte slot before the offsets)

@@ -903,6 +903,18 @@ __device__ __forceinline__ void load_tile(const Ctx<T, IN>& t, const T* data) {
     }
 }
 
+// Zero the code-array entries the predictor does not write: the anchor
+// slots, the pads that 8-align each level region and the level-0 tail.
+struct Gaps {
+    int64_t lo[5], hi[5];
+};
+
+__device__ __forceinline__ void zero_gaps(int32_t* p, const int64_t* lo, const int64_t* hi, int64_t blk,
+                                          int64_t nblk) {
+    for (int r = 0; r < 5; ++r)
+        for (int64_t i = lo[r] + blk * blockDim.x + threadIdx.x; i < hi[r]; i += nblk * blockDim.x) p[i] = 0;
+}
+
 template <typename T>
 struct SplineArgs {
     const T* data;
@@ -919,7 +931,17 @@ struct SplineArgs {
     const T* anchors_in;
     const T* old;
     T* out;
+    Gaps gaps;    // compression: the gaps of qc_out, zeroed by the same launch
+    int gaps_on;  // (its workgroups share them out; disjoint from the codes)
 };
+
+// Compression launches clear the code array's gaps alongside the tiles.
+template <typename T, int M>
+__device__ __forceinline__ void clear_gaps(const SplineArgs<T>& a) {
+    if constexpr (M == COMPRESS) {
+        if (a.gaps_on) zero_gaps(a.qc_out, a.gaps.lo, a.gaps.hi, blockIdx.x, gridDim.x);
+    }
+}
 
 // Per-tile context of tile (bx, by, bz).
 template <typename T, bool IN>
@@ -1250,6 +1272,7 @@ __device__ __forceinline__ SplineArgs<T> resolve(const SplineArgs<T>& a) {
 template <typename T, int M>
 __global__ __launch_bounds__(kThreads) void plane_kernel(SplineArgs<T> a) {
     __shared__ T lds[kPlaneWaves * kPlaneVals];
+    clear_gaps<T, M>(a);
     plane_tiles<T, M>(resolve(a), lds, blockIdx.x);
 }
 
@@ -1440,6 +1463,7 @@ __device__ __forceinline__ void line_tiles(const SplineArgs<T>& a, T* lds, int64
 template <typename T, int M>
 __global__ __launch_bounds__(kThreads) void line_kernel(SplineArgs<T> a) {
     __shared__ T lds[kLineSlots];
+    clear_gaps<T, M>(a);
     line_tiles<T, M>(resolve(a), lds, blockIdx.x);
 }
 
@@ -1454,6 +1478,7 @@ template <typename T, int M>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void spline_kernel(SplineArgs<T> a) {
     __shared__ T tile[kTileVals];
     __shared__ int32_t rowtab[M == DECOMPRESS ? kRowTab : 1];
+    clear_gaps<T, M>(a);
     // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs,
     // so workgroup b runs on XCD b mod 8; each XCD takes a contiguous range
     // of tiles (x fastest) and the cache lines that neighbouring tiles share
@@ -1476,19 +1501,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     else tile_run<T, M, false>(r, tile, rowtab, bx, by, bz);
 }
 
-// Zero the code-array entries the predictor does not write: the anchor
-// slots, the pads that 8-align each level region and the level-0 tail.
-struct Gaps {
-    int64_t lo[5], hi[5];
-};
-
-__device__ __forceinline__ void zero_gaps(int32_t* p, const int64_t* lo, const int64_t* hi, int64_t blk,
-                                          int64_t nblk) {
-    for (int r = 0; r < 5; ++r)
-        for (int64_t i = lo[r] + blk * blockDim.x + threadIdx.x; i < hi[r]; i += nblk * blockDim.x) p[i] = 0;
-}
-
-__global__ void zero_gaps_kernel(int32_t* p, Gaps g) { zero_gaps(p, g.lo, g.hi, blockIdx.x, gridDim.x); }
 
 // ---------------------------------------------------------------- tuner --
 // Grid (sample block, candidate).  Lane p mod 256 of each stage accumulates
@@ -1578,10 +1590,19 @@ __device__ void pick_flags(const double* __restrict__ part, int nblocks, double*
     for (int i = 0; i < 8; ++i) flags[i] = f[i];
 }
 
+// The partial sums go through LDS first (one cooperative load instead of a
+// chain of dependent global loads in the one thread that adds them in order).
+constexpr int kPickLds = 2048;
 __global__ void tune_pick_kernel(const double* __restrict__ part, int nblocks, double* __restrict__ scores,
                                  uint8_t* __restrict__ flags) {
+    __shared__ double s[kPickLds];
+    const int n = nblocks * 15;
+    const bool staged = n <= kPickLds;
+    if (staged)
+        for (int i = threadIdx.x; i < n; i += blockDim.x) s[i] = part[i];
+    __syncthreads();
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    pick_flags(part, nblocks, scores, flags);
+    pick_flags(staged ? s : part, nblocks, scores, flags);
 }
 
 __global__ void zero_flags_kernel(double* scores, uint8_t* flags) {
@@ -1820,13 +1841,17 @@ template <typename T, int M>
 int launch_spline(const pa_layout* L, const T* data, double eb, double rel_eb, const uint8_t* flags8,
                   const int32_t* d_maxlevel, int32_t* qc_out, const int32_t* qc_in, T* anchors,
                   const T* anchors_in, const T* old, T* out, hipStream_t st, const double* le_dev = nullptr,
-                  const int32_t* narrow = nullptr) {
+                  const int32_t* narrow = nullptr, const Gaps* gaps = nullptr) {
     Geom g = make_geom(L);
     const int64_t ntiles = (int64_t)g.gx * g.gy * g.gz;
     if (ntiles > 0x7fffffff) return PA_ERR_ARG;
     LevelEb<T> le = le_dev ? LevelEb<T>{} : level_ebs<T>(eb, rel_eb);
-    const SplineArgs<T> args{data, g, le, le_dev, narrow, false, flags8, d_maxlevel, qc_out, qc_in, anchors, anchors_in,
-                             old, out};
+    SplineArgs<T> args{data, g, le, le_dev, narrow, false, flags8, d_maxlevel, qc_out, qc_in, anchors, anchors_in,
+                       old, out};
+    if (gaps) {
+        args.gaps = *gaps;
+        args.gaps_on = 1;
+    }
     if (line_field(g))
         hipLaunchKernelGGL((line_kernel<T, M>), dim3((unsigned)((ntiles + kLineTiles - 1) / kLineTiles)),
                            dim3(kThreads), 0, st, args);
@@ -1901,14 +1926,13 @@ extern "C" int pa_spline_construct(const void* d_data, int dtype, const pa_layou
         return PA_ERR_ARG;
     hipStream_t st = as_stream(stream);
     const Gaps gaps = make_gaps(L);
-    const int blocks = gap_blocks(gaps);
-    hipLaunchKernelGGL(zero_gaps_kernel, dim3(blocks), dim3(256), 0, st, d_qc, gaps);
-    PA_LAUNCH_CHECK();
     if (dtype == PA_F32)
         return launch_spline<float, COMPRESS>(L, (const float*)d_data, eb, rel_eb, d_flags8, nullptr, d_qc,
-                                              nullptr, (float*)d_anchors, nullptr, nullptr, nullptr, st);
+                                              nullptr, (float*)d_anchors, nullptr, nullptr, nullptr, st, nullptr,
+                                              nullptr, &gaps);
     return launch_spline<double, COMPRESS>(L, (const double*)d_data, eb, rel_eb, d_flags8, nullptr, d_qc,
-                                           nullptr, (double*)d_anchors, nullptr, nullptr, nullptr, st);
+                                           nullptr, (double*)d_anchors, nullptr, nullptr, nullptr, st, nullptr,
+                                           nullptr, &gaps);
 }
 
 extern "C" int pa_level_bounds(const double* d_minmax, double eb, int dtype, double* d_bounds, void* stream) {
@@ -1925,13 +1949,13 @@ extern "C" int pa_spline_construct_dev(const void* d_data, int dtype, const pa_l
     if (!L || !d_data || !d_bounds || !d_flags8 || !d_qc || !d_anchors || !valid_dtype(dtype)) return PA_ERR_ARG;
     hipStream_t st = as_stream(stream);
     const Gaps gaps = make_gaps(L);
-    hipLaunchKernelGGL(zero_gaps_kernel, dim3(gap_blocks(gaps)), dim3(256), 0, st, d_qc, gaps);
-    PA_LAUNCH_CHECK();
     if (dtype == PA_F32)
         return launch_spline<float, COMPRESS>(L, (const float*)d_data, 0.0, 0.0, d_flags8, nullptr, d_qc, nullptr,
-                                              (float*)d_anchors, nullptr, nullptr, nullptr, st, d_bounds);
+                                              (float*)d_anchors, nullptr, nullptr, nullptr, st, d_bounds, nullptr,
+                                              &gaps);
     return launch_spline<double, COMPRESS>(L, (const double*)d_data, 0.0, 0.0, d_flags8, nullptr, d_qc, nullptr,
-                                           (double*)d_anchors, nullptr, nullptr, nullptr, st, d_bounds);
+                                           (double*)d_anchors, nullptr, nullptr, nullptr, st, d_bounds, nullptr,
+                                           &gaps);
 }
 
 extern "C" int pa_spline_reconstruct(const int32_t* d_qc, const void* d_anchors, int dtype, const pa_layout* L,
