@@ -183,8 +183,10 @@ __device__ __forceinline__ void unshuffle_block(const uint8_t* __restrict__ bp, 
     if (has) {
         const uint8_t* row = bp + v.base[lv] + g;
 #pragma unroll
-        for (int k = 0; k < 32; ++k)
-            a[31 - k] = (planes >> k) & 1u ? *reinterpret_cast<const uint32_t*>(row + k * v.stride[lv]) : 0u;
+        for (int k = 0; k < 32; ++k)  // planes are read once per round: no cache allocation
+            a[31 - k] = (planes >> k) & 1u
+                            ? __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(row + k * v.stride[lv]))
+                            : 0u;
         transpose32(a);
     }
     const int64_t c0 = 8 * g0;

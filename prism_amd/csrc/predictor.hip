@@ -1020,7 +1020,8 @@ __device__ __forceinline__ void prev_load(const Ctx<T, IN>& t, const T* old, T (
 #pragma unroll
     for (int z = 0; z < kOutIters; ++z) {
         prev[z] = (T)0;
-        if (in_xy && t.template inf<2>(t.oz + z)) prev[z] = src[z * nxy];
+        // read once, here only: no cache allocation (reconstruction 158 -> 154 us)
+        if (in_xy && t.template inf<2>(t.oz + z)) prev[z] = __builtin_nontemporal_load(&src[z * nxy]);
     }
 }
 
