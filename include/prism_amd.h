@@ -73,6 +73,15 @@ size_t pa_autotune_workspace(const pa_layout* L);
 int pa_autotune(const void* d_data, int dtype, const pa_layout* L, double* d_scores15,
                 uint8_t* d_flags8, void* d_ws, size_t ws_bytes, void* stream);
 
+/* pa_extrema followed by pa_autotune (StatBuffer::findMinMax,
+ * compressor.hpp:23-31, and the auto-tuning part of spline_construct,
+ * InterpolationPredictor.cu:2273-2355: the two reads of the field before the
+ * prediction), run side by side in one launch: the same d_minmax, scores and
+ * flags as the two calls.  d_data must be 16-byte aligned; the workspace is
+ * pa_autotune's. */
+int pa_extrema_autotune(const void* d_data, int dtype, const pa_layout* L, double* d_minmax,
+                        double* d_scores15, uint8_t* d_flags8, void* d_ws, size_t ws_bytes, void* stream);
+
 /* Prediction part of spline_construct (kernel interpolation,
  * InterpolationPredictor.cu:2137-2173): quantisation codes in level-major
  * order (d_qc: L->code_len int32, fully written) and the anchors (n4 T). */

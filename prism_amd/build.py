@@ -24,7 +24,7 @@ ARCH = os.environ.get("PRISM_AMD_ARCH", "gfx950")
 
 SOURCES = ["layout.cpp", "extrema.hip", "predictor.hip", "bitplane.hip", "codec.hip", "strategy.hip", "quality.hip",
            "batch.hip"]
-HEADERS = ["common.hpp", "batch.hpp"]
+HEADERS = ["common.hpp", "batch.hpp", "extrema.hpp"]
 
 # C++ API (prism::Compressor & co.) and the `prism` command line, host code
 # over the C ABI

@@ -14,6 +14,7 @@
 // 1823-2026, tile staging 997-1170, compaction 2028-2088, auto-tuner 110-967.
 #include "batch.hpp"
 #include "common.hpp"
+#include "extrema.hpp"
 
 namespace pa {
 namespace {
@@ -1606,6 +1607,48 @@ __global__ void tune_pick_kernel(const double* __restrict__ part, int nblocks, d
     pick_flags(staged ? s : part, nblocks, scores, flags);
 }
 
+// ------------------------------------------------ extrema beside the tuner --
+// Both only read the field, so one launch can run them side by side:
+// workgroups [0, 15 nblocks) are the tuner's (sample block, candidate) pairs
+// (the blocks of tune_kernel), the rest stream the extrema, `span` 16-byte
+// vectors each (extrema_span).  A large field has few tuner blocks (135 for
+// 384 x 384 x 256), too few to fill the GPU on their own.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void extrema_tune_kernel(const T* __restrict__ data, int64_t n,
+                                                                unsigned long long* __restrict__ keys, int64_t span,
+                                                                Geom g, int sx0, int sy0, int sz0, int nsx, int nsy,
+                                                                int nblocks, double* __restrict__ part) {
+    __shared__ T tile[kTileVals];
+    __shared__ double red[kThreads];
+    const int ntune = 15 * nblocks;
+    if ((int)blockIdx.x < ntune)
+        tune_block<T>(data, g, sx0, sy0, sz0, nsx, nsy, (int)blockIdx.x % nblocks, (int)blockIdx.x / nblocks, part,
+                      tile, red);
+    else
+        extrema_span<T>(data, n, (int64_t)blockIdx.x - ntune, span, keys);
+}
+
+__global__ void extrema_keys_init_kernel(unsigned long long* keys) {
+    keys[0] = ~0ull;  // running min
+    keys[1] = 0ull;   // running max
+}
+
+// min / max from the keys, and the tuner's pick (as tune_pick_kernel)
+__global__ void extrema_tune_finish_kernel(const unsigned long long* __restrict__ keys, double* __restrict__ minmax,
+                                           const double* __restrict__ part, int nblocks, double* __restrict__ scores,
+                                           uint8_t* __restrict__ flags) {
+    __shared__ double s[kPickLds];
+    const int n = nblocks * 15;
+    const bool staged = n <= kPickLds;
+    if (staged)
+        for (int i = threadIdx.x; i < n; i += blockDim.x) s[i] = part[i];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    minmax[0] = key_value(keys[0]);
+    minmax[1] = key_value(keys[1]);
+    pick_flags(staged ? s : part, nblocks, scores, flags);
+}
+
 __global__ void zero_flags_kernel(double* scores, uint8_t* flags) {
     const int i = threadIdx.x;
     if (i < 8) flags[i] = 0;
@@ -1916,6 +1959,60 @@ extern "C" int pa_autotune(const void* d_data, int dtype, const pa_layout* L, do
                            s[1], s[2], n[0], n[1], part);
     PA_LAUNCH_CHECK();
     hipLaunchKernelGGL(tune_pick_kernel, dim3(1), dim3(64), 0, st, part, nblocks, d_scores15, d_flags8);
+    PA_LAUNCH_CHECK();
+    return PA_OK;
+}
+
+// Compute units of the current device (cached per device).
+static int device_cus() {
+    static int cache[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cache[dev] == 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cache[dev] = n;
+    }
+    return cache[dev];
+}
+
+extern "C" int pa_extrema_autotune(const void* d_data, int dtype, const pa_layout* L, double* d_minmax,
+                                   double* d_scores15, uint8_t* d_flags8, void* d_ws, size_t ws_bytes, void* stream) {
+    if (!L || !d_data || !d_minmax || !d_flags8 || !valid_dtype(dtype)) return PA_ERR_ARG;
+    if (((uintptr_t)d_data & 15) != 0) return PA_ERR_ARG;
+    int s[3], n[3];
+    if (!tune_axis(L->nx, &s[0], &n[0]) || !tune_axis(L->ny, &s[1], &n[1]) || !tune_axis(L->nz, &s[2], &n[2])) {
+        const int e = pa_extrema(d_data, dtype, L->n, d_minmax, stream);  // no tuner samples: flags 0
+        return e != PA_OK ? e : pa_autotune(d_data, dtype, L, d_scores15, d_flags8, d_ws, ws_bytes, stream);
+    }
+    const int nblocks = n[0] * n[1] * n[2];
+    if (ws_bytes < pa_autotune_workspace(L) || !d_ws) return PA_ERR_WORKSPACE;
+    hipStream_t st = as_stream(stream);
+    double* part = static_cast<double*>(d_ws);
+    unsigned long long* keys = reinterpret_cast<unsigned long long*>(d_minmax + 2);  // after the result, as pa_extrema
+    hipLaunchKernelGGL(extrema_keys_init_kernel, dim3(1), dim3(1), 0, st, keys);
+    PA_LAUNCH_CHECK();
+    // the tuner's LDS tile leaves three workgroups per CU; the extrema take
+    // the slots the tuner blocks leave (at least one workgroup per CU)
+    const int ntune = 15 * nblocks;
+    const int64_t nv = L->n / (16 / dtype);
+    int64_t ext = 3 * (int64_t)device_cus() - ntune;
+    if (ext < device_cus()) ext = device_cus();
+    int64_t span = (nv + ext - 1) / ext;
+    if (span < 1) span = 1;
+    ext = (nv + span - 1) / span;
+    if (ext < 1) ext = 1;  // the last extrema workgroup also takes the scalar tail
+    const unsigned grid = (unsigned)(ntune + ext);
+    Geom g = make_geom(L);
+    if (dtype == PA_F32)
+        hipLaunchKernelGGL(extrema_tune_kernel<float>, dim3(grid), dim3(kThreads), 0, st, (const float*)d_data, L->n,
+                           keys, span, g, s[0], s[1], s[2], n[0], n[1], nblocks, part);
+    else
+        hipLaunchKernelGGL(extrema_tune_kernel<double>, dim3(grid), dim3(kThreads), 0, st, (const double*)d_data,
+                           L->n, keys, span, g, s[0], s[1], s[2], n[0], n[1], nblocks, part);
+    PA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(extrema_tune_finish_kernel, dim3(1), dim3(64), 0, st, keys, d_minmax, part, nblocks, d_scores15,
+                       d_flags8);
     PA_LAUNCH_CHECK();
     return PA_OK;
 }

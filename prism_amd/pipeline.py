@@ -187,23 +187,26 @@ def compress(data: torch.Tensor, eb: float, rel: bool = True, progressive: bool 
     if not eb > 0:
         raise ValueError("error bound must be positive")
     data = data.contiguous()
-    mm = None
-    if rel:  # first, so the device starts while the host prepares the rest
-        with tm("extrema"):
-            mm = extrema(data)
     nx, ny, nz = shape_xyz(data)
     L = layout(nx, ny, nz)
     dt = DTYPES[data.dtype]
     st = _stream()
     dev = data.device
-    bounds = torch.empty(16, dtype=torch.float64, device=dev)
-    check(lib().pa_level_bounds(_p(mm), float(eb), dt, _p(bounds), st), "pa_level_bounds")
     flags = torch.empty(8, dtype=torch.uint8, device=dev)
     scores = torch.empty(15, dtype=torch.float64, device=dev)
     tws = _empty(max(8, lib().pa_autotune_workspace(C.byref(L))))
-    with tm("autotune"):
-        check(lib().pa_autotune(_p(data), dt, C.byref(L), _p(scores), _p(flags), _p(tws), tws.numel(), st),
-              "pa_autotune")
+    mm = None
+    if rel:  # extrema and auto-tuning share one launch (both only read the field)
+        mm = torch.empty(4, dtype=torch.float64, device=dev)
+        with tm("extrema"):
+            check(lib().pa_extrema_autotune(_p(data), dt, C.byref(L), _p(mm), _p(scores), _p(flags), _p(tws),
+                                            tws.numel(), st), "pa_extrema_autotune")
+    else:
+        with tm("autotune"):
+            check(lib().pa_autotune(_p(data), dt, C.byref(L), _p(scores), _p(flags), _p(tws), tws.numel(), st),
+                  "pa_autotune")
+    bounds = torch.empty(16, dtype=torch.float64, device=dev)
+    check(lib().pa_level_bounds(_p(mm), float(eb), dt, _p(bounds), st), "pa_level_bounds")
     qc = torch.empty(L.code_len, dtype=torch.int32, device=dev)
     anchors = torch.empty(L.n4, dtype=data.dtype, device=dev)
     with tm("predict"):

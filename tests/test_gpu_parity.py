@@ -136,6 +136,18 @@ def test_tuner_parity(pa, oracle, dtype, shape):
     _lib.check(_lib.lib().pa_autotune(P(d), dt, C.byref(L), P(sc), P(fl), P(ws), ws.numel(), S()))
     assert np.array_equal(host(sc).view(np.uint64), sc_o.view(np.uint64))
     assert list(host(fl)) == list(fl_o)
+    # the extrema and the tuner in one launch: the same scores and flags, and
+    # the exact min / max (a field too small for tuner samples takes the two
+    # separate calls inside)
+    announce("extrema + tuner", shape, dtype.__name__)
+    mm = torch.empty(4, dtype=torch.float64, device="cuda")
+    fl2 = torch.full((8,), 7, dtype=torch.uint8, device="cuda")
+    sc2 = torch.empty(15, dtype=torch.float64, device="cuda")
+    _lib.check(_lib.lib().pa_extrema_autotune(P(d), dt, C.byref(L), P(mm), P(sc2), P(fl2), P(ws), ws.numel(), S()))
+    assert np.array_equal(host(sc2).view(np.uint64), sc_o.view(np.uint64))
+    assert list(host(fl2)) == list(fl_o)
+    mmh = host(mm)
+    assert mmh[0] == float(data.min()) and mmh[1] == float(data.max())
 
 
 def _codes_for(oracle, shape, dtype, rel=1e-4, seed=0):
