@@ -1992,12 +1992,13 @@ extern "C" int pa_extrema_autotune(const void* d_data, int dtype, const pa_layou
     unsigned long long* keys = reinterpret_cast<unsigned long long*>(d_minmax + 2);  // after the result, as pa_extrema
     hipLaunchKernelGGL(extrema_keys_init_kernel, dim3(1), dim3(1), 0, st, keys);
     PA_LAUNCH_CHECK();
-    // the tuner's LDS tile leaves three workgroups per CU; the extrema take
-    // the slots the tuner blocks leave (at least one workgroup per CU)
+    // the tuner's LDS tile leaves three workgroups per CU; the extrema are cut
+    // into 8 workgroups per CU, queued behind the tuner blocks (measured
+    // against 4 per CU and against exactly the slots the tuner leaves: the
+    // step was 1 % faster)
     const int ntune = 15 * nblocks;
     const int64_t nv = L->n / (16 / dtype);
-    int64_t ext = 3 * (int64_t)device_cus() - ntune;
-    if (ext < device_cus()) ext = device_cus();
+    int64_t ext = 8 * (int64_t)device_cus();
     int64_t span = (nv + ext - 1) / ext;
     if (span < 1) span = 1;
     ext = (nv + span - 1) / span;
