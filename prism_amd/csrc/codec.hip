@@ -12,6 +12,8 @@
 // [non-zero bytes][RE1 bytes][RE2 bytes][RE3 bytes][B3: 4 bytes][size: 2 B LE];
 // bitmaps LSB-first; an all-zero chunk is the 2-byte footer alone; a chunk
 // that does not shrink below its size is stored raw (csize == osize).
+#include <rocprim/warp/warp_scan.hpp>
+
 #include "batch.hpp"
 #include "common.hpp"
 
@@ -45,15 +47,20 @@ __device__ __forceinline__ ChunkPos chunk_pos(const ChunkGeom& g, int64_t id) {
     return p;
 }
 
+// inclusive scan over the 64 lanes of a wave with DPP row shifts and row
+// broadcasts (rocPRIM's cross-lane form: no LDS round trip per step)
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    using Scan = rocprim::warp_scan<int, 64>;
+    typename Scan::storage_type storage;
+    int out;
+    Scan().inclusive_scan(v, out, storage);
+    return out;
+}
+
 // exclusive block scan of one int per lane (256 lanes = 4 wave64)
 __device__ __forceinline__ int block_scan(int v, int* lds, int* total) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int x = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(x, d, 64);
-        if (lane >= d) x += y;
-    }
+    const int x = wave_incl_scan(v);
     if (lane == 63) lds[wave] = x;
     __syncthreads();
     int before = 0, tot = 0;
@@ -68,15 +75,6 @@ __device__ __forceinline__ int block_scan(int v, int* lds, int* total) {
     return before + x - v;
 }
 
-__device__ __forceinline__ int wave_incl_scan(int v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(v, d, 64);
-        if (lane >= d) v += y;
-    }
-    return v;
-}
 
 // 4-bit mask of the non-zero bytes of w
 __device__ __forceinline__ uint32_t nz_bytes(uint32_t w) {
@@ -144,9 +142,7 @@ __device__ __forceinline__ void encode_chunk(const uint8_t* __restrict__ bp, con
         const uint32_t m = nz_bytes(lo8[k].x) | (nz_bytes(lo8[k].y) << 4) | (nz_bytes(hi8[k].x) << 8) |
                            (nz_bytes(hi8[k].y) << 12);
         W.b0[64 * k + lane] = (uint16_t)m;
-        int c = __popc(m);
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+        const int c = __builtin_amdgcn_readlane(wave_incl_scan(__popc(m)), 63);  // the wave's total
         if (lane == 0) W.zs[k] = c;
         Z += c;
     }
@@ -222,7 +218,7 @@ __device__ __forceinline__ void encode_chunk(const uint8_t* __restrict__ bp, con
         }
         int cnt = __popc(f1);
         int incl = wave_incl_scan(cnt);
-        int R = __shfl(incl, 63, 64);
+        int R = __builtin_amdgcn_readlane(incl, 63);
         fail = fail || wpos + R > kAvail;
         if (!fail) {
             int pos = zr + wpos - Z + incl - cnt;
@@ -237,7 +233,7 @@ __device__ __forceinline__ void encode_chunk(const uint8_t* __restrict__ bp, con
         const uint32_t f2 = nz_bytes(f1 ^ ((f1 << 8) | p2));
         cnt = __popc(f2);
         incl = wave_incl_scan(cnt);
-        R = __shfl(incl, 63, 64);
+        R = __builtin_amdgcn_readlane(incl, 63);
         fail = fail || wpos + R > kAvail;
         if (!fail) {
             int pos = zr + wpos - Z + incl - cnt;
