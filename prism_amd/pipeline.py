@@ -429,27 +429,69 @@ class Retriever:
         self.L = layout(nx, ny, nz)
         self.dt = DTYPES[c.dtype]
         dev = c.stream.device
-        self.t0, self.flags, self.cum, self.anchors = _trailer(c, self.L)
         # the range compress() returns is the one the trailer records
         self.range = c.range if c.range is not None else stream_range(c)
         self.eb_abs = c.eb_input * self.range if c.rel else c.eb_input
         self.rel_eb = c.eb_input if c.rel else 0.0
-        small = torch.zeros(18, dtype=torch.int32, device=dev)  # one fill for state and the byte counter
-        self.state = small[0:16]  # begin[4] end[4] maxlevel narrow
-        self.begin, self.end, self.maxlevel = self.state[0:4], self.state[4:8], self.state[8:9]
-        self.narrow = self.state[9:10]  # codes of this round handed over as int16
+        # state (begin[4] end[4] maxlevel narrow, int32[16]) and the byte
+        # counter (int64) in one zero-filled block; views on demand
+        self._small = torch.zeros(18, dtype=torch.int32, device=dev)
         self.bp = _empty(self.L.host_size)
         self.qc = torch.empty(self.L.code_len, dtype=torch.int32, device=dev)
         self.dws = _empty(lib().pa_decode_workspace(C.byref(self.L)))
         self.sws = _empty(lib().pa_strategy_workspace())
-        self.bytes_read = small[16:18].view(torch.int64)
         self.out = None
         self.rounds = 0
-        # argument values of the per-round calls
-        self._a = dict(cum=_p(self.cum), begin=_p(self.begin), end=_p(self.end), maxlevel=_p(self.maxlevel),
-                       sws=_p(self.sws), stream=_p(c.stream), bp=_p(self.bp), bytes_read=_p(self.bytes_read),
-                       dws=_p(self.dws), qc=_p(self.qc), narrow=_p(self.narrow), anchors=_p(self.anchors),
-                       flags=_p(self.flags), L=C.byref(self.L))
+        # argument values of the per-round calls, from base addresses (the
+        # trailer layout of _trailer: flags at t0, cum at t0 + 16, anchors at
+        # t0 + 1040)
+        item = 4 if c.dtype == torch.float32 else 8
+        self.t0 = c.nbytes - (self.L.trailer_fixed + self.L.n4 * item)
+        if self.t0 < 8 + 2 * self.L.nchunks or self.t0 % 8:
+            raise _lib.PrismError("stream too short or misaligned for this shape")
+        sb, sp = c.stream.data_ptr(), self._small.data_ptr()
+        V = C.c_void_p
+        self._a = dict(cum=V(sb + self.t0 + 16), begin=V(sp), end=V(sp + 16), maxlevel=V(sp + 32),
+                       sws=_p(self.sws), stream=V(sb), bp=_p(self.bp), bytes_read=V(sp + 64), dws=_p(self.dws),
+                       qc=_p(self.qc), narrow=V(sp + 36), anchors=V(sb + self.t0 + 1040), flags=V(sb + self.t0),
+                       L=C.byref(self.L))
+
+    # views of the device state and of the trailer
+    @property
+    def state(self):
+        return self._small[0:16]
+
+    @property
+    def begin(self):
+        return self._small[0:4]
+
+    @property
+    def end(self):
+        return self._small[4:8]
+
+    @property
+    def maxlevel(self):
+        return self._small[8:9]
+
+    @property
+    def narrow(self):  # codes of this round handed over as int16
+        return self._small[9:10]
+
+    @property
+    def bytes_read(self):
+        return self._small[16:18].view(torch.int64)
+
+    @property
+    def flags(self):
+        return _trailer(self.c, self.L)[1]
+
+    @property
+    def cum(self):
+        return _trailer(self.c, self.L)[2]
+
+    @property
+    def anchors(self):
+        return _trailer(self.c, self.L)[3]
 
     def retrieve(self, target: float, timer=None) -> torch.Tensor:
         """Refines to the target error bound (relative in REL mode)."""
@@ -485,8 +527,9 @@ class Retriever:
         desc.nx, desc.ny, desc.nz = nx, ny, nz
         desc.eb, desc.rel_eb = self.eb_abs, self.rel_eb
         desc.target = target * self.range if c.rel else target
-        desc.state, desc.bp, desc.qc = self.state.data_ptr(), self.bp.data_ptr(), self.qc.data_ptr()
-        desc.dws, desc.bytes_read = self.dws.data_ptr(), self.bytes_read.data_ptr()
+        sp = self._small.data_ptr()
+        desc.state, desc.bp, desc.qc = sp, self.bp.data_ptr(), self.qc.data_ptr()
+        desc.dws, desc.bytes_read = self.dws.data_ptr(), sp + 64
         desc.round = self.rounds
         desc.old = self.out.data_ptr() if self.out is not None else None
         desc.out = out.data_ptr()
