@@ -76,11 +76,13 @@ int pa_autotune(const void* d_data, int dtype, const pa_layout* L, double* d_sco
 /* pa_extrema followed by pa_autotune (StatBuffer::findMinMax,
  * compressor.hpp:23-31, and the auto-tuning part of spline_construct,
  * InterpolationPredictor.cu:2273-2355: the two reads of the field before the
- * prediction), run side by side in one launch: the same d_minmax, scores and
- * flags as the two calls.  d_data must be 16-byte aligned; the workspace is
- * pa_autotune's. */
-int pa_extrema_autotune(const void* d_data, int dtype, const pa_layout* L, double* d_minmax,
-                        double* d_scores15, uint8_t* d_flags8, void* d_ws, size_t ws_bytes, void* stream);
+ * prediction), run side by side in one launch, and with d_bounds non-NULL
+ * pa_level_bounds(d_minmax, rel_eb, dtype, d_bounds) after them: the same
+ * d_minmax, scores, flags and bounds as the separate calls.  d_data must be
+ * 16-byte aligned; the workspace is pa_autotune's. */
+int pa_extrema_autotune(const void* d_data, int dtype, const pa_layout* L, double* d_minmax, double rel_eb,
+                        double* d_bounds, double* d_scores15, uint8_t* d_flags8, void* d_ws, size_t ws_bytes,
+                        void* stream);
 
 /* Prediction part of spline_construct (kernel interpolation,
  * InterpolationPredictor.cu:2137-2173): quantisation codes in level-major

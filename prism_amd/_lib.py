@@ -87,7 +87,7 @@ SIGNATURES = {
     "pa_extrema": (I32, [P, I32, I64, P, P]),
     "pa_autotune_workspace": (SZ, [LP]),
     "pa_autotune": (I32, [P, I32, LP, P, P, P, SZ, P]),
-    "pa_extrema_autotune": (I32, [P, I32, LP, P, P, P, P, SZ, P]),
+    "pa_extrema_autotune": (I32, [P, I32, LP, P, D, P, P, P, P, SZ, P]),
     "pa_spline_construct": (I32, [P, I32, LP, D, D, P, P, P, P]),
     "pa_level_bounds": (I32, [P, D, I32, P, P]),
     "pa_spline_construct_dev": (I32, [P, I32, LP, P, P, P, P, P]),

@@ -1638,10 +1638,14 @@ __global__ void extrema_keys_init_kernel(unsigned long long* keys) {
     keys[1] = 0ull;   // running max
 }
 
-// min / max from the keys, and the tuner's pick (as tune_pick_kernel)
+__device__ int64_t fill_bounds(const double* minmax, double eb_in, int rel, int dtype, double alpha, double* e);
+
+// min / max from the keys, the tuner's pick (as tune_pick_kernel) and, with
+// `bounds`, the level bounds of a relative eb (as level_bounds_kernel)
 __global__ void extrema_tune_finish_kernel(const unsigned long long* __restrict__ keys, double* __restrict__ minmax,
                                            const double* __restrict__ part, int nblocks, double* __restrict__ scores,
-                                           uint8_t* __restrict__ flags) {
+                                           uint8_t* __restrict__ flags, double eb, int dtype, double alpha,
+                                           double* __restrict__ bounds) {
     __shared__ double s[kPickLds];
     const int n = nblocks * 15;
     const bool staged = n <= kPickLds;
@@ -1652,6 +1656,10 @@ __global__ void extrema_tune_finish_kernel(const unsigned long long* __restrict_
     minmax[0] = key_value(keys[0]);
     minmax[1] = key_value(keys[1]);
     pick_flags(staged ? s : part, nblocks, scores, flags);
+    if (bounds) {
+        bounds[14] = (double)fill_bounds(minmax, eb, true, dtype, alpha, bounds);
+        bounds[15] = 0.0;
+    }
 }
 
 __global__ void zero_flags_kernel(double* scores, uint8_t* flags) {
@@ -1981,14 +1989,18 @@ static int device_cus() {
     return cache[dev];
 }
 
-extern "C" int pa_extrema_autotune(const void* d_data, int dtype, const pa_layout* L, double* d_minmax,
-                                   double* d_scores15, uint8_t* d_flags8, void* d_ws, size_t ws_bytes, void* stream) {
+extern "C" int pa_extrema_autotune(const void* d_data, int dtype, const pa_layout* L, double* d_minmax, double rel_eb,
+                                   double* d_bounds, double* d_scores15, uint8_t* d_flags8, void* d_ws,
+                                   size_t ws_bytes, void* stream) {
     if (!L || !d_data || !d_minmax || !d_flags8 || !valid_dtype(dtype)) return PA_ERR_ARG;
+    if (d_bounds && !(rel_eb > 0)) return PA_ERR_ARG;
     if (((uintptr_t)d_data & 15) != 0) return PA_ERR_ARG;
     int s[3], n[3];
     if (!tune_axis(L->nx, &s[0], &n[0]) || !tune_axis(L->ny, &s[1], &n[1]) || !tune_axis(L->nz, &s[2], &n[2])) {
-        const int e = pa_extrema(d_data, dtype, L->n, d_minmax, stream);  // no tuner samples: flags 0
-        return e != PA_OK ? e : pa_autotune(d_data, dtype, L, d_scores15, d_flags8, d_ws, ws_bytes, stream);
+        int e = pa_extrema(d_data, dtype, L->n, d_minmax, stream);  // no tuner samples: flags 0
+        if (e == PA_OK) e = pa_autotune(d_data, dtype, L, d_scores15, d_flags8, d_ws, ws_bytes, stream);
+        if (e == PA_OK && d_bounds) e = pa_level_bounds(d_minmax, rel_eb, dtype, d_bounds, stream);
+        return e;
     }
     const int nblocks = n[0] * n[1] * n[2];
     if (ws_bytes < pa_autotune_workspace(L) || !d_ws) return PA_ERR_WORKSPACE;
@@ -2018,7 +2030,7 @@ extern "C" int pa_extrema_autotune(const void* d_data, int dtype, const pa_layou
                            L->n, keys, span, g, s[0], s[1], s[2], n[0], n[1], nblocks, part);
     PA_LAUNCH_CHECK();
     hipLaunchKernelGGL(extrema_tune_finish_kernel, dim3(1), dim3(64), 0, st, keys, d_minmax, part, nblocks, d_scores15,
-                       d_flags8);
+                       d_flags8, rel_eb, dtype, alpha_of(rel_eb), d_bounds);
     PA_LAUNCH_CHECK();
     return PA_OK;
 }

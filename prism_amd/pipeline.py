@@ -195,18 +195,17 @@ def compress(data: torch.Tensor, eb: float, rel: bool = True, progressive: bool 
     flags = torch.empty(8, dtype=torch.uint8, device=dev)
     scores = torch.empty(15, dtype=torch.float64, device=dev)
     tws = _empty(max(8, lib().pa_autotune_workspace(C.byref(L))))
-    mm = None
-    if rel:  # extrema and auto-tuning share one launch (both only read the field)
+    bounds = torch.empty(16, dtype=torch.float64, device=dev)
+    if rel:  # extrema, auto-tuning and the level bounds share one launch sequence
         mm = torch.empty(4, dtype=torch.float64, device=dev)
         with tm("extrema"):
-            check(lib().pa_extrema_autotune(_p(data), dt, C.byref(L), _p(mm), _p(scores), _p(flags), _p(tws),
-                                            tws.numel(), st), "pa_extrema_autotune")
+            check(lib().pa_extrema_autotune(_p(data), dt, C.byref(L), _p(mm), float(eb), _p(bounds), _p(scores),
+                                            _p(flags), _p(tws), tws.numel(), st), "pa_extrema_autotune")
     else:
         with tm("autotune"):
             check(lib().pa_autotune(_p(data), dt, C.byref(L), _p(scores), _p(flags), _p(tws), tws.numel(), st),
                   "pa_autotune")
-    bounds = torch.empty(16, dtype=torch.float64, device=dev)
-    check(lib().pa_level_bounds(_p(mm), float(eb), dt, _p(bounds), st), "pa_level_bounds")
+        check(lib().pa_level_bounds(None, float(eb), dt, _p(bounds), st), "pa_level_bounds")
     qc = torch.empty(L.code_len, dtype=torch.int32, device=dev)
     anchors = torch.empty(L.n4, dtype=data.dtype, device=dev)
     with tm("predict"):

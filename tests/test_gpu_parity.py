@@ -143,11 +143,17 @@ def test_tuner_parity(pa, oracle, dtype, shape):
     mm = torch.empty(4, dtype=torch.float64, device="cuda")
     fl2 = torch.full((8,), 7, dtype=torch.uint8, device="cuda")
     sc2 = torch.empty(15, dtype=torch.float64, device="cuda")
-    _lib.check(_lib.lib().pa_extrema_autotune(P(d), dt, C.byref(L), P(mm), P(sc2), P(fl2), P(ws), ws.numel(), S()))
+    bd = torch.zeros(16, dtype=torch.float64, device="cuda")  # slots of the other dtype stay 0
+    _lib.check(_lib.lib().pa_extrema_autotune(P(d), dt, C.byref(L), P(mm), 1e-4, P(bd), P(sc2), P(fl2), P(ws),
+                                              ws.numel(), S()))
     assert np.array_equal(host(sc2).view(np.uint64), sc_o.view(np.uint64))
     assert list(host(fl2)) == list(fl_o)
     mmh = host(mm)
     assert mmh[0] == float(data.min()) and mmh[1] == float(data.max())
+    # the level bounds it appends equal pa_level_bounds on the same min / max
+    bd2 = torch.zeros(16, dtype=torch.float64, device="cuda")
+    _lib.check(_lib.lib().pa_level_bounds(P(mm), 1e-4, dt, P(bd2), S()))
+    assert np.array_equal(host(bd)[:16].view(np.uint64), host(bd2)[:16].view(np.uint64))
 
 
 def _codes_for(oracle, shape, dtype, rel=1e-4, seed=0):
