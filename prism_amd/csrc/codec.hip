@@ -833,6 +833,14 @@ __global__ __launch_bounds__(kThreads) void retr_decode_kernel(const RetrJob* __
 }
 
 }  // namespace
+
+// Workgroups of a window decode: as many as stay resident at once (the
+// decoder's 22 KB of LDS allows 7 per CU), each walking every grid-th
+// selected chunk, so no workgroup waits for a slot behind the others.
+unsigned decode_grid(int64_t chunks) {
+    const int64_t resident = 7 * (int64_t)device_cus();
+    return (unsigned)(chunks < resident ? chunks : resident);
+}
 }  // namespace pa
 
 using namespace pa;
@@ -843,7 +851,7 @@ int pa::retr_decode(const RetrDev& r, hipStream_t st) {
     hipLaunchKernelGGL(retr_count_scan_kernel, dim3(1), dim3(1024), 0, st, r.jobs, r.nj, r.p_chunk);
     PA_LAUNCH_CHECK();
     if (r.max_chunks > 0) {
-        const unsigned grid = (unsigned)(r.max_chunks < 2048 ? r.max_chunks : 2048);
+        const unsigned grid = decode_grid(r.max_chunks);
         hipLaunchKernelGGL(retr_decode_kernel, dim3(grid), dim3(kThreads), 0, st, r.jobs, r.nj, r.p_chunk);
         PA_LAUNCH_CHECK();
     }
@@ -912,7 +920,7 @@ static int decode_common(const uint8_t* d_in, int64_t in_len, const pa_layout* L
         PA_LAUNCH_CHECK();
     }
     if (window) {
-        const unsigned grid = (unsigned)(L->nchunks < 2048 ? L->nchunks : 2048);
+        const unsigned grid = decode_grid(L->nchunks);
         hipLaunchKernelGGL(decode_kernel<true>, dim3(grid), dim3(kThreads), 0, st, d_in, g, offsets, d_bp, d_begin,
                            d_end, d_bytes_read, err, in_len);
     } else {

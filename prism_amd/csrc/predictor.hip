@@ -1976,19 +1976,6 @@ extern "C" int pa_autotune(const void* d_data, int dtype, const pa_layout* L, do
     return PA_OK;
 }
 
-// Compute units of the current device (cached per device).
-static int device_cus() {
-    static int cache[64] = {0};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-    if (cache[dev] == 0) {
-        int n = 0;
-        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
-        cache[dev] = n;
-    }
-    return cache[dev];
-}
-
 extern "C" int pa_extrema_autotune(const void* d_data, int dtype, const pa_layout* L, double* d_minmax, double rel_eb,
                                    double* d_bounds, double* d_scores15, uint8_t* d_flags8, void* d_ws,
                                    size_t ws_bytes, void* stream) {
