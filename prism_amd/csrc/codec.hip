@@ -316,7 +316,24 @@ __device__ __forceinline__ const uint16_t* scan_runs(const uint16_t* __restrict_
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const bool staged = n <= kScanLds;
     if (staged) {
-        for (int64_t i = tid; i < n; i += 1024) S.sz[i] = sizes[i];
+        // 8-byte words (4 sizes), all of a lane's loads in flight at once:
+        // the table was just written by other CUs, so each round trip is long
+        const bool words = ((uintptr_t)sizes & 7) == 0;
+        const int64_t nw = words ? n / 4 : 0;
+        const uint2* w = reinterpret_cast<const uint2*>(sizes);
+        constexpr int kU = 4;  // 4 x 1024 words cover kScanLds sizes
+        uint2 v[kU];
+#pragma unroll
+        for (int k = 0; k < kU; ++k) {
+            const int64_t i = tid + 1024 * k;
+            v[k] = i < nw ? w[i] : make_uint2(0u, 0u);
+        }
+#pragma unroll
+        for (int k = 0; k < kU; ++k) {
+            const int64_t i = tid + 1024 * k;
+            if (i < nw) reinterpret_cast<uint2*>(S.sz)[i] = v[k];
+        }
+        for (int64_t i = 4 * nw + tid; i < n; i += 1024) S.sz[i] = sizes[i];
         __syncthreads();
     }
     const uint16_t* sz = staged ? S.sz : sizes;

@@ -24,7 +24,7 @@ __device__ __forceinline__ uint64_t plane_cost(int lr, int b) {
     const int sh = 32 - b - den;
     if (sh >= 0) return num << sh;
     const uint64_t d = 1ull << (-sh);
-    return (num + d - 1) / d;
+    return (num + d - 1) >> (-sh);  // the ceiling of num / d, d a power of two
 }
 
 // begin <- end; end <- max(end, choice); maxlevel = largest level that
