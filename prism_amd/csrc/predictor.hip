@@ -1569,6 +1569,8 @@ __global__ __launch_bounds__(kThreads) void tune_kernel(const T* __restrict__ da
 
 // Sum over sample blocks in block order, then the host's pick rule
 // (InterpolationPredictor.cu:2306-2355) on the device.
+__device__ void pick_rule(const double* e, uint8_t* __restrict__ flags);
+
 __device__ void pick_flags(const double* __restrict__ part, int nblocks, double* __restrict__ scores,
                            uint8_t* __restrict__ flags) {
     double e[15];
@@ -1578,6 +1580,11 @@ __device__ void pick_flags(const double* __restrict__ part, int nblocks, double*
         e[c] = s;
         if (scores) scores[c] = s;
     }
+    pick_rule(e, flags);
+}
+
+// The pick rule on the 15 candidate sums.
+__device__ void pick_rule(const double* e, uint8_t* __restrict__ flags) {
     uint8_t f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     double best;
     if (e[0] > e[1]) { best = e[1]; f[7] = 1; } else { best = e[0]; f[7] = 0; }
@@ -1599,17 +1606,33 @@ __device__ void pick_flags(const double* __restrict__ part, int nblocks, double*
 
 // The partial sums go through LDS first (one cooperative load instead of a
 // chain of dependent global loads in the one thread that adds them in order).
+// Lane c < 15 then adds candidate c's sums in block order (the order of
+// pick_flags), and lane 0 applies the rule.
 constexpr int kPickLds = 2048;
-__global__ void tune_pick_kernel(const double* __restrict__ part, int nblocks, double* __restrict__ scores,
-                                 uint8_t* __restrict__ flags) {
-    __shared__ double s[kPickLds];
+__device__ __forceinline__ void pick_parallel(const double* __restrict__ part, int nblocks, double* __restrict__ scores,
+                                              uint8_t* __restrict__ flags, double* s, double* e) {
     const int n = nblocks * 15;
     const bool staged = n <= kPickLds;
     if (staged)
         for (int i = threadIdx.x; i < n; i += blockDim.x) s[i] = part[i];
     __syncthreads();
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    pick_flags(staged ? s : part, nblocks, scores, flags);
+    const double* src = staged ? s : part;
+    if (threadIdx.x < 15) {
+        const int c = threadIdx.x;
+        double acc = 0.0;
+        for (int b = 0; b < nblocks; ++b) acc += src[(int64_t)b * 15 + c];
+        e[c] = acc;
+        if (scores) scores[c] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) pick_rule(e, flags);
+}
+
+__global__ void tune_pick_kernel(const double* __restrict__ part, int nblocks, double* __restrict__ scores,
+                                 uint8_t* __restrict__ flags) {
+    __shared__ double s[kPickLds];
+    __shared__ double e[15];
+    pick_parallel(part, nblocks, scores, flags, s, e);
 }
 
 // ------------------------------------------------ extrema beside the tuner --
@@ -1647,15 +1670,11 @@ __global__ void extrema_tune_finish_kernel(const unsigned long long* __restrict_
                                            uint8_t* __restrict__ flags, double eb, int dtype, double alpha,
                                            double* __restrict__ bounds) {
     __shared__ double s[kPickLds];
-    const int n = nblocks * 15;
-    const bool staged = n <= kPickLds;
-    if (staged)
-        for (int i = threadIdx.x; i < n; i += blockDim.x) s[i] = part[i];
-    __syncthreads();
+    __shared__ double e[15];
+    pick_parallel(part, nblocks, scores, flags, s, e);
     if (threadIdx.x != 0) return;
     minmax[0] = key_value(keys[0]);
     minmax[1] = key_value(keys[1]);
-    pick_flags(staged ? s : part, nblocks, scores, flags);
     if (bounds) {
         bounds[14] = (double)fill_bounds(minmax, eb, true, dtype, alpha, bounds);
         bounds[15] = 0.0;
