@@ -9,6 +9,8 @@
 //     value << 22 | (63 - rev6(b0)) << 16 | (1023 - rev10(tid)) << 6 | (63 - k)
 // (pos = tid + 1024k), which any reduction order finds: each workgroup
 // writes the maximum key of its candidates and one workgroup reduces them.
+#include <rocprim/warp/warp_reduce.hpp>
+
 #include "batch.hpp"
 #include "common.hpp"
 
@@ -56,6 +58,16 @@ __device__ void strategy_apply(unsigned long long key, int32_t* begin, int32_t* 
 
 constexpr int kSearchBlocks = 33 * 33;
 
+// maximum over the 64 lanes of a wave (DPP row shifts, rocPRIM's cross-lane
+// reduction); valid in lane 0
+__device__ __forceinline__ unsigned long long wave_max(unsigned long long v) {
+    using Reduce = rocprim::warp_reduce<unsigned long long, 64>;
+    typename Reduce::storage_type storage;
+    unsigned long long out;
+    Reduce().reduce(v, out, storage, rocprim::maximum<unsigned long long>());
+    return out;
+}
+
 // Grid (b0, b1) = 33 x 33 workgroups; lane p of a workgroup scores b2 =
 // p / 33, b3 = p mod 33 (p < 1089) and the workgroup writes its best key to
 // its own slot (no shared atomics: they serialise across workgroups).
@@ -91,11 +103,7 @@ __device__ __forceinline__ void strategy_search(const uint64_t* __restrict__ cum
             if (key > best_key) best_key = key;
         }
     }
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        const unsigned long long o = __shfl_xor(best_key, d, 64);
-        best_key = o > best_key ? o : best_key;
-    }
+    best_key = wave_max(best_key);
     if ((tid & 63) == 0) s_best[tid >> 6] = best_key;
     __syncthreads();
     if (tid == 0) {
@@ -117,11 +125,7 @@ __device__ __forceinline__ void strategy_reduce_apply(const unsigned long long* 
     __shared__ unsigned long long s_best[4];
     unsigned long long k = 0ull;
     for (int i = threadIdx.x; i < kSearchBlocks; i += 256) k = keys[i] > k ? keys[i] : k;
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        const unsigned long long o = __shfl_xor(k, d, 64);
-        k = o > k ? o : k;
-    }
+    k = wave_max(k);
     if ((threadIdx.x & 63) == 0) s_best[threadIdx.x >> 6] = k;
     __syncthreads();
     if (threadIdx.x == 0) {
