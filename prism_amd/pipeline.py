@@ -192,42 +192,44 @@ def compress(data: torch.Tensor, eb: float, rel: bool = True, progressive: bool 
     dt = DTYPES[data.dtype]
     st = _stream()
     dev = data.device
-    flags = torch.empty(8, dtype=torch.uint8, device=dev)
-    scores = torch.empty(15, dtype=torch.float64, device=dev)
-    tws = _empty(max(8, lib().pa_autotune_workspace(C.byref(L))))
-    bounds = torch.empty(16, dtype=torch.float64, device=dev)
+    V = C.c_void_p
+    # the small per-call buffers in one allocation (byte offsets): bounds
+    # double[16] at 0, min/max double[4] at 128, scores double[15] at 160,
+    # flags uint8[8] at 280, cum int64[128] at 288, sizes int64[2] at 1312,
+    # then the tuner's workspace at 1344
+    tws_bytes = max(8, lib().pa_autotune_workspace(C.byref(L)))
+    small = torch.empty((1344 + tws_bytes + 7) // 8, dtype=torch.float64, device=dev)
+    sp = small.data_ptr()
+    p_bounds, p_mm, p_scores, p_flags, p_cum, p_sizes, p_tws = (V(sp), V(sp + 128), V(sp + 160), V(sp + 280),
+                                                                V(sp + 288), V(sp + 1312), V(sp + 1344))
     if rel:  # extrema, auto-tuning and the level bounds share one launch sequence
-        mm = torch.empty(4, dtype=torch.float64, device=dev)
         with tm("extrema"):
-            check(lib().pa_extrema_autotune(_p(data), dt, C.byref(L), _p(mm), float(eb), _p(bounds), _p(scores),
-                                            _p(flags), _p(tws), tws.numel(), st), "pa_extrema_autotune")
+            check(lib().pa_extrema_autotune(_p(data), dt, C.byref(L), p_mm, float(eb), p_bounds, p_scores, p_flags,
+                                            p_tws, tws_bytes, st), "pa_extrema_autotune")
     else:
         with tm("autotune"):
-            check(lib().pa_autotune(_p(data), dt, C.byref(L), _p(scores), _p(flags), _p(tws), tws.numel(), st),
-                  "pa_autotune")
-        check(lib().pa_level_bounds(None, float(eb), dt, _p(bounds), st), "pa_level_bounds")
+            check(lib().pa_autotune(_p(data), dt, C.byref(L), p_scores, p_flags, p_tws, tws_bytes, st), "pa_autotune")
+        check(lib().pa_level_bounds(None, float(eb), dt, p_bounds, st), "pa_level_bounds")
     qc = torch.empty(L.code_len, dtype=torch.int32, device=dev)
     anchors = torch.empty(L.n4, dtype=data.dtype, device=dev)
     with tm("predict"):
-        check(lib().pa_spline_construct_dev(_p(data), dt, C.byref(L), _p(bounds), _p(flags), _p(qc), _p(anchors),
-                                            st), "pa_spline_construct_dev")
+        check(lib().pa_spline_construct_dev(_p(data), dt, C.byref(L), p_bounds, p_flags, _p(qc), _p(anchors), st),
+              "pa_spline_construct_dev")
     bp = _empty(L.host_size)
     mode = PA_MODE_NB if progressive else PA_MODE_SM
     with tm("shuffle"):
         check(lib().pa_bitshuffle(_p(qc), C.byref(L), mode, _p(bp), st), "pa_bitshuffle")
     bound = lib().pa_compress_bound(C.byref(L), dt)
     out = _empty(bound)
-    cum = torch.empty(128, dtype=torch.int64, device=dev)
-    sizes = torch.empty(2, dtype=torch.int64, device=dev)
     ews = _empty(lib().pa_encode_workspace(C.byref(L)))
     with tm("encode"):
-        check(lib().pa_lossless_encode(_p(bp), C.byref(L), _p(out), _p(cum), _p(sizes), _p(ews), ews.numel(), st),
+        check(lib().pa_lossless_encode(_p(bp), C.byref(L), _p(out), p_cum, p_sizes, _p(ews), ews.numel(), st),
               "pa_lossless_encode")
     with tm("merge"):  # the stream length goes to bounds[15] (int64 bits)
-        check(lib().pa_compress_merge(_p(out), C.byref(L), dt, _p(sizes), _p(flags), _p(bounds[1:2]), _p(cum),
-                                      _p(anchors), _p(bounds[15:16]), st), "pa_compress_merge")
+        check(lib().pa_compress_merge(_p(out), C.byref(L), dt, p_sizes, p_flags, V(sp + 8), p_cum, _p(anchors),
+                                      V(sp + 120), st), "pa_compress_merge")
     # the one host read: stream length, status, range
-    vals = _host_read(bounds)
+    vals = _host_read(small[0:16])
     if vals[14] != 0:
         raise ValueError("error bound must be positive (constant field in REL mode?)")
     total = int(vals.view(np.int64)[15])
@@ -235,8 +237,8 @@ def compress(data: torch.Tensor, eb: float, rel: bool = True, progressive: bool 
     c = Compressed(stream=out[:total], shape=(nx, ny, nz), dtype=data.dtype, eb_input=eb, rel=rel,
                    progressive=progressive, range=rng, flags=[])
     if return_parts:
-        return c, {"qc": qc, "anchors": anchors, "bp": bp, "cum": cum, "flags": flags, "scores": scores,
-                   "layout": L}
+        return c, {"qc": qc, "anchors": anchors, "bp": bp, "cum": small[36:164].view(torch.int64),
+                   "flags": small[35:36].view(torch.uint8), "scores": small[20:35], "layout": L}
     return c
 
 
