@@ -437,10 +437,8 @@ class Retriever:
         # state (begin[4] end[4] maxlevel narrow, int32[16]) and the byte
         # counter (int64) in one zero-filled block; views on demand
         self._small = torch.zeros(18, dtype=torch.int32, device=dev)
-        self.bp = _empty(self.L.host_size)
-        self.qc = torch.empty(self.L.code_len, dtype=torch.int32, device=dev)
-        self.dws = _empty(lib().pa_decode_workspace(C.byref(self.L)))
         self.sws = _empty(lib().pa_strategy_workspace())
+        self._bufs = None  # bit-planes, codes and decoder workspace, allocated on first use
         self.out = None
         self.rounds = 0
         # argument values of the per-round calls, from base addresses (the
@@ -453,9 +451,32 @@ class Retriever:
         sb, sp = c.stream.data_ptr(), self._small.data_ptr()
         V = C.c_void_p
         self._a = dict(cum=V(sb + self.t0 + 16), begin=V(sp), end=V(sp + 16), maxlevel=V(sp + 32),
-                       sws=_p(self.sws), stream=V(sb), bp=_p(self.bp), bytes_read=V(sp + 64), dws=_p(self.dws),
-                       qc=_p(self.qc), narrow=V(sp + 36), anchors=V(sb + self.t0 + 1040), flags=V(sb + self.t0),
-                       L=C.byref(self.L))
+                       sws=_p(self.sws), stream=V(sb), bytes_read=V(sp + 64), narrow=V(sp + 36),
+                       anchors=V(sb + self.t0 + 1040), flags=V(sb + self.t0), L=C.byref(self.L))
+
+    def _buffers(self):
+        """(bp, qc, dws), allocated at first use: the first round's strategy
+        search is already queued by then."""
+        if self._bufs is None:
+            dev = self._small.device
+            bp = _empty(self.L.host_size)
+            qc = torch.empty(self.L.code_len, dtype=torch.int32, device=dev)
+            dws = _empty(lib().pa_decode_workspace(C.byref(self.L)))
+            self._bufs = (bp, qc, dws)
+            self._a.update(bp=_p(bp), qc=_p(qc), dws=_p(dws))
+        return self._bufs
+
+    @property
+    def bp(self):
+        return self._buffers()[0]
+
+    @property
+    def qc(self):
+        return self._buffers()[1]
+
+    @property
+    def dws(self):
+        return self._buffers()[2]
 
     # views of the device state and of the trailer
     @property
@@ -504,15 +525,16 @@ class Retriever:
         with tm("strategy"):
             check(lb.pa_find_strategy(a["cum"], a["begin"], a["end"], a["maxlevel"], self.eb_abs, target_abs,
                                       a["sws"], self.sws.numel(), st), "pa_find_strategy")
+        dws = self._buffers()[2]
         with tm("decode"):
             check(lb.pa_lossless_decode_window(a["stream"], self.c.nbytes, a["L"], a["begin"], a["end"], a["bp"],
-                                               a["bytes_read"], int(self.rounds > 0), a["dws"], self.dws.numel(), st),
+                                               a["bytes_read"], int(self.rounds > 0), a["dws"], dws.numel(), st),
                   "pa_lossless_decode_window")
         with tm("unshuffle"):
             check(lb.pa_inverse_bitshuffle_window_narrow(a["bp"], a["L"], a["begin"], a["end"], a["qc"], a["narrow"],
                                                          st), "pa_inverse_bitshuffle_window_narrow")
         nx, ny, nz = self.c.shape
-        new = torch.empty((nz, ny, nx), dtype=self.c.dtype, device=self.qc.device)
+        new = torch.empty((nz, ny, nx), dtype=self.c.dtype, device=self._small.device)
         with tm("reconstruct"):
             self._reconstruct(new, st)
         self.out = new
