@@ -2,6 +2,8 @@
 // compressor.cu:228-302) shared by the extrema kernels and the launch that
 // runs the extrema beside the auto-tuner.
 #pragma once
+#include <rocprim/warp/warp_reduce.hpp>
+
 #include "common.hpp"
 
 namespace pa {
@@ -18,6 +20,25 @@ __device__ inline double key_value(unsigned long long k) {
 }
 
 constexpr int64_t kExtBatch = 8 * 256;  // vectors per load batch (8 per lane in flight)
+
+struct FMin {
+    __device__ double operator()(double a, double b) const { return fmin(a, b); }
+};
+struct FMax {
+    __device__ double operator()(double a, double b) const { return fmax(a, b); }
+};
+
+// min / max over the 64 lanes of a wave with DPP row shifts (exact: the
+// order of the comparisons does not matter); valid in every lane
+__device__ inline void wave_minmax(double& mn, double& mx) {
+    using Reduce = rocprim::warp_reduce<double, 64>;
+    typename Reduce::storage_type storage;
+    double a, b;
+    Reduce().reduce(mn, a, storage, FMin());
+    Reduce().reduce(mx, b, storage, FMax());
+    mn = a;
+    mx = b;
+}
 
 // Workgroup b of the batch streams `span` 16-byte vectors of its field in
 // batches of 8 loads per lane (the field's tail elements go to its last
@@ -58,11 +79,7 @@ __device__ inline void extrema_span(const T* in, int64_t n, int64_t wg, int64_t 
             mx = fmax(mx, (double)in[i]);
         }
     }
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        mn = fmin(mn, __shfl_xor(mn, d, 64));
-        mx = fmax(mx, __shfl_xor(mx, d, 64));
-    }
+    wave_minmax(mn, mx);
     if ((threadIdx.x & 63) == 0) {
         s_min[threadIdx.x >> 6] = mn;
         s_max[threadIdx.x >> 6] = mx;
