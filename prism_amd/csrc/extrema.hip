@@ -1,7 +1,7 @@
 // extrema.hip -- exact min / max of a field (StatBuffer::findMinMax,
 // compressor.cu:228-302) on gfx950.
 //
-// One streaming pass with 16-byte loads; wave64 then workgroup reduction,
+// One streaming pass with 16-byte loads; wave64 (DPP) then workgroup reduction,
 // folded into two 64-bit words with atomicMin / atomicMax on an
 // order-preserving integer image of the double value.  Min and max are
 // exact, so the result does not depend on the reduction order.
@@ -63,11 +63,7 @@ __global__ __launch_bounds__(256) void extrema_kernel(const T* __restrict__ in, 
         mn = fmin(mn, (double)in[i]);
         mx = fmax(mx, (double)in[i]);
     }
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        mn = fmin(mn, __shfl_xor(mn, d, 64));
-        mx = fmax(mx, __shfl_xor(mx, d, 64));
-    }
+    wave_minmax(mn, mx);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) {
         s_min[wave] = mn;
