@@ -76,6 +76,41 @@ def test_extrema(pa, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", [(33, 47, 29), (65, 101, 129)])
+def test_extrema_planted(pa, dtype, shape):
+    """Extremes at the first element, in the scalar tail (n is not a whole
+    number of 16-byte vectors) and in the middle, over all-negative and
+    mixed-sign fields: the order-preserving key image flips every bit of a
+    negative value, so a sign slip there shows up as a wrong min or max.
+    Both the standalone entry and the one beside the tuner."""
+    _lib, pipe = pa
+    nz, ny, nx = shape
+    n = nx * ny * nz
+    assert n % (16 // np.dtype(dtype).itemsize) != 0
+    L = _lib.layout(nx, ny, nz)
+    dt = _lib.PA_F32 if dtype == np.float32 else _lib.PA_F64
+    ws = torch.empty(max(8, _lib.lib().pa_autotune_workspace(C.byref(L))), dtype=torch.uint8, device="cuda")
+    rng = np.random.default_rng(5)
+    big = 1e30 if dtype == np.float32 else 1e300
+    for case, (imin, imax) in enumerate([(0, n - 1), (n - 1, 0), (n // 2, n // 3)]):
+        for offset in (-50.0, 0.0):
+            announce("extrema planted", shape, dtype.__name__, imin, imax, offset)
+            a = (rng.uniform(-1, 1, n) + offset).astype(dtype)
+            a[imin] = -big if offset == 0.0 else offset - 10
+            a[imax] = big if offset == 0.0 else offset + 10
+            mm = host(pipe.extrema(dev(a)))
+            assert mm[0] == float(a.min()) and mm[1] == float(a.max())
+            mm2 = torch.empty(4, dtype=torch.float64, device="cuda")
+            sc = torch.empty(15, dtype=torch.float64, device="cuda")
+            fl = torch.empty(8, dtype=torch.uint8, device="cuda")
+            bd = torch.zeros(16, dtype=torch.float64, device="cuda")
+            _lib.check(_lib.lib().pa_extrema_autotune(P(dev(a)), dt, C.byref(L), P(mm2), 1e-4, P(bd), P(sc), P(fl),
+                                                      P(ws), ws.numel(), S()))
+            mmh = host(mm2)
+            assert mmh[0] == float(a.min()) and mmh[1] == float(a.max())
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("shape", SHAPES)
 def test_predictor_parity(pa, oracle, dtype, shape):
     _lib, pipe = pa
