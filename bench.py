@@ -52,7 +52,7 @@ def parse():
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build shards nothing)",
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (prism_amd runs each field on one GPU and shards nothing)",
               file=sys.stderr)
         sys.exit(2)
     import numpy as np
