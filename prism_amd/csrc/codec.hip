@@ -149,33 +149,18 @@ __device__ __forceinline__ void encode_chunk(const uint8_t* __restrict__ bp, con
     bool fail = Z >= kAvail;
     wave_sync();
     if (!fail) {
-        // the surviving bytes, step by step; a step's 16 bytes per lane are
-        // read again (from cache) rather than kept live in registers.  The
-        // staging slot is written in whole 4-byte words, each once: a step's
-        // bytes follow the r < 4 bytes carried from the previous step's last
-        // partial word, and its own last partial word is carried on (to the
-        // tail after the last step).
-        auto load_step = [&](int k, uint32_t (&w)[4]) {
-            const int b = 16 * (64 * k + lane);
-            w[0] = w[1] = w[2] = w[3] = 0u;
-            if (b < osize) {
-                const uint2 lo = *reinterpret_cast<const uint2*>(src + b);
-                w[0] = lo.x;
-                w[1] = lo.y;
-                if (b + 8 < osize) {
-                    const uint2 hi = *reinterpret_cast<const uint2*>(src + b + 8);
-                    w[2] = hi.x;
-                    w[3] = hi.y;
-                }
-            }
-        };
+        // the surviving bytes, step by step, from the registers the bitmap
+        // pass filled (the loop is unrolled so they stay registers).  Read
+        // again instead, they missed L2 -- 16 KiB per wave in flight is more
+        // than an XCD's L2 holds -- and the chunk crossed the fabric twice.
+        // The staging slot is written in whole 4-byte words, each once: a
+        // step's bytes follow the r < 4 bytes carried from the previous
+        // step's last partial word, and its own last partial word is carried
+        // on (to the tail after the last step).
         int zpos = 0;
-        uint32_t nx[4];
-        load_step(0, nx);
-#pragma unroll 1
+#pragma unroll
         for (int k = 0; k < kSteps; ++k) {
-            uint32_t w[4] = {nx[0], nx[1], nx[2], nx[3]};
-            if (k + 1 < kSteps) load_step(k + 1, nx);  // the next step's bytes load during this one
+            const uint32_t w[4] = {lo8[k].x, lo8[k].y, hi8[k].x, hi8[k].y};
             const int zs = W.zs[k];
             if (zs == 0) continue;  // wave-uniform
             const int r = zpos & 3;
