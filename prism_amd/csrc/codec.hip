@@ -150,7 +150,7 @@ __device__ __forceinline__ void encode_chunk(const uint8_t* __restrict__ bp, con
     wave_sync();
     if (!fail) {
         // the surviving bytes, step by step, from the registers the bitmap
-        // pass filled (the loop is unrolled so they stay registers).  Read
+        // pass filled (the loop is unrolled so they stay in registers).  Read
         // again instead, they missed L2 -- 16 KiB per wave in flight is more
         // than an XCD's L2 holds -- and the chunk crossed the fabric twice.
         // The staging slot is written in whole 4-byte words, each once: a
