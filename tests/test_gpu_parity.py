@@ -237,8 +237,12 @@ def _gpu_encode(_lib, L, bp_np):
 
 
 @pytest.mark.parametrize("shape", [(33, 47, 29), (64, 64, 64), (300, 1, 1), (1, 100, 70), (128, 128, 128)])
-@pytest.mark.parametrize("kind", ["codes", "sparse", "dense", "zeros"])
+@pytest.mark.parametrize("kind", ["codes", "sparse", "dense", "zeros", "full", "blocky"])
 def test_codec_parity(pa, oracle, shape, kind):
+    """Chunk codec against the oracle.  `full` (no zero byte) stores every
+    chunk raw; `blocky` gives each 1 KiB step of a chunk its own density --
+    empty, sparse, dense or full -- so the zero-eliminated bytes skip whole
+    steps and carry partial words across them."""
     _lib, pipe = pa
     nz, ny, nx = shape
     announce("codec", shape, kind)
@@ -251,6 +255,12 @@ def test_codec_parity(pa, oracle, shape, kind):
         bp_np = np.where(rng.random(L.host_size) < 0.03, rng.integers(1, 256, L.host_size), 0).astype(np.uint8)
     elif kind == "dense":
         bp_np = np.where(rng.random(L.host_size) < 0.6, rng.integers(1, 256, L.host_size), 0).astype(np.uint8)
+    elif kind == "full":
+        bp_np = rng.integers(1, 256, L.host_size).astype(np.uint8)
+    elif kind == "blocky":
+        nblk = (L.host_size + 1023) // 1024
+        p = np.repeat(rng.choice([0.0, 0.01, 0.5, 1.0], nblk), 1024)[:L.host_size]
+        bp_np = np.where(rng.random(L.host_size) < p, rng.integers(1, 256, L.host_size), 0).astype(np.uint8)
     else:
         bp_np = np.zeros(L.host_size, np.uint8)
     s_o, cum_o = oracle.lossless_encode(bp_np, Lo)
