@@ -81,7 +81,8 @@ def main():
         c = pipeline.compress(field, args.rel_eb, rel=True, progressive=True, timer=timer)
         r = pipeline.Retriever(c)
         for t in args.targets:
-            r.retrieve(t, timer=timer)
+            r.retrieve(t, timer=timer, check_stream=False)
+        r.check()
         return c, r
 
     for _ in range(args.warmup):
@@ -100,7 +101,8 @@ def main():
         b = time.perf_counter()
         r = pipeline.Retriever(c)
         for t in args.targets:
-            r.retrieve(t)
+            r.retrieve(t, check_stream=False)
+        r.check()  # the decoder's error word, read once after the last round
         torch.cuda.synchronize()
         e = time.perf_counter()
         t_cmp += b - a
@@ -287,57 +289,58 @@ def serving_section(args):
     pool = [fields.batch_field(j, 300 + j, device="cuda") for j in range(16)]
     torch.cuda.synchronize()
     # warm-up on a throwaway server (allocator, first launches)
-    warm = serve.Server(budget_bytes=100e9)
-    for k, c in enumerate(comp):
-        warm.add_field(k, c)
-    warm.run(serve.poisson_requests(64, rate=5000.0, seed=99, field_ids=list(range(16)), pool=pool), pace=True)
+    serving_run(comp, serve.poisson_requests(64, rate=5000.0, seed=99, field_ids=list(range(16)), pool=pool))
     torch.cuda.synchronize()
-    del warm
-    srv = serve.Server(budget_bytes=100e9, max_batch=64, max_inflight=2)
-    for k, c in enumerate(comp):
-        srv.add_field(k, c)
     reqs = serve.poisson_requests(args.serve_requests, rate=args.serve_rate, seed=5, field_ids=list(range(16)),
                                   pool=pool)
-    out = serving_run(srv, reqs)
+    out = serving_run(comp, reqs)
     out["rate_offered"] = args.serve_rate
     # the same request mix offered as a burst (1e6 req/s): what the loop
     # sustains when requests queue up and batches fill
-    sat = serve.Server(budget_bytes=100e9, max_batch=64, max_inflight=2)
-    for k, c in enumerate(comp):
-        sat.add_field(k, c)
     reqs = serve.poisson_requests(args.serve_requests, rate=1e6, seed=5, field_ids=list(range(16)), pool=pool)
-    out["saturated"] = serving_run(sat, reqs)
+    out["saturated"] = serving_run(comp, reqs)
     out["saturated"]["rate_offered"] = 1e6
     return out
 
 
-def serving_run(srv, reqs):
+def serving_run(comp, reqs):
+    """Runs `reqs` on a fresh server over the resident fields `comp`; each
+    finished request is accounted in the server's completion callback (its
+    latency and field bytes), after which the server drops its result."""
     import numpy as np
     import torch
 
     from prism_amd import serve
 
+    acct = []  # (kind, state, latency, field bytes)
+
+    def on_done(r):
+        acct.append((r.kind, r.state, r.latency, serve.field_bytes(srv, r), r.done))
+
+    srv = serve.Server(budget_bytes=100e9, max_batch=64, max_inflight=2, on_done=on_done)
+    for k, c in enumerate(comp):
+        srv.add_field(k, c)
     t0 = srv.run(reqs, pace=True)
     torch.cuda.synchronize()
-    done = [r for r in reqs if r.state == "done"]
-    lat = np.array([r.latency for r in done]) * 1e3
-    wall = max(r.done for r in done) - t0
+    done = [a for a in acct if a[1] == "done"]
+    lat = np.array([a[2] for a in done]) * 1e3
+    wall = max(a[4] for a in done) - t0
     payload = sum(int(res.retriever.bytes_read.item()) for res in srv.fields.values())
-    nbytes = sum(serve.field_bytes(srv, r) for r in done) + payload
+    nbytes = sum(a[3] for a in done) + payload
 
     def pct(a, q):
         return float(np.percentile(a, q)) if len(a) else None
 
     by = {}
     for kind in ("retrieve", "compress"):
-        a = np.array([r.latency for r in done if r.kind == kind]) * 1e3
+        a = np.array([d[2] for d in done if d[0] == kind]) * 1e3
         by[kind] = {"count": int(len(a)), "p50_ms": pct(a, 50), "p99_ms": pct(a, 99)}
     return {"requests": len(reqs), "completed": len(done),
             "wall_s": wall, "throughput_req_s": len(done) / wall, "GBps": nbytes / wall / 1e9,
             "algorithmic_bytes": nbytes, "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "mean_ms": float(lat.mean()),
             "max_ms": float(lat.max()), "by_kind": by, "batches": srv.batches,
             "mean_batch": len(done) / max(srv.batches, 1), "resident_fields": len(srv.fields),
-            "resident_bytes": srv.resident_bytes, "budget_bytes": srv.budget}
+            "resident_bytes": srv.resident_bytes, "held_bytes_at_end": srv.held_bytes, "budget_bytes": srv.budget}
 
 
 def measured_traffic(stage, nx, ny, nz, s):

@@ -11,7 +11,7 @@
  * through a workspace argument sized by the matching *_workspace() query.
  *
  * Data are T[nz][ny][nx] with x fastest; dtype is PA_F32 or PA_F64 (the
- * reference's prism_dtype values F4 / F8, include/utils/traits.hpp:81-95).
+ * reference's prism_dtype values F4 / F8, include/utils/traits.hpp:21-35).
  * flags8 = {use_md[0..3], reverse[0..3]} indexed by level_id (0 = unit 1).
  * begin/end are int32[4] indexed by lr = 3 - level_id, like the reference.
  */
@@ -55,7 +55,7 @@ typedef struct pa_layout {
 } pa_layout;
 
 /* Bitplane::Bitplane + calculate_aligned_buffer_size (io.hpp:86-92,
- * io.cu:283-302) and Compressor::init sizes (compressor.cu:77-117). Host only. */
+ * io.cu:117-127) and Compressor::init sizes (compressor.cu:77-117). Host only. */
 int pa_layout_init(pa_layout* L, uint32_t nx, uint32_t ny, uint32_t nz);
 /* upper bound of a compressed stream (payload + pad + trailer) in bytes */
 int64_t pa_compress_bound(const pa_layout* L, int dtype);

@@ -32,6 +32,10 @@ CPP = os.path.join(PKG, "cpp")
 CPP_SOURCES = ["parameters.cpp", "compressor.cpp", "prism.cpp"]
 CPP_LIB = os.path.join(OUT, "libprism_cpp.so")
 CLI = os.path.join(OUT, "prism")
+# a driver written against the reference's header names only (include path
+# = the directory of the API headers), linked like any external caller
+EXAMPLE_SRC = os.path.join(CPP, "examples", "reference_driver.cpp")
+EXAMPLE = os.path.join(OUT, "reference_driver")
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
           "-I", os.path.join(ROOT, "include"), "-I", SRC]
@@ -94,9 +98,22 @@ def _link(target: str, inputs, extra, force: bool):
     os.replace(target + ".tmp", target)
 
 
+def _build_example(force: bool):
+    if not force and not _stale(EXAMPLE, [EXAMPLE_SRC, CPP_LIB] + _cpp_headers()):
+        return
+    obj = EXAMPLE + ".o"
+    cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(CPP, "include", "prism"), "-I",
+           os.path.join(ROOT, "include"), "-c", EXAMPLE_SRC, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for the reference-API driver:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    _link(EXAMPLE, [obj, CPP_LIB, LIB], ["-Wl,-rpath,$ORIGIN"], True)
+
+
 def build(force: bool = False, verbose: bool = False) -> str:
-    """Builds libprism_amd.so (kernels + C ABI), libprism_cpp.so (C++ API)
-    and the `prism` executable into prism_amd/_build; returns the first."""
+    """Builds libprism_amd.so (kernels + C ABI), libprism_cpp.so (C++ API),
+    the `prism` executable and the reference-API driver into
+    prism_amd/_build; returns the first."""
     os.makedirs(OUT, exist_ok=True)
     jobs = min(len(SOURCES) + len(CPP_SOURCES), int(os.environ.get("MAX_JOBS", "8")))
     with cf.ThreadPoolExecutor(jobs) as ex:
@@ -106,10 +123,12 @@ def build(force: bool = False, verbose: bool = False) -> str:
     rpath = "-Wl,-rpath,$ORIGIN"
     _link(CPP_LIB, cpp_objs[:-1] + [LIB], ["-shared", "-fPIC", rpath], force)
     _link(CLI, cpp_objs[-1:] + [CPP_LIB, LIB], [rpath], force)
+    _build_example(force)
     if verbose:
         print(LIB)
         print(CPP_LIB)
         print(CLI)
+        print(EXAMPLE)
     return LIB
 
 

@@ -5,6 +5,10 @@
 // 20 log10(range) - 10 log10(MSE), NRMSE = sqrt(MSE) / range, per-stage times
 // with GiB/s of original data, and, per progressive round, the payload bytes
 // that the reference's accounting attributes to the planes retrieved.
+//
+// statistic, print_result and calculate_accumulate_size are at global scope
+// with the reference's parameter lists; prism::calculate_accumulate_size adds
+// the payload bytes the decoder actually read (the covering chunks).
 #pragma once
 
 #include <cmath>
@@ -12,8 +16,6 @@
 #include <cstdio>
 
 #include <hip/hip_runtime.h>
-
-namespace prism {
 
 template <typename T>
 void statistic(const T* ori, const T* data, size_t n) {
@@ -37,10 +39,14 @@ void statistic(const T* ori, const T* data, size_t n) {
 }
 
 // decmp = 0: compression stages; decmp = 1: decompression stages, plus the
-// strategy search when progressive = 1.  Times in ms.
+// strategy search when progressive = 1.  Times in ms.  targeteb and
+// targetreb are accepted for the reference's signature and not printed
+// (analyze.hpp:75-77 does not print them either).
 template <typename T, int decmp = 1, int progressive = 1>
-void print_result(size_t size, size_t total_compressed_size, double time_enum, double time_pred,
-                  double time_bitplane, double time_lossless) {
+void print_result(size_t size, size_t total_compressed_size, double targeteb, double targetreb, double time_enum,
+                  double time_pred, double time_bitplane, double time_lossless) {
+    (void)targeteb;
+    (void)targetreb;
     std::printf("\n  \033[1m%-12s %-12s %-20s\033[0m\n", "type", "time (ms)", "throughtput (GB/s)");
     const double total_bytes = (double)size * sizeof(T);
     auto th = [total_bytes](double ms) { return total_bytes / 1024 / 1024 / 1024 / ms * 1000; };
@@ -62,13 +68,16 @@ void print_result(size_t size, size_t total_compressed_size, double time_enum, d
     }
 }
 
+namespace prism {
+
 // Payload bytes of the planes [begin, end) of each level from the
 // cumulative size table, the reference's accounting (analyze.hpp:118-158);
 // loaded_size accumulates over rounds.  actually_read is the payload the
-// decoder read this round (the covering chunks).
+// decoder read this round (the covering chunks); a negative value omits
+// that line.
 template <typename T>
 void calculate_accumulate_size(size_t& loaded_size, size_t total_size, const int* begin_d, const int* end_d,
-                               const size_t* cum_d, uint64_t actually_read) {
+                               const size_t* cum_d, long long actually_read) {
     int b0[4], b1[4];
     size_t cum[4][32];
     if (hipMemcpy(b0, begin_d, sizeof(b0), hipMemcpyDeviceToHost) != hipSuccess ||
@@ -93,8 +102,15 @@ void calculate_accumulate_size(size_t& loaded_size, size_t total_size, const int
     std::printf("  Data Chunk: %zu (%lf%%)\n", round, 100.0 * round / total_bytes);
     std::printf("  Total Retrieved size: %zu (%lf%%)\n", loaded_size, 100.0 * loaded_size / total_bytes);
     std::printf("  Bit rate: %lf\n", sizeof(T) * 8.0 * loaded_size / total_bytes);
-    std::printf("  Payload read by the decoder: %llu\n", (unsigned long long)actually_read);
+    if (actually_read >= 0) std::printf("  Payload read by the decoder: %lld\n", actually_read);
     std::printf("--------------------------------------\n");
 }
 
 }  // namespace prism
+
+// the reference's form (analyze.hpp:118)
+template <typename T>
+void calculate_accumulate_size(size_t& loaded_size, size_t total_size, int* begin_d, int* end_d,
+                               size_t* compressedSize_bp_d) {
+    prism::calculate_accumulate_size<T>(loaded_size, total_size, begin_d, end_d, compressedSize_bp_d, -1);
+}

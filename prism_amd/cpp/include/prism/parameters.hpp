@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "err.hpp"
 #include "traits.hpp"
 
 struct interpolation_parameters {

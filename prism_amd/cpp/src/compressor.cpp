@@ -2,6 +2,7 @@
 // the C ABI.  Stage order and stream contents follow the reference's
 // src/compressor.cu:63-226 and src/utils/io.cu:10-127.
 #include "prism/compressor.hpp"
+#include "prism/dataloader.hpp"
 
 #include <cstdio>
 #include <cstdlib>
@@ -113,15 +114,25 @@ void Buffer::H2D_async(void* st, long long n) {
 
 // -------------------------------------------------------------- StatBuffer --
 template <typename T>
-double StatBuffer<T>::findMinMax() {
+void extrema_scan(T* data, size_t len, T* result) {
     double* d_mm = nullptr;
     check_hip(hipMalloc((void**)&d_mm, 4 * sizeof(double)), "hipMalloc");
-    check_pa(pa_extrema(d, sizeof(T) == 4 ? PA_F32 : PA_F64, (int64_t)len, d_mm, nullptr), "pa_extrema");
+    check_pa(pa_extrema(data, sizeof(T) == 4 ? PA_F32 : PA_F64, (int64_t)len, d_mm, nullptr), "pa_extrema");
     double mm[2];
     check_hip(hipMemcpy(mm, d_mm, sizeof(mm), hipMemcpyDeviceToHost), "D2H");
     check_hip(hipFree(d_mm), "hipFree");
-    minVal = mm[0];
-    maxVal = mm[1];
+    result[0] = (T)mm[0];  // the extrema of T values are T values: exact
+    result[1] = (T)mm[1];
+}
+template void extrema_scan<float>(float*, size_t, float*);
+template void extrema_scan<double>(double*, size_t, double*);
+
+template <typename T>
+double StatBuffer<T>::findMinMax() {
+    T result[2];
+    extrema_scan<T>(reinterpret_cast<T*>(d), len, result);
+    minVal = result[0];
+    maxVal = result[1];
     range = maxVal - minVal;
     return range;
 }
@@ -147,6 +158,23 @@ Bitplane::Bitplane(uint32_t x, uint32_t y, uint32_t z) {
 }
 
 // -------------------------------------------------------------- Compressor --
+// Bytes of the code buffer beyond the n element codes: L.code_len = n + n4 +
+// dev_align int32 (the anchor slots and the padding of the level regions
+// come on top of the elements' codes).
+static int code_buffer_extra(const pa_layout& L) { return (int)(4 * (L.n4 + L.dev_align)); }
+
+}  // namespace prism
+
+// Size in bytes of the code buffer Compressor::init allocates for a shape
+// (0 for an invalid shape); host only, for checking it against the layout.
+extern "C" size_t prism_code_buffer_bytes(uint32_t x, uint32_t y, uint32_t z) {
+    pa_layout L;
+    if (pa_layout_init(&L, x, y, z) != PA_OK) return 0;
+    return 4 * (size_t)L.n + (size_t)prism::code_buffer_extra(L);
+}
+
+namespace prism {
+
 template <typename T, typename E>
 Compressor<T, E>::~Compressor() {
     delete qc;
@@ -175,7 +203,11 @@ void Compressor<T, E>::init(prism_context* config) {
     len = (size_t)L.n;
     bytes_ = len * sizeof(T);
     compute_align();
-    qc = new inBuffer(I4, (int)(4 * L.dev_align), x, y, z);  // L.code_len int32
+    qc = new inBuffer(I4, code_buffer_extra(L), x, y, z);
+    if (qc->bytes < 4 * (size_t)L.code_len) {
+        std::fprintf(stderr, "[PRISM ERROR] code buffer %zu B < %lld B\n", qc->bytes, 4LL * L.code_len);
+        std::exit(EXIT_FAILURE);
+    }
     ap = new inBuffer(config->dtype, 0, (x + 15) / 16, (y + 15) / 16, (z + 15) / 16);
     compressed_data = new Buffer(U1, 0, (uint32_t)1);
     profiling_errors = new Buffer(F8, 0, 15);
@@ -392,3 +424,33 @@ template class Compressor<float, i4>;
 template class Compressor<double, i4>;
 
 }  // namespace prism
+
+// ------------------------------------------------------------ dataloader --
+template <typename E>
+void findStrategy_h(size_t* compressedSize_bp_d, int* begin, int* end, double eb, double targetError, double& time,
+                    void* stream) {
+    using prism::check_hip;
+    const size_t wsb = pa_strategy_workspace();
+    void* ws = nullptr;
+    int32_t* d_maxlevel = nullptr;
+    check_hip(hipMalloc(&ws, wsb ? wsb : 8), "hipMalloc");
+    check_hip(hipMalloc((void**)&d_maxlevel, sizeof(int32_t)), "hipMalloc");
+    hipEvent_t a, b;
+    check_hip(hipEventCreate(&a), "hipEventCreate");
+    check_hip(hipEventCreate(&b), "hipEventCreate");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    check_hip(hipEventRecord(a, st), "hipEventRecord");
+    prism::check_pa(pa_find_strategy(reinterpret_cast<const uint64_t*>(compressedSize_bp_d), begin, end, d_maxlevel, eb,
+                                     targetError, ws, wsb, stream),
+                    "pa_find_strategy");
+    check_hip(hipEventRecord(b, st), "hipEventRecord");
+    check_hip(hipEventSynchronize(b), "hipEventSynchronize");
+    float ms = 0.f;
+    check_hip(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+    time = ms;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    check_hip(hipFree(ws), "hipFree");
+    check_hip(hipFree(d_maxlevel), "hipFree");
+}
+template void findStrategy_h<int32_t>(size_t*, int*, int*, double, double, double&, void*);

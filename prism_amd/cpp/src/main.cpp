@@ -1,5 +1,5 @@
 // main.cpp -- the `prism` command line (flags of the reference's
-// src/utils/parameters.cpp; see usage()).
+// src/utils/parameters.cu; see usage()).
 #include <cstdio>
 #include <cstdlib>
 #include <stdexcept>

@@ -1,5 +1,5 @@
 // parameters.cpp -- the PRISM command line (flags, defaults and file-name
-// rules of the reference's src/utils/parameters.cpp:9-234).
+// rules of the reference's src/utils/parameters.cu:9-234).
 #include "prism/parameters.hpp"
 
 #include <hip/hip_runtime.h>

@@ -41,8 +41,8 @@ void prism_compress(prism_context* c, void* stream) {
     cmp.compressed_data->unload_tofile(c->cmpFilePath, (long long)cmp.total_compressed_size,
                                        prism::typetofile::deviceTofile);
     if (c->report_time)
-        prism::print_result<T, 0, 0>(c->size, cmp.total_compressed_size, 0.0, cmp.time_pred, cmp.time_bitplane,
-                                     cmp.time_encode);
+        print_result<T, 0, 0>(c->size, cmp.total_compressed_size, c->eb, c->rel_eb, cmp.itime_enum, cmp.time_pred,
+                              cmp.time_bitplane, cmp.time_encode);
 }
 
 template <typename T>
@@ -91,8 +91,8 @@ void prism_decompress(prism_context* c, void* stream) {
     out->D2H();
     out->unload_tofile(c->decFilePath, prism::typetofile::hostTofile);
     if (c->report_time)
-        prism::print_result<T, 1, 0>(c->size, d.total_compressed_size, d.itime_enum, d.itime_pred, d.itime_bitplane,
-                                     d.itime_decode);
+        print_result<T, 1, 0>(c->size, d.total_compressed_size, c->eb, c->rel_eb, d.itime_enum, d.itime_pred,
+                              d.itime_bitplane, d.itime_decode);
     if (c->report_cr) report_quality<T>(c, out.get());
 }
 
@@ -119,12 +119,12 @@ void prism_progressive_decompress(prism_context* c, void* stream) {
                                           i ? c->target_ebs[i - 1] : 0.0, stream);
         cur->D2H();
         if (c->report_time)
-            prism::print_result<T, 1, 1>(c->size, d.total_compressed_size, d.itime_enum, d.itime_pred,
-                                         d.itime_bitplane, d.itime_decode);
+            print_result<T, 1, 1>(c->size, d.total_compressed_size, c->eb, c->rel_eb, d.itime_enum, d.itime_pred,
+                                  d.itime_bitplane, d.itime_decode);
         if (c->report_cr) {
             report_quality<T>(c, cur.get());
             prism::calculate_accumulate_size<T>(loaded, c->size, c->begin, c->end, d.compressedSize_bp_d,
-                                                d.bytes_read);
+                                                (long long)d.bytes_read);
         }
     }
     cur->unload_tofile(c->decFilePath, prism::typetofile::hostTofile);

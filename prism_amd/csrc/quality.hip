@@ -4,7 +4,8 @@
 // Figures: L-infinity error and the first index where it occurs, the sum of
 // squared errors, min and max of the original, then MSE, PSNR = 20 log10
 // (range) - 10 log10(MSE), NRMSE = sqrt(MSE) / range and Max_RE = L-inf /
-// range.  Workgroups write partials to their own slots and one workgroup
+// range.  A NaN in either array gives an infinite L-infinity error.
+// Workgroups write partials to their own slots and one workgroup
 // folds them in a fixed order, so every figure is deterministic.
 #include "common.hpp"
 
@@ -19,6 +20,10 @@ struct Part {
     int64_t idx;
     double sse, mn, mx;
 };
+
+// A NaN error (a NaN in either array) counts as an infinite one, so that it
+// wins the maximum instead of failing every comparison.
+__device__ __forceinline__ double nan_as_inf(double e) { return e == e ? e : __builtin_inf(); }
 
 __device__ __forceinline__ void fold(Part& a, const Part& b) {
     if (b.max_err > a.max_err || (b.max_err == a.max_err && b.idx < a.idx)) {
@@ -60,7 +65,7 @@ __global__ __launch_bounds__(kThreads) void quality_kernel(const T* __restrict__
         }
 #pragma unroll
         for (int k = 0; k < kBatch; ++k) {
-            const double ov = (double)o[k], e = __builtin_fabs((double)r[k] - ov);
+            const double ov = (double)o[k], e = nan_as_inf(__builtin_fabs((double)r[k] - ov));
             if (e > p.max_err) {
                 p.max_err = e;
                 p.idx = i + k * kThreads;
@@ -71,7 +76,7 @@ __global__ __launch_bounds__(kThreads) void quality_kernel(const T* __restrict__
         }
     }
     for (; i < hi; i += kThreads) {
-        const double ov = (double)ori[i], e = __builtin_fabs((double)rec[i] - ov);
+        const double ov = (double)ori[i], e = nan_as_inf(__builtin_fabs((double)rec[i] - ov));
         if (e > p.max_err) {
             p.max_err = e;
             p.idx = i;

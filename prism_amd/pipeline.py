@@ -393,9 +393,18 @@ def stream_range(c: Compressed) -> float:
     return float(c.stream[t0 + 8:t0 + 16].view(torch.float64).item())
 
 
-def decompress(c: Compressed) -> torch.Tensor:
+def _decoder_error(dws: torch.Tensor) -> int:
+    """The decoder's error word (first int32 of its workspace); waits for the
+    current stream."""
+    return int(_host_read(dws[0:4].view(torch.int32))[0])
+
+
+def decompress(c: Compressed, check_stream: bool = True) -> torch.Tensor:
     """Full decompression of a plain (SM) stream, decompress_pipeline
-    (compressor.cu:153-164)."""
+    (compressor.cu:153-164).  With `check_stream` (default) the call waits
+    for the device and raises PrismError when the decoder found the stream
+    malformed (a size-table entry pointing outside the stream: the chunks it
+    names decode to zeros); without it nothing is synchronised."""
     nx, ny, nz = c.shape
     L = layout(nx, ny, nz)
     dt = DTYPES[c.dtype]
@@ -413,6 +422,10 @@ def decompress(c: Compressed) -> torch.Tensor:
     out = torch.empty((nz, ny, nx), dtype=c.dtype, device=c.stream.device)
     check(lib().pa_spline_reconstruct(_p(qc), _p(anchors), dt, C.byref(L), eb_abs, rel_eb, _p(flags), None,
                                       _p(out), st), "pa_spline_reconstruct")
+    if check_stream:
+        err = _decoder_error(ws)
+        if err:
+            raise _lib.PrismError(f"malformed compressed stream (decoder status {err})")
     return out
 
 
@@ -515,8 +528,26 @@ class Retriever:
     def anchors(self):
         return _trailer(self.c, self.L)[3]
 
-    def retrieve(self, target: float, timer=None) -> torch.Tensor:
-        """Refines to the target error bound (relative in REL mode)."""
+    def error(self) -> int:
+        """The decoder's error bits for this field so far (0: every chunk
+        read was well formed); waits for the current stream.  The bits are
+        sticky: once a round has met a malformed chunk, every later round of
+        the field reports it."""
+        if self._bufs is None:
+            return 0
+        return _decoder_error(self._bufs[2])
+
+    def check(self) -> None:
+        """Raises PrismError if a round of this field met a malformed chunk."""
+        err = self.error()
+        if err:
+            raise _lib.PrismError(f"malformed compressed stream (decoder status {err})")
+
+    def retrieve(self, target: float, timer=None, check_stream: bool = True) -> torch.Tensor:
+        """Refines to the target error bound (relative in REL mode).  With
+        `check_stream` (default) the call waits for the round and raises
+        PrismError if the decoder met a malformed chunk; without it the round
+        is only queued, and check() reports the error later."""
         tm = timer or _NO_TIMER
         st = _stream()
         a = self._a
@@ -539,6 +570,8 @@ class Retriever:
             self._reconstruct(new, st)
         self.out = new
         self.rounds += 1
+        if check_stream:
+            self.check()
         return new
 
     def job(self, target: float, out: torch.Tensor, desc) -> None:
@@ -574,12 +607,22 @@ class Retriever:
 
 
 class RetrievalBatch:
-    """One enqueued batch of retrieval rounds: the new reconstructions and
-    the buffers that must outlive the device work."""
+    """One enqueued batch of retrieval rounds: the new reconstructions, the
+    jobs' decoder error words (copied to pinned host memory behind the
+    batch) and the buffers that must outlive the device work."""
 
-    def __init__(self, outs, keep):
+    def __init__(self, outs, keep, errors=None):
         self.outs = outs
         self._keep = keep
+        self._errors = errors
+
+    def errors(self):
+        """Per job, the field's decoder error bits after this round (0: well
+        formed).  Valid once the batch's work has completed (an event
+        recorded after it has fired, or after a synchronisation)."""
+        if self._errors is None:
+            return [0] * len(self.outs)
+        return [int(v) for v in self._errors.tolist()]
 
 
 def retrieve_batch(items, pinned: bool = False) -> RetrievalBatch:
@@ -608,6 +651,15 @@ def retrieve_batch(items, pinned: bool = False) -> RetrievalBatch:
         tb = lib().pa_retrieve_table_bytes(nj)
         h = torch.empty(tb, dtype=torch.uint8, pin_memory=True)
     check(lib().pa_retrieve_batch(arr, nj, _p(h), tb, _p(ws), ws.numel(), _stream()), "pa_retrieve_batch")
+    # each job's error word, gathered by one kernel and copied to the host
+    # behind the batch (read by errors() once the batch has completed)
+    errs = torch.cat([r.dws[0:4].view(torch.int32) for r, _ in items])
+    errs_h = torch.empty(nj, dtype=torch.int32, pin_memory=True)
+    errs_h.copy_(errs, non_blocking=True)
+    # the previous reconstructions the batch reads (job.old) stay alive until
+    # the caller has synchronised, even if commit() drops the retriever's
+    # own reference to them
+    olds = [r.out for r, _ in items]
     for (r, _), new in zip(items, outs):
         r.commit(new)
-    return RetrievalBatch(outs, (ws, h, [r.out for r, _ in items]))
+    return RetrievalBatch(outs, (ws, h, olds, errs), errs_h)

@@ -70,29 +70,76 @@ def _compress_need(t: torch.Tensor) -> int:
     return int(bound + 4 * L.code_len + L.host_size + L.nchunks * (16384 + 8) + L.n4 * t.element_size() + 8192)
 
 
+def _result_bytes(req: Request) -> int:
+    """Device bytes a completed request's result holds: a reconstruction,
+    or the stream of a compression (a view of its batch's output buffer,
+    which stays allocated as long as any of its streams is held)."""
+    res = req.result
+    if res is None:
+        return 0
+    if isinstance(res, torch.Tensor):
+        return _tensor_bytes(res)
+    return _tensor_bytes(res.stream)
+
+
 class Server:
     """Continuous-batching request loop on the current HIP stream.
 
     budget_bytes bounds the device memory the server holds: resident fields
-    (stream, decoded bit-planes, codes, current reconstruction) plus the
+    (stream, decoded bit-planes, codes, current reconstruction), the
     transient memory of admitted requests (new reconstructions, compression
-    workspaces and outputs) until they complete.  max_inflight batches may be
-    queued on the stream at once; max_batch requests form one batch, chosen
-    among the first 4 x max_batch queued (a field's later rounds wait their
-    turn)."""
+    workspaces and outputs) until they complete, and the results of
+    completed requests until they are handed over.  Results are handed over
+    in one of two ways:
+      - on_done: called with each finished request (done, failed, rejected
+        or cancelled) as soon as the loop sees it; the server keeps nothing
+        after the call returns (it drops req.result), so the callback must
+        consume or copy what it needs;
+      - otherwise finished requests collect in `completed`, their results
+        counted against the budget (`held_bytes`) until pop_completed()
+        hands them to the caller.
+    A request that cannot be admitted because held results fill the budget
+    while nothing is in flight raises MemoryError instead of waiting for
+    ever.  max_inflight batches may be queued on the stream at once;
+    max_batch requests form one batch, chosen among the first 4 x max_batch
+    queued (a field's later rounds wait their turn)."""
 
-    def __init__(self, budget_bytes: int, max_batch: int = 64, max_inflight: int = 2, clock=time.perf_counter):
+    def __init__(self, budget_bytes: int, max_batch: int = 64, max_inflight: int = 2, clock=time.perf_counter,
+                 on_done=None):
         self.budget = int(budget_bytes)
         self.max_batch = max_batch
         self.max_inflight = max_inflight
         self.clock = clock
+        self.on_done = on_done
         self.fields: dict[int, Resident] = {}
         self.queue: deque[Request] = deque()
         self.inflight: deque = deque()  # (event, requests, transient bytes, keep-alive)
         self.resident_bytes = 0
         self.transient_bytes = 0
+        self.held_bytes = 0
         self.completed: list[Request] = []
+        self.finished = 0  # requests finished so far, whichever way they were handed over
         self.batches = 0
+
+    def _finish(self, req: Request) -> None:
+        self.finished += 1
+        if self.on_done is not None:
+            self.on_done(req)
+            req.result = None
+            return
+        nb = _result_bytes(req)
+        req._held = nb
+        self.held_bytes += nb
+        self.completed.append(req)
+
+    def pop_completed(self) -> list:
+        """Hands the finished requests (and their device results) to the
+        caller; the server stops counting their results."""
+        out, self.completed = self.completed, []
+        for r in out:
+            self.held_bytes -= getattr(r, "_held", 0)
+            r._held = 0
+        return out
 
     # ------------------------------------------------------------ fields --
     def add_field(self, fid: int, c: pipeline.Compressed) -> None:
@@ -110,21 +157,21 @@ class Server:
 
     @property
     def used(self) -> int:
-        return self.resident_bytes + self.transient_bytes
+        return self.resident_bytes + self.transient_bytes + self.held_bytes
 
     # ---------------------------------------------------------- requests --
     def submit(self, req: Request) -> None:
         if req.kind == "retrieve":
             if req.field not in self.fields:
                 req.state, req.error = "rejected", f"no resident field {req.field}"
-                self.completed.append(req)
+                self._finish(req)
                 return
         elif req.kind != "compress":
             raise ValueError(f"unknown request kind {req.kind!r}")
         need = self._need(req)
         if need > self.budget - self.resident_bytes:
             req.state, req.error = "rejected", f"needs {need} B, more than the budget leaves"
-            self.completed.append(req)
+            self._finish(req)
             return
         self.queue.append(req)
 
@@ -134,7 +181,7 @@ class Server:
             if r.rid == rid:
                 self.queue.remove(r)
                 r.state = "cancelled"
-                self.completed.append(r)
+                self._finish(r)
                 return True
         return False
 
@@ -154,6 +201,9 @@ class Server:
         while self.queue and len(self.inflight) < self.max_inflight:
             batch, need = self._admit()
             if not batch:
+                if not self.inflight and self.held_bytes > 0:
+                    raise MemoryError(f"request {self.queue[0].rid} waits for memory held by {len(self.completed)} "
+                                      f"results not yet taken with pop_completed() ({self.held_bytes} B)")
                 break
             self._launch(batch, need)
             launched += len(batch)
@@ -197,6 +247,8 @@ class Server:
             for r, out in zip(rets, rb.outs):
                 r.result = out
             keep.append(rb)
+        else:
+            rb = None
         bc = None
         if comps:
             bc = pipeline.BatchCompressor([r.data for r in comps], [r.eb for r in comps], rel=True,
@@ -208,14 +260,14 @@ class Server:
         for r in batch:
             r.state, r.admitted = "running", now
         self.transient_bytes += need
-        self.inflight.append((ev, batch, need, keep, comps, bc))
+        self.inflight.append((ev, batch, need, keep, comps, bc, rets, rb))
         self.batches += 1
 
     def poll(self) -> int:
         """Marks the requests of completed batches done (in stream order)."""
         n = 0
         while self.inflight and self.inflight[0][0].query():
-            ev, batch, need, keep, comps, bc = self.inflight.popleft()
+            ev, batch, need, keep, comps, bc, rets, rb = self.inflight.popleft()
             now = self.clock()
             if bc is not None:
                 res = bc.results(strict=False)
@@ -223,13 +275,21 @@ class Server:
                     if c is None:
                         r.state, r.error = "failed", "non-positive error bound"
                     else:
+                        if self.on_done is None:
+                            # a held stream gets its own allocation, so that it
+                            # does not keep the whole batch output alive
+                            c.stream = c.stream.clone()
                         r.result = c
+            if rb is not None:
+                for r, err in zip(rets, rb.errors()):
+                    if err:
+                        r.state, r.error, r.result = "failed", f"malformed compressed stream (decoder status {err})", None
+            self.transient_bytes -= need
             for r in batch:
                 if r.state == "running":
                     r.state = "done"
                 r.done = now
-                self.completed.append(r)
-            self.transient_bytes -= need
+                self._finish(r)
             n += len(batch)
         return n
 

@@ -54,3 +54,60 @@ def test_rejects_bad_arguments():
     assert prism_amd.lib().pa_layout_init(C.byref(L), 0, 4, 4) == _lib.PA_ERR_ARG
     assert prism_amd.lib().pa_layout_init(C.byref(L), 2048, 2048, 1024) == _lib.PA_ERR_ARG  # 2^32 elements
     assert prism_amd.lib().pa_bitshuffle(None, C.byref(L), 0, None, None) == _lib.PA_ERR_ARG
+
+
+# ---- the C++ API library (libprism_cpp.so) and a caller that uses only the
+# reference's API names ----
+
+CPP_LIB = os.path.join(ROOT, "prism_amd", "_build", "libprism_cpp.so")
+DRIVER = os.path.join(ROOT, "prism_amd", "_build", "reference_driver")
+
+
+@pytest.mark.parametrize("shape", [(384, 384, 256), (512, 512, 512), (20, 24, 28), (64, 64, 64), (17, 1, 1),
+                                   (1, 1, 1), (2048, 2048, 1), (10000000, 1, 1), (33, 47, 129)])
+def test_cpp_code_buffer_holds_code_len(shape):
+    """Compressor::init's code buffer is exactly L.code_len int32: the
+    predictor clears up to code_len and the inverse shuffle writes the
+    phantom level-0 codes past the n elements."""
+    import ctypes as C
+    import prism_amd
+    lib = C.CDLL(CPP_LIB)
+    lib.prism_code_buffer_bytes.restype = C.c_size_t
+    lib.prism_code_buffer_bytes.argtypes = [C.c_uint32] * 3
+    L = prism_amd.layout(*shape)
+    assert lib.prism_code_buffer_bytes(*shape) == 4 * L.code_len
+
+
+def _exported(path):
+    import subprocess
+    out = subprocess.run(["nm", "-D", "-C", "--defined-only", path], capture_output=True, text=True, check=True)
+    return out.stdout
+
+
+def test_cpp_library_exports_the_reference_api():
+    """The names a driver of the reference's C++ API links against
+    (compressor.hpp, io.hpp, dataloader.hpp, parameters.hpp, prism.hpp)."""
+    syms = _exported(CPP_LIB)
+    for name in ["prism::extrema_scan<float>(float*, unsigned long, float*)",
+                 "prism::extrema_scan<double>(double*, unsigned long, double*)",
+                 "prism::StatBuffer<float>::findMinMax()",
+                 "prism::Compressor<float, int>::compress_pipeline(", "prism::Compressor<double, int>::init(",
+                 "prism::Compressor<double, int>::decompress_progressive_pipeline(",
+                 "prism::Buffer::H2D(long long)", "prism::Bitplane::Bitplane(",
+                 "void findStrategy_h<int>(unsigned long*, int*, int*, double, double, double&, void*)",
+                 "parse_argv(prism_context*, int, char**)", "initialize(prism_context*)"]:
+        assert name in syms, name
+
+
+def test_reference_api_driver_is_built_and_linked():
+    """The reference-API driver (prism_amd/cpp/examples) compiled against the
+    API headers only and linked with libprism_cpp.so; its GPU run is in
+    tests/test_cli.py."""
+    import subprocess
+    assert os.path.exists(DRIVER), "run __graft_entry__.build()"
+    dyn = subprocess.run(["readelf", "-d", DRIVER], capture_output=True, text=True, check=True).stdout
+    assert "libprism_cpp.so" in dyn and "libprism_amd.so" in dyn
+    undef = subprocess.run(["nm", "-D", "-C", "--undefined-only", DRIVER], capture_output=True, text=True,
+                           check=True).stdout
+    for name in ("prism::extrema_scan<float>", "findStrategy_h<int>", "prism::Compressor<float, int>::init"):
+        assert name in undef, name

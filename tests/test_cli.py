@@ -131,3 +131,40 @@ def test_malformed_stream_is_rejected(oracle, tmp_path):
     r = run("-i", "f.bin", "-f", "-3", nx, ny, nz, "-R", 1e-3, "-x", cwd=tmp_path)
     assert r.returncode != 0
     assert "malformed compressed stream" in r.stderr
+
+
+DRIVER = os.path.join(ROOT, "prism_amd", "_build", "reference_driver")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype, flag", [(np.float32, "-f"), (np.float64, "-d")])
+def test_reference_api_driver_files_equal_oracle(oracle, tmp_path, dtype, flag):
+    """A driver written against the reference's C++ API names only (one
+    Compressor, one stream reload and one output buffer per progressive
+    round, as src/prism.cu does) writes the oracle's stream and, after the
+    last round, the oracle's reconstruction; the free findStrategy_h finds
+    nothing left to read for the last target."""
+    if not os.path.exists(DRIVER):
+        pytest.fail("reference_driver not built (run __graft_entry__.build())")
+    shape = (29, 34, 41)
+    data = _field(shape, dtype, 13)
+    src = tmp_path / "field.bin"
+    data.tofile(src)
+    nz, ny, nx = shape
+    targets = [1e-2, 1e-3, 1e-5]
+    print("CASE reference driver", dtype.__name__, flush=True)
+    r = subprocess.run([DRIVER, "-i", str(src), flag, "-3", str(nx), str(ny), str(nz), "-R", "1e-5", "-z",
+                        str(tmp_path / "c.prisma"), "-x", str(tmp_path / "d.out"), "-prog", "-errors",
+                        f"-{len(targets)}", *map(str, targets), "--report", "time,cr"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("Total Retrieved size") == len(targets)
+    assert r.stdout.count("PSNR") == len(targets) and "extrema disagree" not in r.stdout
+    assert "findStrategy_h after the last round: no new planes" in r.stdout
+    ref, _ = oracle.compress(data, 1e-5, mode=0)
+    assert np.array_equal(np.fromfile(tmp_path / "c.prisma", np.uint8), ref)
+    p = oracle.Progressive(ref, shape, dtype, 1e-5)
+    for t in targets:
+        last, _, _ = p.retrieve(t)
+    dec = np.fromfile(tmp_path / "d.out", dtype).reshape(shape)
+    assert np.array_equal(dec.view(np.uint8), last.view(np.uint8))

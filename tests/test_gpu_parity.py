@@ -425,3 +425,78 @@ def test_narrow_handoff_equals_int32(pa, oracle):
         assert torch.equal(got.view(torch.uint8), new.view(torch.uint8)), k
         assert torch.equal(r.state[0:9], st32[0:9])
     assert flags_seen[0] == 0 and 1 in flags_seen[1:], flags_seen
+
+
+# ---- the closed-form pins of the tuner's sample geometry and of the
+# partial-tile rule selection, run through the HIP kernels ----
+
+from closed_forms import (RULE_COEFS, RULE_SHAPES, TUNER_SHAPES, codes_by_point, cubic_axes_field,  # noqa: E402
+                          expected_codes, noise_outside_windows, tuner_blocks)
+
+
+def _gpu_scores(_lib, d, L, dt, both=True):
+    fl = torch.empty(8, dtype=torch.uint8, device="cuda")
+    sc = torch.empty(15, dtype=torch.float64, device="cuda")
+    ws = torch.empty(max(8, _lib.lib().pa_autotune_workspace(C.byref(L))), dtype=torch.uint8, device="cuda")
+    _lib.check(_lib.lib().pa_autotune(P(d), dt, C.byref(L), P(sc), P(fl), P(ws), ws.numel(), S()))
+    out = [host(sc)]
+    if both:
+        mm = torch.empty(4, dtype=torch.float64, device="cuda")
+        sc2 = torch.empty(15, dtype=torch.float64, device="cuda")
+        _lib.check(_lib.lib().pa_extrema_autotune(P(d), dt, C.byref(L), P(mm), 1e-4, None, P(sc2), P(fl), P(ws),
+                                                  ws.numel(), S()))
+        out.append(host(sc2))
+    return out
+
+
+@pytest.mark.parametrize("shape_xyz", TUNER_SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_tuner_sample_windows(pa, oracle, shape_xyz):
+    """Both tuner entries read exactly the hand-derived sample windows:
+    zero scores on noise outside them, positive scores for an impulse at
+    either corner of every window, and scores bit-equal to the oracle's."""
+    _lib, pipe = pa
+    nx, ny, nz = shape_xyz
+    L = _lib.layout(nx, ny, nz)
+    announce("tuner windows", shape_xyz)
+    f = noise_outside_windows(shape_xyz, np.float32)
+    d = dev(f)
+    sc_o, _, _ = oracle.autotune(f)
+    for sc in _gpu_scores(_lib, d, L, _lib.PA_F32):
+        assert not sc.any() and np.array_equal(sc.view(np.uint64), sc_o.view(np.uint64))
+    d.zero_()
+    for ox, oy, oz in tuner_blocks(shape_xyz):
+        far = [o + min(16, 8 * ((n - 1 - o) // 8)) for o, n in ((ox, nx), (oy, ny), (oz, nz))]
+        for px, py, pz in ((ox, oy, oz), tuple(far)):
+            announce("tuner impulse", shape_xyz, (px, py, pz))
+            d[pz, py, px] = 1.0
+            for sc in _gpu_scores(_lib, d, L, _lib.PA_F32, both=(px, py, pz) == (ox, oy, oz)):
+                assert (sc > 0).all(), ((px, py, pz), sc)
+            d[pz, py, px] = 0.0
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", RULE_SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_partial_tile_rules(pa, oracle, dtype, shape):
+    """Codes of the one-axis cubic fields equal the closed-form rule
+    remainders (and the oracle's), and both reconstructions are exact."""
+    _lib, pipe = pa
+    nz, ny, nx = shape
+    L = _lib.layout(nx, ny, nz)
+    dt = _lib.PA_F32 if dtype == np.float32 else _lib.PA_F64
+    for coef in RULE_COEFS:
+        data = cubic_axes_field(shape, coef, dtype)
+        d = dev(data)
+        for flags in FLAGSETS + [[1] * 8]:
+            announce("partial-tile rules", shape, dtype.__name__, coef, flags)
+            fl = dev(np.array(flags, np.uint8))
+            qc = torch.empty(L.code_len, dtype=torch.int32, device="cuda")
+            an = torch.empty(L.n4, dtype=d.dtype, device="cuda")
+            _lib.check(_lib.lib().pa_spline_construct(P(d), dt, C.byref(L), 0.25, 0.0, P(fl), P(qc), P(an), S()))
+            got = host(qc)
+            assert np.array_equal(codes_by_point(L, got, shape), expected_codes(shape, coef, flags))
+            qc_o, _ = oracle.spline_construct(data, 0.25, 0.0, flags)
+            assert np.array_equal(got, qc_o)
+            out = torch.empty(shape, dtype=d.dtype, device="cuda")
+            _lib.check(_lib.lib().pa_spline_reconstruct(P(qc), P(an), dt, C.byref(L), 0.25, 0.0, P(fl), None,
+                                                         P(out), S()))
+            assert np.array_equal(host(out), data)

@@ -2,6 +2,8 @@
 memory admission, cancellation and failed requests."""
 import sys
 
+import numpy as np
+
 import pytest
 
 pytestmark = pytest.mark.gpu
@@ -96,3 +98,92 @@ def test_admission_cancel_and_failure(mods):
     want = pipeline.compress(big, big_eb, rel=True, progressive=True).stream.cpu()
     for i in range(3):
         assert torch.equal(reqs[i].result.stream.cpu(), want)
+
+
+def test_held_results_count_against_the_budget(mods):
+    """Results kept in `completed` count against the budget until
+    pop_completed() hands them over; a queue blocked only by held results
+    raises MemoryError instead of spinning; with a completion callback the
+    server holds nothing.  Results equal the plain path either way."""
+    fields, pipeline, serve = mods
+    announce("serve: held results and the budget")
+    comp = resident_set(fields, pipeline, 2, 910)
+    out_bytes = max(int(np.prod(c.shape)) * (4 if c.dtype == torch.float32 else 8) for c in comp)
+    targets = [1e-2, 1e-3, 1e-4, 1e-5, 1e-6, 1e-6, 1e-5, 1e-3]
+
+    def requests():
+        return [serve.Request(rid=i, kind="retrieve", arrival=0.0, field=i % 2, target=t)
+                for i, t in enumerate(targets)]
+
+    srv = serve.Server(budget_bytes=1 << 40)
+    for k, c in enumerate(comp):
+        srv.add_field(k, c)
+    srv.budget = srv.resident_bytes + 3 * out_bytes  # room for three reconstructions
+    reqs = requests()
+    for r in reqs:
+        srv.submit(r)
+    taken, blocked = [], 0
+    while srv.queue or srv.inflight:
+        try:
+            srv.drain()
+        except MemoryError:
+            blocked += 1
+            assert srv.held_bytes > 0 and not srv.inflight
+            taken += srv.pop_completed()
+            assert srv.held_bytes == 0
+    taken += srv.pop_completed()
+    assert blocked >= 1 and len(taken) == len(reqs) and all(r.state == "done" for r in reqs)
+    alone = [pipeline.Retriever(c) for c in comp]
+    for r in reqs:
+        want = alone[r.field].retrieve(r.target)
+        assert torch.equal(want.view(torch.uint8), r.result.view(torch.uint8)), r.rid
+    # the same requests with a callback: nothing is held, nothing blocks
+    seen = []
+    srv2 = serve.Server(budget_bytes=1 << 40, on_done=lambda r: seen.append((r.rid, r.state, r.result is not None)))
+    for k, c in enumerate(comp):
+        srv2.add_field(k, c)
+    srv2.budget = srv2.resident_bytes + 3 * out_bytes
+    for r in requests():
+        srv2.submit(r)
+    srv2.drain()
+    assert sorted(s[0] for s in seen) == list(range(len(targets))) and all(s[1] == "done" and s[2] for s in seen)
+    assert srv2.held_bytes == 0 and not srv2.completed
+
+
+def _corrupted(pipeline, c):
+    """The stream with four size-table entries pointing far past its end."""
+    bad = c.stream.clone()
+    bad[8:16] = 0xFF
+    return pipeline.Compressed(stream=bad, shape=c.shape, dtype=c.dtype, eb_input=c.eb_input, rel=c.rel,
+                               progressive=c.progressive, range=c.range)
+
+
+def test_malformed_streams_are_reported(mods):
+    """A stream whose size table points outside it: decompress() and
+    Retriever.retrieve() raise PrismError, a served retrieval of it fails
+    alone (its neighbours in the batch complete)."""
+    fields, pipeline, serve = mods
+    from prism_amd._lib import PrismError
+    announce("malformed streams: plain, progressive, served")
+    t, e = fields.batch_field(0, 960, device="cuda", **LIMITS)
+    plain = pipeline.compress(t, e, rel=True, progressive=False)
+    pipeline.decompress(plain)  # well formed: no error
+    with pytest.raises(PrismError, match="malformed"):
+        pipeline.decompress(_corrupted(pipeline, plain))
+    prog = pipeline.compress(t, e, rel=True, progressive=True)
+    with pytest.raises(PrismError, match="malformed"):
+        pipeline.Retriever(_corrupted(pipeline, prog)).retrieve(1e-2)
+    r = pipeline.Retriever(_corrupted(pipeline, prog))
+    r.retrieve(1e-2, check_stream=False)
+    assert r.error() != 0
+    good = resident_set(fields, pipeline, 1, 970)[0]
+    srv = serve.Server(budget_bytes=8 << 30)
+    srv.add_field(0, good)
+    srv.add_field(1, _corrupted(pipeline, prog))
+    reqs = [serve.Request(rid=i, kind="retrieve", arrival=0.0, field=i % 2, target=1e-3) for i in range(4)]
+    for q in reqs:
+        srv.submit(q)
+    srv.drain()
+    st = {q.rid: q.state for q in reqs}
+    assert st == {0: "done", 1: "failed", 2: "done", 3: "failed"}, st
+    assert "malformed" in reqs[1].error and reqs[1].result is None

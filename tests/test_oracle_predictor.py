@@ -208,3 +208,107 @@ def test_pick_flags_tie_rules(oracle):
     f = oracle.pick_flags(s)
     # level 3: e1<e0 -> reverse, e2<e1 -> md; level 1: best 11 -> md, reverse; level 0: best 14 -> md
     assert list(f) == [1, 1, 1, 1, 1, 1, 1, 1]
+
+
+# ---- closed-form pins of the tuner's sample geometry and of the rule
+# selection in partial tiles (derivations in tests/closed_forms.py) ----
+
+from closed_forms import (RULE_COEFS, RULE_SHAPES, TUNER_ORIGINS, TUNER_SHAPES, codes_by_point,  # noqa: E402
+                          cubic_axes_field, expected_codes, noise_outside_windows, tuner_blocks)
+
+
+def test_tuner_origin_table_is_the_reference_formula_by_hand():
+    """The hand-derived origin table: spacing 128, the middle origin at
+    d/2 (or d/2 - 128 k), the last window start at most d - 9."""
+    for d, orig in TUNER_ORIGINS.items():
+        assert all(b - a == 128 for a, b in zip(orig, orig[1:]))
+        assert d // 2 in orig and orig[0] >= 8 and orig[-1] + 8 <= d - 1
+        assert orig[0] - 128 < 8 and orig[-1] + 128 + 8 > d - 1
+    assert len(tuner_blocks((384, 384, 256))) == 9
+    assert len(tuner_blocks((512, 512, 512))) == 27
+
+
+@pytest.mark.parametrize("shape_xyz", TUNER_SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_tuner_reads_exactly_the_sample_windows(oracle, shape_xyz):
+    """Noise outside the hand-derived windows and zeros inside: all 15 scores
+    are exactly 0.  Then one impulse at the near and at the far corner of
+    each window, on a zero field: every score is positive.  The far corner
+    is local coordinate 16, or 8 where the window is cut by the field's end
+    (a multiple of 8 is a grid point of every level, so every candidate
+    sees it).  A window off by one point, an extra block or a missing block
+    fails one of the two."""
+    nx, ny, nz = shape_xyz
+    dtype = np.float32
+    scores, _, rc = oracle.autotune(noise_outside_windows(shape_xyz, dtype))
+    assert rc == 0 and not scores.any(), scores
+    f = np.zeros((nz, ny, nx), dtype)
+    for ox, oy, oz in tuner_blocks(shape_xyz):
+        far = [o + min(16, 8 * ((d - 1 - o) // 8)) for o, d in ((ox, nx), (oy, ny), (oz, nz))]
+        for px, py, pz in ((ox, oy, oz), tuple(far)):
+            f[pz, py, px] = 1.0
+            scores, _, rc = oracle.autotune(f)
+            f[pz, py, px] = 0.0
+            assert rc == 0 and (scores > 0).all(), ((px, py, pz), scores)
+    # just outside a window: one point before the first origin of x is read by no block
+    ox, oy, oz = tuner_blocks(shape_xyz)[0]
+    f[oz, oy, ox - 1] = 1.0
+    scores, _, _ = oracle.autotune(f)
+    assert not scores.any()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", RULE_SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_partial_tile_rule_selection(oracle, dtype, shape):
+    """Codes of f = cx x^3 + cy y^3 + cz z^3 at eb = 0.25 equal, point by
+    point, 2 * the interpolation remainder of the rule that the closed-form
+    availability test selects (cubic 0, quadratics -/+3u^3, linear -3gu^2,
+    copy 3g^2u - 3gu^2 + u^3), for the four flag settings and a per-level
+    mix; the reconstruction is exact."""
+    nz, ny, nx = shape
+    L = oracle.layout(nx, ny, nz)
+    for coef in RULE_COEFS:
+        data = cubic_axes_field(shape, coef, dtype)
+        for flags in ALL_FLAGS + [np.array([1, 0, 1, 0, 0, 1, 0, 1], np.uint8)]:
+            qc, an = oracle.spline_construct(data, 0.25, 0.0, flags)
+            got = codes_by_point(L, qc, shape)
+            want = expected_codes(shape, coef, flags)
+            bad = np.argwhere(got != want)
+            assert bad.size == 0, (coef, list(flags), bad[:4].tolist(),
+                                   [(int(got[tuple(b)]), int(want[tuple(b)])) for b in bad[:4]])
+            assert (want != 0).any()
+            rec = oracle.spline_reconstruct(qc, an, shape, 0.25, 0.0, flags)
+            assert np.array_equal(rec, data)
+
+
+def test_cubic_polynomial_field_has_zero_codes_where_the_cubic_rule_applies(oracle):
+    """A full trivariate polynomial of degree <= 3 in each variable (with
+    cross terms) is predicted exactly by every cubic stencil and by the
+    averages of cubic stencils, so every point whose used directions are all
+    of order 4 has code 0; only points near a tile edge or the field end
+    carry codes.  eb = 1/64 makes every error here (a multiple of 1/32)
+    quantise exactly, so the reconstruction equals the field."""
+    shape = (37, 35, 41)
+    nz, ny, nx = shape
+    z, y, x = np.meshgrid(*[np.arange(d, dtype=np.float64) for d in shape], indexing="ij")
+    f = x ** 3 - 2 * x * x * y + 3 * x * y * z - y ** 3 + 2 * z ** 3 - 5 * x * z * z + 7 * y + 11
+    L = oracle.layout(nx, ny, nz)
+    for flags in ALL_FLAGS:
+        qc, an = oracle.spline_construct(f, 1.0 / 64, 0.0, flags)
+        got = codes_by_point(L, qc, shape)
+        # points all of whose neighbours at distance 3u exist inside their tile
+        # on every axis: codes zero for every rule set
+        from closed_forms import point_level
+        lv = point_level(shape)
+        interior = np.zeros(shape, bool)
+        for k in range(4):
+            u = 1 << k
+            ok = lv == k
+            for g, n in ((x, nx), (y, ny), (z, nz)):
+                c = g.astype(int) % 16
+                odd = (g.astype(int) % (2 * u)) == u
+                ok &= ~odd | ((c >= 3 * u) & (c + 3 * u <= 16) & (g + 3 * u < n))
+            interior |= ok
+        assert not got[interior].any()
+        assert got[~interior & (lv < 4)].any()
+        rec = oracle.spline_reconstruct(qc, an, shape, 1.0 / 64, 0.0, flags)
+        assert np.array_equal(rec, f)
