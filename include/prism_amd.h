@@ -184,6 +184,20 @@ int pa_lossless_decode_window(const uint8_t* d_in, int64_t in_len, const pa_layo
                               const int32_t* d_end, uint8_t* d_bp, uint64_t* d_bytes_read,
                               int offsets_ready, void* d_ws, size_t ws_bytes, void* stream);
 
+/* Allocation-free pa_lossless_decode.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * scratch (error slot + chunk offsets, pa_decode_workspace(L) bytes) lives
+ * in a fixed device array of pa_noalloc_scratch_bytes() = 1 MiB per GPU,
+ * held until the process ends: fields of more than 131,064 chunks (host
+ * bit-plane bytes above about 2 GiB) are rejected with PA_ERR_WORKSPACE
+ * before any device work.  d_err (device int32) receives the decoder's
+ * error word (0: well-formed).  Every call on one GPU shares that GPU's
+ * scratch: calls that overlap on one GPU (other streams or threads) race. */
+size_t pa_noalloc_scratch_bytes(void);
+int pa_lossless_decode_noalloc(const uint8_t* d_in, int64_t in_len, const pa_layout* L, uint8_t* d_bp,
+                               int32_t* d_err, void* stream);
+
 /* findStrategy_h (dataloader.cu:179-208): updates d_begin/d_end in place and
  * writes d_maxlevel.  eb and target_error are absolute. */
 size_t pa_strategy_workspace(void);

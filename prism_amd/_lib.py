@@ -105,6 +105,8 @@ SIGNATURES = {
     "pa_decode_workspace": (SZ, [LP]),
     "pa_lossless_decode": (I32, [P, I64, LP, P, P, SZ, P]),
     "pa_lossless_decode_window": (I32, [P, I64, LP, P, P, P, P, I32, P, SZ, P]),
+    "pa_noalloc_scratch_bytes": (SZ, []),
+    "pa_lossless_decode_noalloc": (I32, [P, I64, LP, P, P, P]),
     "pa_strategy_workspace": (SZ, []),
     "pa_find_strategy": (I32, [P, P, P, P, D, D, P, SZ, P]),
     "pa_quality_workspace": (SZ, [I64]),

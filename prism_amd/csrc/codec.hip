@@ -14,6 +14,8 @@
 // that does not shrink below its size is stored raw (csize == osize).
 #include <rocprim/warp/warp_scan.hpp>
 
+#include <mutex>
+
 #include "batch.hpp"
 #include "common.hpp"
 
@@ -946,6 +948,54 @@ extern "C" int pa_lossless_decode_window(const uint8_t* d_in, int64_t in_len, co
     if (!d_begin || !d_end) return PA_ERR_ARG;
     return decode_common(d_in, in_len, L, d_begin, d_end, d_bp, d_bytes_read, true, offsets_ready != 0, d_ws, ws_bytes,
                          as_stream(stream));
+}
+
+// ------------------------------------------- allocation-free decode --
+// pa_lossless_decode with its workspace (error slot + chunk offsets) in a
+// file-scope device array instead of caller memory.  Host code never passes
+// `scratch` itself to a kernel: its device address, different on every GPU,
+// comes from hipGetSymbolAddress once per device.
+namespace {
+constexpr size_t kNoAllocScratch = (size_t)1 << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratch];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64] = {};
+
+// the device address of this GPU's scratch (nullptr on failure)
+void* scratch_on_current_device() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return nullptr;
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
+// copies the decoder's error word out of the scratch
+__global__ void publish_error_kernel(const int32_t* __restrict__ err, int32_t* __restrict__ out) {
+    if (threadIdx.x == 0) *out = *err;
+}
+}  // namespace
+
+extern "C" size_t pa_noalloc_scratch_bytes(void) { return kNoAllocScratch; }
+
+extern "C" int pa_lossless_decode_noalloc(const uint8_t* d_in, int64_t in_len, const pa_layout* L, uint8_t* d_bp,
+                                          int32_t* d_err, void* stream) {
+    if (!L || !d_in || !d_bp || !d_err) return PA_ERR_ARG;
+    if (pa_decode_workspace(L) > kNoAllocScratch) return PA_ERR_WORKSPACE;  // before any HIP call
+    void* ws = scratch_on_current_device();
+    if (!ws) return PA_ERR_HIP;
+    hipStream_t st = as_stream(stream);
+    const int rc = decode_common(d_in, in_len, L, nullptr, nullptr, d_bp, nullptr, false, false, ws,
+                                 kNoAllocScratch, st);
+    if (rc != PA_OK) return rc;
+    hipLaunchKernelGGL(publish_error_kernel, dim3(1), dim3(64), 0, st, static_cast<const int32_t*>(ws), d_err);
+    PA_LAUNCH_CHECK();
+    return PA_OK;
 }
 
 int64_t pa::batch_merge_blocks(const pa_layout* L, int dtype) {

@@ -56,6 +56,22 @@ def test_rejects_bad_arguments():
     assert prism_amd.lib().pa_bitshuffle(None, C.byref(L), 0, None, None) == _lib.PA_ERR_ARG
 
 
+def test_noalloc_decode_limit_is_checked_on_the_host():
+    """The allocation-free decode rejects a field whose scratch exceeds its
+    fixed array before any HIP call (so this runs without a GPU)."""
+    import ctypes as C
+    import prism_amd
+    from prism_amd import _lib
+    lib = prism_amd.lib()
+    assert lib.pa_noalloc_scratch_bytes() == 1 << 20
+    small, big = prism_amd.layout(384, 384, 256), prism_amd.layout(2048, 2048, 128)
+    assert lib.pa_decode_workspace(C.byref(small)) <= lib.pa_noalloc_scratch_bytes()
+    assert lib.pa_decode_workspace(C.byref(big)) > lib.pa_noalloc_scratch_bytes()
+    fake = C.c_void_p(256)  # never dereferenced: the call returns first
+    assert lib.pa_lossless_decode_noalloc(fake, 1 << 20, C.byref(big), fake, fake, None) == _lib.PA_ERR_WORKSPACE
+    assert lib.pa_lossless_decode_noalloc(None, 0, C.byref(small), fake, fake, None) == _lib.PA_ERR_ARG
+
+
 # ---- the C++ API library (libprism_cpp.so) and a caller that uses only the
 # reference's API names ----
 
