@@ -150,8 +150,12 @@ int pa_spline_delta_reconstruct_narrow(const void* d_qc, const int32_t* d_narrow
 /* lossless_encode (zbpc.cu:476-520, kernel zbpc_encode 155-263): writes
  * header, size table and payloads to d_out (capacity >= pa_compress_bound),
  * the 4x32 cumulative size table to d_cum and the unpadded stream length to
- * d_outsize.  Three passes: chunks encoded into staging slots, payload
- * offsets by a scan of the size table, a gather into the stream. */
+ * d_outsize.  One pass over the chunks: each finds its payload offset by a
+ * decoupled look-back over the chunks before it and writes its bytes
+ * straight into the stream.  Workspace (8-byte aligned): an int32 error
+ * word in an 8-byte slot (non-zero only if a chunk gave up waiting for the
+ * sizes before it, which in-order dispatch rules out), then one 8-byte
+ * status word per chunk. */
 size_t pa_encode_workspace(const pa_layout* L);
 int pa_lossless_encode(const uint8_t* d_bp, const pa_layout* L, uint8_t* d_out, uint64_t* d_cum,
                        uint64_t* d_outsize, void* d_ws, size_t ws_bytes, void* stream);
@@ -216,7 +220,7 @@ int pa_quality(const void* d_ori, const void* d_rec, int dtype, int64_t n, doubl
 /* ---- Batched compression (no reference counterpart) ----
  * Many fields of mixed shapes and dtypes compressed by one fixed sequence of
  * launches: one per stage (extrema, bound, tuner, code gaps, spline,
- * shuffle, encode, scan, gather, trailer; at most 15 kernels) whatever the
+ * shuffle, encoder status clear, encode, trailer; at most 14 kernels) whatever the
  * number of fields, optionally captured once in a HIP graph and replayed.
  * Each field's stream is byte-identical to the one the plain per-field
  * calls (pa_extrema .. pa_compress_merge) produce for the same input. */
@@ -255,11 +259,12 @@ int pa_batch_plan_launches(const pa_batch_plan* plan);
  * on the plan's own fields (reps timed launches each; synchronises
  * `stream`), keeps the fastest and re-captures the graph.  Tuned: extrema
  * load batches per workgroup {2, 4, 8, 16}, spline work order (contiguous
- * per XCD or launch order), gather chunks per wave {1, 2, 4}.  Streams are
+ * per XCD or launch order), encoder chunks per workgroup {2, 4, 8}.  Streams are
  * unchanged by any choice. */
 int pa_batch_plan_autotune(pa_batch_plan* plan, int reps, void* stream);
-/* params3 = {extrema batches, XCD order, gather run}; timings9 (may be
- * NULL) = the autotuner's ms per candidate in that order (4 + 2 + 3) */
+/* params3 = {extrema batches, XCD order, encoder chunks per workgroup};
+ * timings9 (may be NULL) = the autotuner's ms per candidate in that order
+ * (4 + 2 + 3) */
 int pa_batch_plan_params(const pa_batch_plan* plan, int32_t* params3, float* timings9);
 int pa_batch_plan_destroy(pa_batch_plan* plan);
 

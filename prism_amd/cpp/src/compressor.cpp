@@ -269,7 +269,14 @@ void Compressor<T, E>::compress_pipeline(context* config, StatBuffer<T>* input, 
     check_hip(hipMemcpyAsync(compressedSize_bp.data(), compressedSize_bp_d, sizeof(compressedSize_bp),
                              hipMemcpyDeviceToHost, S(stream)),
               "D2H");
+    int32_t enc_err = 0;
+    check_hip(hipMemcpyAsync(&enc_err, ws + ws_off[1], sizeof(enc_err), hipMemcpyDeviceToHost, S(stream)), "D2H");
     sync_flags_to_host(config, stream);
+    if (enc_err) {
+        std::fprintf(stderr, "[PRISM ERROR] lossless_encode: a chunk gave up waiting for its payload offset (%d)\n",
+                     enc_err);
+        std::exit(EXIT_FAILURE);
+    }
     compressed_lossless_size = (size_t)((sz[0] + 7) / 8 * 8);
     total_compressed_size = (size_t)sz[1];
     compressed_bp = (uint8_t*)compressed_data->d;

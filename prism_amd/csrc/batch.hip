@@ -21,7 +21,7 @@ struct pa_batch_plan {
     pa::BatchDev dev{};
     const int64_t* ext_tab[4] = {};  // extrema prefix per ext_batches candidate
     int64_t ext_n[4] = {};
-    float tuned_ms[9] = {};          // autotuner timings: extrema x4, spline order x2, gather run x3
+    float tuned_ms[9] = {};          // autotuner timings: extrema x4, spline order x2, encoder waves x3
     uint8_t* table = nullptr;  // descriptors and prefixes inside the workspace
     int launches = 0;
     hipGraphExec_t exec = nullptr;
@@ -36,9 +36,9 @@ size_t up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 // per-field scratch, in the order the plan lays it out
 struct FieldSizes {
-    size_t minmax, ebinfo, flags, scores, tune, qc, anchors, bp, offsets, staging, cum, outsize;
+    size_t minmax, ebinfo, flags, scores, tune, qc, anchors, bp, encws, cum, outsize;
     size_t total() const {
-        return minmax + ebinfo + flags + scores + tune + qc + anchors + bp + offsets + staging + cum + outsize;
+        return minmax + ebinfo + flags + scores + tune + qc + anchors + bp + encws + cum + outsize;
     }
 };
 
@@ -53,8 +53,7 @@ FieldSizes field_sizes(const BatchField& f) {
     z.qc = up((size_t)L.code_len * 4);
     z.anchors = up((size_t)L.n4 * f.dtype);
     z.bp = up((size_t)L.host_size);
-    z.offsets = up((size_t)L.nchunks * 8);
-    z.staging = up((size_t)L.nchunks * pa::kCS + 16);
+    z.encws = up(((size_t)L.nchunks + 1) * 8);
     z.cum = up(128 * sizeof(uint64_t));
     z.outsize = up(sizeof(uint64_t));
     return z;
@@ -109,7 +108,7 @@ int count_launches(const pa::BatchDev& b) {
            + 2 + (b.n_tune32 > 0) + (b.n_tune > b.n_tune32) + (b.n_gap > 0)       // bound, tuner, pick, gaps
            + (b.n_tile32 > 0) + (b.n_tile > b.n_tile32)                           // spline
            + (b.n_shuf > 0)                                                       // shuffle
-           + 2 + 2 * (b.n_chunk > 0);                                             // encode, scan, gather, trailer
+           + 2 + (b.n_chunk > 0);                                                 // clear, encode, trailer
 }
 
 // descriptors (with device pointers) of every field, in plan order; fails
@@ -147,10 +146,8 @@ int build_descriptors(const pa_batch_field* fields, int nf, const std::vector<in
         b += z.anchors;
         f.bp = b;
         b += z.bp;
-        f.offsets = reinterpret_cast<int64_t*>(b);
-        b += z.offsets;
-        f.staging = b;
-        b += z.staging;
+        f.encws = reinterpret_cast<uint64_t*>(b);
+        b += z.encws;
         f.cum = reinterpret_cast<uint64_t*>(b);
         b += z.cum;
         f.outsize = reinterpret_cast<uint64_t*>(b);
@@ -177,7 +174,7 @@ int upload(pa_batch_plan* p, hipStream_t st) {
                                         pa::batch_gap_blocks(f),
                                         pa::batch_tile_units(&f.L),
                                         pa::batch_shuf_blocks(&f.L),
-                                        f.L.nchunks,
+                                        (f.L.nchunks + 7) / 8 * 8,  // encoder launch slots: whole workgroups
                                         pa::batch_merge_blocks(&f.L, f.dtype)};
         for (int s = 0; s < kPrefixes; ++s) P[s][k + 1] = P[s][k] + cnt[s];
     }
@@ -192,7 +189,7 @@ int upload(pa_batch_plan* p, hipStream_t st) {
         p->ext_n[i] = P[i][nf];
     }
     d.ext_batches = pa::kExtChoices[2];  // defaults until the autotuner runs
-    d.gather_run = 1;
+    d.encode_waves = 4;
     d.xcd_order = 1;
     d.p_ext = p->ext_tab[2];
     d.n_ext = p->ext_n[2];
@@ -212,7 +209,7 @@ int upload(pa_batch_plan* p, hipStream_t st) {
     d.n_tile32 = P[6][p->nf32];
     const int64_t lim = 0x7fffffff;
     if (p->ext_n[0] > lim || d.n_tune > lim || d.n_gap > lim || d.n_tile > lim || d.n_shuf > lim ||
-        (d.n_chunk + 3) / 4 > lim || d.n_merge > lim)
+        d.n_chunk + d.nf > lim || d.n_merge > lim)
         return PA_ERR_ARG;
     if (hipMemcpyAsync(p->table, p->host.data(), (size_t)nf * sizeof(BatchField), hipMemcpyHostToDevice, st) !=
         hipSuccess)
@@ -356,11 +353,11 @@ extern "C" int pa_batch_plan_autotune(pa_batch_plan* p, int reps, void* stream) 
     d.xcd_order = p->tuned_ms[4] <= p->tuned_ms[5] ? 1 : 0;
     best = 0;
     for (int i = 0; i < 3; ++i) {
-        d.gather_run = pa::kGatherChoices[i];
-        p->tuned_ms[6 + i] = time(pa::batch_gather);
+        d.encode_waves = pa::kEncodeChoices[i];
+        p->tuned_ms[6 + i] = time(pa::batch_encode_chunks);
         if (p->tuned_ms[6 + i] < p->tuned_ms[6 + best]) best = i;
     }
-    d.gather_run = pa::kGatherChoices[best];
+    d.encode_waves = pa::kEncodeChoices[best];
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     if (status != PA_OK) return status;
@@ -376,7 +373,7 @@ extern "C" int pa_batch_plan_params(const pa_batch_plan* p, int32_t* params3, fl
     if (!p || !params3) return PA_ERR_ARG;
     params3[0] = p->dev.ext_batches;
     params3[1] = p->dev.xcd_order;
-    params3[2] = p->dev.gather_run;
+    params3[2] = p->dev.encode_waves;
     if (timings9)
         for (int i = 0; i < 9; ++i) timings9[i] = p->tuned_ms[i];
     return PA_OK;

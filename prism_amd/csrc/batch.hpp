@@ -18,7 +18,7 @@ struct BatchDev {
     const int64_t* p_gap;    // code-gap workgroups
     const int64_t* p_tile;   // spline work units (a tile; 4 tiles of a 2-D, 256 of a 1-D field)
     const int64_t* p_shuf;   // shuffle workgroups
-    const int64_t* p_chunk;  // codec chunks (one per wave64)
+    const int64_t* p_chunk;  // encoder launch slots (one per wave64): chunks rounded up to a multiple of 8
     const int64_t* p_merge;  // trailer workgroups
     // host copies of the totals, and of the f32 part where a kernel is
     // specialised by dtype
@@ -26,12 +26,12 @@ struct BatchDev {
     int64_t n_tune32, n_tile32;
     // launch parameters (the plan's autotuner may change them)
     int ext_batches;  // 2048-vector load batches per extrema workgroup (p_ext is built for it)
-    int gather_run;   // chunks per wave in the gather
+    int encode_waves; // chunks (waves) per encoder workgroup
     int xcd_order;    // 1: spline work units dealt to XCDs in contiguous ranges; 0: launch order
 };
 
 constexpr int kExtChoices[4] = {2, 4, 8, 16};  // ext_batches candidates
-constexpr int kGatherChoices[3] = {1, 2, 4};   // gather_run candidates
+constexpr int kEncodeChoices[3] = {2, 4, 8};   // encode_waves candidates
 
 // per-field work counts of each stage (host)
 int64_t batch_ext_blocks(const pa_layout* L, int dtype, int ext_batches);
@@ -64,8 +64,8 @@ int64_t retr_unshuf_blocks(const pa_layout* L);
 int batch_extrema(const BatchDev& b, hipStream_t st);  // extrema.hip
 int batch_predict(const BatchDev& b, hipStream_t st);  // predictor.hip: eb, tuner, code gaps, spline
 int batch_shuffle(const BatchDev& b, hipStream_t st);  // bitplane.hip
-int batch_encode(const BatchDev& b, hipStream_t st);   // codec.hip: encode, scan, gather, trailer
+int batch_encode(const BatchDev& b, hipStream_t st);   // codec.hip: status clear, encode, trailer
 int batch_spline(const BatchDev& b, hipStream_t st);   // the spline launches of batch_predict
-int batch_gather(const BatchDev& b, hipStream_t st);   // the gather launch of batch_encode
+int batch_encode_chunks(const BatchDev& b, hipStream_t st);  // the clear and encode launches of batch_encode
 
 }  // namespace pa

@@ -1,9 +1,10 @@
 // codec.hip -- lossless bit-plane stream on gfx950: zero elimination plus
 // three rounds of repetition elimination per 16 KiB chunk (one chunk per
-// wave64, encoded into a staging slot), chunk offsets by a scan of the size
-// table, a gather into the stream, the parallel decoder, and the trailer.
-// No pass takes an atomic on a shared address: on MI355X those serialise
-// across workgroups at about 12 ns each.
+// wave64), each chunk's payload offset by a decoupled look-back over the
+// chunks in dispatch order and its bytes written straight into the stream,
+// the parallel decoder, and the trailer.  No pass takes an atomic
+// read-modify-write on a shared address: on MI355X those serialise across
+// workgroups at about 12 ns each.
 //
 // Stream (zbpc.cu:155-263, RZE.hpp:10-167): u64 insize | u16 csize[C] |
 // payloads in chunk order | zero pad to 8 | trailer.  Chunk order is level
@@ -87,43 +88,212 @@ __device__ __forceinline__ uint32_t nz_bytes(uint32_t w) {
 // keep the ordering of LDS accesses of one wave across phases
 __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); __builtin_amdgcn_wave_barrier(); }
 
-constexpr int kWaves = 4;  // chunks per workgroup, one per wave64
-
-// LDS of one wave: the zero-elimination bitmap, one step's surviving bytes
-// and the repetition-elimination bytes with the footer.  The surviving bytes
-// go to the chunk's staging slot step by step, so a wave needs 5.4 KiB of
-// LDS (an encoded chunk held whole would need 16 KiB and cap the encoder at
-// two waves per SIMD).
+// LDS of one wave: the zero-elimination bitmap, one step's bytes and the
+// repetition-elimination bytes with the footer.  The surviving bytes go to
+// the stream step by step, so a wave needs 5.5 KiB of LDS (an encoded chunk
+// held whole would need 16 KiB and cap the encoder at two waves per SIMD).
 constexpr int kTailBytes = 2048 + 256 + 32 + 4 + 2;  // RE1 + RE2 + RE3 kept bytes, B3, size
 struct __align__(16) WaveSlot {
-    uint16_t b0[kCS / 16];           // zero-elimination bitmap: 2 bytes per 16-byte segment
-    uint8_t step[1024 + 16];         // carried bytes (< 4) + surviving bytes of one 1 KiB step
-    uint8_t tail[(kTailBytes + 8 + 15) / 16 * 16];  // carried bytes + RE bytes and footer
-    int zs[16];                      // surviving bytes per step
+    uint16_t b0[kCS / 16];  // zero-elimination bitmap: 2 bytes per 16-byte segment
+    // a step's bytes: the zero-eliminated form keeps the (< 4) bytes carried
+    // from the previous step's last partial word in front of its surviving
+    // bytes; a raw chunk puts the previous step's last word at 12..16 and
+    // the step's 1 KiB from byte 16 (plus one word of slack)
+    uint8_t step[16 + 1024 + 16];
+    // the RE bytes, B3 and footer from byte 4, after up to 4 bytes that
+    // carry the last partial word of the zero-eliminated bytes (plus slack
+    // for the funnel reads of the last word)
+    uint8_t tail[(4 + kTailBytes + 8 + 15) / 16 * 16];
+    int zs[16];  // surviving bytes per step
 };
 static_assert(sizeof(WaveSlot) % 16 == 0, "slots must keep 16-byte alignment");
 
-// n words of LDS to global memory, consecutive lanes on consecutive words
-__device__ __forceinline__ void wave_copy_words(const uint8_t* src, uint8_t* dst, int n, int lane) {
+// Writes the 4-byte-aligned global words dst[0..nw) from the LDS byte
+// sequence at src + d0 + 4i (any byte offset: funnel-shifted from two
+// aligned LDS words), but only the bytes whose offset from dst lies in
+// [lo, hi): a word shared with a neighbouring chunk is written byte by
+// byte, so the two chunks' waves never overwrite each other's bytes.
+__device__ __forceinline__ void store_funnel(const uint8_t* src, int d0, uint32_t* dst, int nw, int lo, int hi,
+                                             int lane) {
     const uint32_t* s = reinterpret_cast<const uint32_t*>(src);
-    uint32_t* d = reinterpret_cast<uint32_t*>(dst);
-    for (int i = lane; i < n; i += 64) d[i] = s[i];
+    for (int i = lane; i < nw; i += 64) {
+        const int d = d0 + 4 * i;
+        const uint32_t v = __builtin_amdgcn_alignbyte(s[(d >> 2) + 1], s[d >> 2], d & 3);
+        const int x = 4 * i;
+        if (x >= lo && x + 4 <= hi) {
+            dst[i] = v;
+        } else {
+            uint8_t* a = reinterpret_cast<uint8_t*>(dst + i);
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                if (x + b >= lo && x + b < hi) a[b] = (uint8_t)(v >> (8 * b));
+        }
+    }
 }
 
-// One chunk per wave64.  Zero elimination streams the chunk in 16 steps of
-// 1 KiB (lane l owns 16-byte segment 64k+l, so every load is coalesced);
-// the three repetition-elimination rounds work on registers and shuffles.
-__device__ __forceinline__ void encode_chunk(const uint8_t* __restrict__ bp, const ChunkGeom& g,
-                                             uint8_t* __restrict__ dst_stream, uint8_t* __restrict__ staging,
-                                             int64_t id, WaveSlot& W) {
-    const int lane = threadIdx.x & 63;
-    const ChunkPos cp = chunk_pos(g, id);
-    const int osize = cp.osize;
-    const uint8_t* src = bp + cp.base;
-    uint8_t* dst = staging + id * kCS;
+// ---- decoupled look-back over the encoder's workgroups in index order ----
+// A workgroup encodes kW consecutive chunks (one per wave) and publishes one
+// status word: flag in bits 62..63 (0: not yet published, 1: the compressed
+// size of its chunks, 2: the inclusive payload offset after them) and the
+// value in bits 0..61.  Workgroups are dispatched in index order, so every
+// workgroup one waits for has been dispatched before it: the lowest
+// unfinished workgroup never waits.  The wait is bounded all the same: a
+// wave that polls kSpinLimit times records an error, takes the missing
+// sizes as 0 (its offset can then only be too small, never past the
+// stream's end) and goes on.
+constexpr uint64_t kStAgg = 1ull << 62, kStInc = 2ull << 62, kStVal = (1ull << 62) - 1;
+constexpr int kSpinLimit = 1 << 20;
 
-    // zero elimination; the whole chunk is requested before the first step
-    // (16 KiB per wave, 64 registers per lane) so it costs one memory latency
+__device__ __forceinline__ void status_store(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t status_load(const uint64_t* p) {
+    return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Publishes the size `agg` of workgroup w's chunks and returns its exclusive
+// payload offset (zbpc.cu:113-153), run by one wave.  A poll reads a window
+// of 64 x kLbPer status words (lane l: distances kLbPer*l .. kLbPer*l +
+// kLbPer - 1 below `top`); the sizes up to the nearest published offset are
+// summed, and a window with no published offset adds all its sizes and the
+// next poll reads the window below it.  Status words bypass the caches, so
+// a poll that finds a size still missing backs off before it reads again:
+// the polls of thousands of waiting waves would otherwise crowd out the
+// chunk loads.
+constexpr int kLbPer = 1;
+
+__device__ __forceinline__ int64_t look_back(uint64_t* status, int64_t w, int64_t agg, int32_t* err) {
+    const int lane = threadIdx.x & 63;
+    if (w == 0) {
+        if (lane == 0) status_store(status, kStInc | (uint64_t)agg);
+        return 0;
+    }
+    if (lane == 0) status_store(status + w, kStAgg | (uint64_t)agg);
+    int64_t excl = 0;
+    int64_t top = w - 1;
+    int polls = 0, backoff = 1;
+    for (;;) {
+        uint64_t s[kLbPer];
+#pragma unroll
+        for (int t = 0; t < kLbPer; ++t) {
+            const int64_t j = top - kLbPer * lane - t;
+            s[t] = j >= 0 ? status_load(status + j) : kStInc;  // before workgroup 0: offset 0
+        }
+        // this lane's entries nearest first: sizes before its first offset
+        int first_t = kLbPer, lsum = 0;
+        bool lmiss = false;
+#pragma unroll
+        for (int t = kLbPer - 1; t >= 0; --t) {
+            const uint32_t fl = (uint32_t)(s[t] >> 62);
+            if (fl == 2) {
+                first_t = t;
+                lsum = 0;
+                lmiss = false;
+            } else {
+                lsum += fl == 1 ? (int)(s[t] & kStVal) : 0;
+                lmiss = lmiss || fl == 0;
+            }
+        }
+        const uint64_t inc = __ballot(first_t < kLbPer);
+        const int first = inc ? (int)__builtin_ctzll(inc) : 64;  // lanes 0..first make up the sum
+        const uint64_t need = first >= 63 ? ~0ull : ((2ull << first) - 1);
+        const bool missing = (__ballot(lmiss) & need) != 0;
+        if (missing && ++polls < kSpinLimit) {
+            for (int b = 0; b < backoff; ++b) __builtin_amdgcn_s_sleep(4);
+            backoff = backoff < 16 ? 2 * backoff : 16;
+            continue;
+        }
+        if (missing && lane == 0) atomicOr(err, 1);
+        // sizes (each below 2^18, at most 64 kLbPer of them: an int sum),
+        // then the offset that lane `first` holds
+        const int sum = __builtin_amdgcn_readlane(wave_incl_scan(lane <= first ? lsum : 0), 63);
+        excl += sum;
+        if (first < 64) {
+            uint64_t v = 0;
+#pragma unroll
+            for (int t = 0; t < kLbPer; ++t) v = t == first_t ? (s[t] & kStVal) : v;
+            const uint32_t vlo = __builtin_amdgcn_readlane((uint32_t)v, first);
+            const uint32_t vhi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), first);
+            excl += (int64_t)(((uint64_t)vhi << 32) | vlo);
+            break;
+        }
+        top -= 64 * kLbPer;
+    }
+    if (lane == 0) status_store(status + w, kStInc | (uint64_t)(excl + agg));
+    return excl;
+}
+
+// Repetition-elimination flags (RZE.hpp:10-86).  Round 1: lane l owns
+// bytes [32l, 32l+32) of the zero-elimination bitmap B0; bit i of the
+// result is set where B0 byte 32l+i differs from the byte before it (byte
+// -1 = 0).
+__device__ __forceinline__ uint32_t re1_flags(const uint16_t* b0, int lane) {
+    const uint4* b0v = reinterpret_cast<const uint4*>(b0) + 2 * lane;
+    const uint4 q0 = b0v[0], q1 = b0v[1];
+    const uint32_t v[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+    uint32_t prev = __shfl_up(v[7] >> 24, 1, 64);
+    if (lane == 0) prev = 0;
+    uint32_t f1 = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const uint32_t shifted = (v[j] << 8) | (j == 0 ? prev : (v[j - 1] >> 24));
+        f1 |= nz_bytes(v[j] ^ shifted) << (4 * j);
+    }
+    return f1;
+}
+// round 2: B1 = the round-1 flags as bytes (lane l holds bytes [4l, 4l+4))
+__device__ __forceinline__ uint32_t re2_flags(uint32_t f1, int lane) {
+    uint32_t p2 = __shfl_up(f1 >> 24, 1, 64);
+    if (lane == 0) p2 = 0;
+    return nz_bytes(f1 ^ ((f1 << 8) | p2));
+}
+// round 3: B2 byte m = the round-2 nibbles of lanes 2m, 2m+1 (m < 32)
+__device__ __forceinline__ uint32_t re2_byte(uint32_t f2, int lane) {
+    return (__shfl(f2, (2 * lane) & 63, 64) | (__shfl(f2, (2 * lane + 1) & 63, 64) << 4)) & 0xFFu;
+}
+__device__ __forceinline__ bool re3_keep(uint32_t f2, int lane) {
+    const uint32_t b2 = re2_byte(f2, lane);
+    uint32_t p3 = __shfl_up(b2, 1, 64);
+    if (lane == 0) p3 = 0;
+    return lane < 32 && b2 != p3;
+}
+
+// Where an encoder writes: the stream, its cumulative size table and
+// length, the look-back status words and the error word.
+struct EncodeOut {
+    uint8_t* stream;
+    uint64_t* cum;
+    uint64_t* outsize;
+    uint64_t* status;
+    int32_t* err;
+};
+
+// a workgroup's chunk sizes and payload offset
+struct GroupLds {
+    int csize[8];
+    int64_t off;
+};
+
+// One chunk per wave64, kW chunks per workgroup (chunks id0 .. id0 + kW - 1,
+// workgroup w of the stream), in one pass: zero elimination (16 steps of
+// 1 KiB; lane l owns 16-byte segment 64k+l, so every load is coalesced),
+// the three repetition-elimination rounds on registers and shuffles, the
+// compressed sizes, the workgroup's look-back for its payload offset, then
+// the bytes straight to their place in the stream (zbpc.cu:155-263).  Every
+// wave of the workgroup calls it (a wave past the last chunk has size 0).
+template <int kW>
+__device__ __forceinline__ void encode_chunk(const uint8_t* __restrict__ bp, const ChunkGeom& g, const EncodeOut& o,
+                                             int64_t id0, int64_t w, WaveSlot& W, GroupLds& G) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t id = id0 + wave;
+    const bool valid = id < g.nchunks;
+    const ChunkPos cp = chunk_pos(g, valid ? id : 0);
+    const int osize = valid ? cp.osize : 0;
+    const uint8_t* src = bp + cp.base;
+
+    // the whole chunk is requested before the first step (16 KiB per wave,
+    // 64 registers per lane) so it costs one memory latency; it stays in
+    // registers until its bytes are written
     constexpr int kSteps = kCS / 1024;
     uint2 lo8[kSteps], hi8[kSteps];
 #pragma unroll
@@ -136,8 +306,7 @@ __device__ __forceinline__ void encode_chunk(const uint8_t* __restrict__ bp, con
             if (b + 8 < osize) hi8[k] = *reinterpret_cast<const uint2*>(src + b + 8);
         }
     }
-    // bitmaps first: the total decides whether any byte is written (a chunk
-    // whose surviving bytes fill it is stored raw)
+    // zero-elimination bitmaps and the surviving-byte count
     int Z = 0;
 #pragma unroll
     for (int k = 0; k < kSteps; ++k) {
@@ -150,22 +319,90 @@ __device__ __forceinline__ void encode_chunk(const uint8_t* __restrict__ bp, con
     }
     bool fail = Z >= kAvail;
     wave_sync();
-    if (!fail) {
-        // the surviving bytes, step by step, from the registers the bitmap
-        // pass filled (the loop is unrolled so they stay in registers).  Read
-        // again instead, they missed L2 -- 16 KiB per wave in flight is more
-        // than an XCD's L2 holds -- and the chunk crossed the fabric twice.
-        // The staging slot is written in whole 4-byte words, each once: a
-        // step's bytes follow the r < 4 bytes carried from the previous
-        // step's last partial word, and its own last partial word is carried
-        // on (to the tail after the last step).
-        int zpos = 0;
+    // repetition elimination, counts only: the flags of rounds 1 and 2 stay
+    // in two registers, the kept bytes are placed after the zero-eliminated
+    // bytes have left (the chunk's registers are free by then)
+    int wpos = Z;  // chunk position
+    uint32_t f1 = 0, f2 = 0, b3 = 0;
+    if (Z != 0 && !fail) {  // (a chunk that already failed is raw whatever follows)
+        // round 1: B0 (2048 bytes) -> B1 (256 bytes); lane l owns B0 bytes [32l, 32l+32)
+        f1 = re1_flags(W.b0, lane);
+        wpos += __builtin_amdgcn_readlane(wave_incl_scan(__popc(f1)), 63);
+        fail = fail || wpos > kAvail;
+        // round 2: B1 (256 bytes, lane l holds bytes [4l, 4l+4) = f1) -> B2 (32 bytes)
+        f2 = re2_flags(f1, lane);
+        wpos += __builtin_amdgcn_readlane(wave_incl_scan(__popc(f2)), 63);
+        fail = fail || wpos > kAvail;
+        // round 3: B2 byte m = nibbles of lanes 2m, 2m+1 -> B3 (4 bytes)
+        b3 = (uint32_t)__ballot(re3_keep(f2, lane));
+        wpos += __popc(b3);
+        fail = fail || wpos > kAvail || wpos >= kAvail - 4;
+    }
+    int csize = Z == 0 ? 2 : (fail ? osize : wpos + 6);
+    const bool raw = fail || csize >= osize;
+    if (raw) csize = osize;
+    if (!valid) csize = 0;
+
+    // the workgroup's payload offset, then the chunk's, the size table, the
+    // per-plane cumulative sizes, the stream header and length
+    if (lane == 0) G.csize[wave] = csize;
+    __syncthreads();
+    if (wave == 0) {
+        int64_t agg = 0;
+#pragma unroll
+        for (int k = 0; k < kW; ++k) agg += G.csize[k];
+        const int64_t goff = look_back(o.status, w, agg, o.err);
+        if (lane == 0) G.off = goff;
+    }
+    __syncthreads();
+    if (!valid) return;
+    int64_t off = G.off;
+#pragma unroll
+    for (int k = 0; k < kW; ++k) off += k < wave ? G.csize[k] : 0;
+    if (lane == 0) {
+        reinterpret_cast<uint16_t*>(o.stream + 8)[id] = (uint16_t)csize;
+        if (cp.last_of_plane) o.cum[(3 - cp.lv) * 32 + cp.plane] = (uint64_t)(off + csize);
+        if (id == g.nchunks - 1) *o.outsize = (uint64_t)(8 + 2 * g.nchunks + off + csize);
+        if (id == 0) *reinterpret_cast<uint64_t*>(o.stream) = (uint64_t)g.host_size;
+    }
+    uint8_t* P = o.stream + 8 + 2 * g.nchunks + off;  // first byte of the payload
+
+    if (raw) {
+        // step k: the previous step's last word at step[12..16), the step's
+        // 1 KiB from step[16]; global word i of the step starts at chunk
+        // position 1024k + 4i - q (q = misalignment of P), LDS byte
+        // 16 + 4i - q
+        const int q = (int)((uintptr_t)P & 3);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(P - q);
+#pragma unroll
+        for (int k = 0; k < kSteps; ++k) {
+            if (1024 * k >= osize) break;  // wave-uniform
+            const uint32_t prev_last = k > 0 ? __shfl(hi8[k - 1].y, 63, 64) : 0u;
+            reinterpret_cast<uint4*>(W.step + 16)[lane] = make_uint4(lo8[k].x, lo8[k].y, hi8[k].x, hi8[k].y);
+            if (lane == 0) reinterpret_cast<uint32_t*>(W.step)[3] = prev_last;
+            wave_sync();
+            const int rest = osize - 1024 * k;  // chunk bytes from this step on
+            // words whose last byte lies in this step (the final word of the
+            // chunk, when it ends past a word boundary, goes with the last step)
+            const int nw = rest > 1024 ? 256 : (rest + q + 3) >> 2;
+            store_funnel(W.step, 16 - q, dst + 256 * k, nw, q - 1024 * k, q + osize - 1024 * k, lane);
+            wave_sync();  // the next step reuses the buffer
+        }
+        return;
+    }
+    // zero-eliminated bytes, step by step from the registers the bitmap
+    // pass filled (the loop is unrolled so they stay in registers): a step's
+    // bytes follow the r < 4 bytes carried from the previous step's last
+    // partial word (r = alignment of the step's first byte in the stream),
+    // whole words go out, the last partial word is carried on
+    int zpos = 0;
+    if (Z != 0) {
 #pragma unroll
         for (int k = 0; k < kSteps; ++k) {
             const uint32_t w[4] = {lo8[k].x, lo8[k].y, hi8[k].x, hi8[k].y};
             const int zs = W.zs[k];
             if (zs == 0) continue;  // wave-uniform
-            const int r = zpos & 3;
+            const int r = (int)((uintptr_t)(P + zpos) & 3);
             const uint32_t m = W.b0[64 * k + lane];
             const int cnt = __popc(m);
             int pos = r + wave_incl_scan(cnt) - cnt;
@@ -174,117 +411,102 @@ __device__ __forceinline__ void encode_chunk(const uint8_t* __restrict__ bp, con
                 if ((m >> i) & 1u) W.step[pos++] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
             wave_sync();
             const int nfull = (r + zs) >> 2;
-            wave_copy_words(W.step, dst + zpos - r, nfull, lane);
-            wave_sync();  // the copy has read the buffer
+            store_funnel(W.step, 0, reinterpret_cast<uint32_t*>(P + zpos - r), nfull, r - zpos, r + csize - zpos, lane);
+            wave_sync();  // the stores have read the buffer
             if (lane == 0) reinterpret_cast<uint32_t*>(W.step)[0] = reinterpret_cast<const uint32_t*>(W.step)[nfull];
             wave_sync();  // the next step reuses the buffer
             zpos += zs;
         }
     }
-    // RE bytes go to tail[zr + wpos - Z], after the zr carried bytes of the
-    // last partial word of the zero-eliminated bytes
-    const int zr = Z & 3;
-    if (lane == 0) reinterpret_cast<uint32_t*>(W.tail)[0] = reinterpret_cast<const uint32_t*>(W.step)[0];
-    int wpos = Z;  // stream position
-    uint32_t b3 = 0;
-    wave_sync();
-    if (Z != 0 && !fail) {  // (a chunk that already failed is raw whatever follows)
-        // round 1: B0 (2048 bytes) -> B1 (256 bytes); lane l owns B0 bytes [32l, 32l+32)
-        uint32_t v[8];
-        const uint4* b0v = reinterpret_cast<const uint4*>(W.b0) + 2 * lane;
-        const uint4 q0 = b0v[0], q1 = b0v[1];
-        v[0] = q0.x; v[1] = q0.y; v[2] = q0.z; v[3] = q0.w;
-        v[4] = q1.x; v[5] = q1.y; v[6] = q1.z; v[7] = q1.w;
-        uint32_t prev = __shfl_up(v[7] >> 24, 1, 64);
-        if (lane == 0) prev = 0;
-        uint32_t f1 = 0;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const uint32_t shifted = (v[j] << 8) | (j == 0 ? prev : (v[j - 1] >> 24));
-            f1 |= nz_bytes(v[j] ^ shifted) << (4 * j);
-        }
-        int cnt = __popc(f1);
-        int incl = wave_incl_scan(cnt);
-        int R = __builtin_amdgcn_readlane(incl, 63);
-        fail = fail || wpos + R > kAvail;
-        if (!fail) {
-            int pos = zr + wpos - Z + incl - cnt;
+    // the RE bytes at tail[4 + (chunk position - Z)], B3 and the footer,
+    // after the carried bytes (stream positions Z - rt .. Z - 1); then every
+    // word from the aligned P + Z - rt to the end
+    const int rt = (int)((uintptr_t)(P + Z) & 3);
+    if (lane < rt) W.tail[4 - rt + lane] = W.step[lane];
+    int pos = 4;
+    if (Z != 0) {
+        {  // round 1: B0 byte 32 lane + i is kept where bit i of f1 is set
+            const int cnt = __popc(f1);
+            int p = pos + wave_incl_scan(cnt) - cnt;
+            const uint4* b0v = reinterpret_cast<const uint4*>(W.b0) + 2 * lane;
+            const uint4 q0 = b0v[0], q1 = b0v[1];
+            const uint32_t v[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
 #pragma unroll
             for (int i = 0; i < 32; ++i)
-                if ((f1 >> i) & 1u) W.tail[pos++] = (uint8_t)(v[i >> 2] >> (8 * (i & 3)));
+                if ((f1 >> i) & 1u) W.tail[p++] = (uint8_t)(v[i >> 2] >> (8 * (i & 3)));
+            pos += __builtin_amdgcn_readlane(wave_incl_scan(cnt), 63);
         }
-        wpos += R;
-        // round 2: B1 (256 bytes, lane l holds bytes [4l, 4l+4) = f1) -> B2 (32 bytes)
-        uint32_t p2 = __shfl_up(f1 >> 24, 1, 64);
-        if (lane == 0) p2 = 0;
-        const uint32_t f2 = nz_bytes(f1 ^ ((f1 << 8) | p2));
-        cnt = __popc(f2);
-        incl = wave_incl_scan(cnt);
-        R = __builtin_amdgcn_readlane(incl, 63);
-        fail = fail || wpos + R > kAvail;
-        if (!fail) {
-            int pos = zr + wpos - Z + incl - cnt;
+        {  // round 2: B1 byte 4 lane + i (byte i of f1) where bit i of f2 is set
+            const int cnt = __popc(f2);
+            int p = pos + wave_incl_scan(cnt) - cnt;
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-                if ((f2 >> i) & 1u) W.tail[pos++] = (uint8_t)(f1 >> (8 * i));
+                if ((f2 >> i) & 1u) W.tail[p++] = (uint8_t)(f1 >> (8 * i));
+            pos += __builtin_amdgcn_readlane(wave_incl_scan(cnt), 63);
         }
-        wpos += R;
-        // round 3: B2 byte m = nibbles of lanes 2m, 2m+1 -> B3 (4 bytes)
-        const uint32_t b2 = (__shfl(f2, (2 * lane) & 63, 64) | (__shfl(f2, (2 * lane + 1) & 63, 64) << 4)) & 0xFFu;
-        uint32_t p3 = __shfl_up(b2, 1, 64);
-        if (lane == 0) p3 = 0;
-        const bool keep = lane < 32 && b2 != p3;
-        const uint64_t m3 = __ballot(keep);
-        b3 = (uint32_t)m3;
-        R = __popcll(m3);
-        fail = fail || wpos + R > kAvail;
-        if (!fail && keep) W.tail[zr + wpos - Z + __popcll(m3 & ((1ull << lane) - 1))] = (uint8_t)b2;
-        wpos += R;
-        fail = fail || wpos >= kAvail - 4;
-    }
-    int csize = Z == 0 ? 2 : (fail ? osize : wpos + 6);
-    const bool raw = fail || csize >= osize;
-    if (raw) csize = osize;
-    if (!raw && lane == 0) {
-        if (Z != 0) {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) W.tail[zr + wpos - Z + j] = (uint8_t)(b3 >> (8 * j));
+        {  // round 3: the B2 bytes where B3 is set
+            const uint32_t b2 = re2_byte(f2, lane);
+            if (lane < 32 && ((b3 >> lane) & 1u)) W.tail[pos + __popc(b3 & ((1u << lane) - 1u))] = (uint8_t)b2;
+            pos += __popc(b3);
         }
-        W.tail[zr + csize - 2 - Z] = (uint8_t)(osize & 0xFF);
-        W.tail[zr + csize - 1 - Z] = (uint8_t)((osize >> 8) & 0xFF);
+        if (lane < 4) W.tail[pos + lane] = (uint8_t)(b3 >> (8 * lane));
+        pos += 4;
     }
-    // RE bytes and footer after the zero-eliminated bytes in the staging
-    // slot (the last word may run up to 3 bytes past csize < kCS, inside the
-    // slot); a raw chunk is copied later straight from the bit-planes
+    if (lane == 0) {
+        W.tail[pos] = (uint8_t)(osize & 0xFF);
+        W.tail[pos + 1] = (uint8_t)((osize >> 8) & 0xFF);
+    }
     wave_sync();
-    if (!raw) wave_copy_words(W.tail, dst + Z - zr, (zr + csize - Z + 3) >> 2, lane);
-    if (lane == 0) reinterpret_cast<uint16_t*>(dst_stream + 8)[id] = (uint16_t)csize;
+    store_funnel(W.tail, 4 - rt, reinterpret_cast<uint32_t*>(P + Z - rt), (rt + csize - Z + 3) >> 2, rt - Z,
+                 rt + csize - Z, lane);
 }
 
-__global__ __launch_bounds__(64 * kWaves) void encode_kernel(const uint8_t* __restrict__ bp, ChunkGeom g,
-                                                             uint8_t* __restrict__ dst_stream,
-                                                             uint8_t* __restrict__ staging) {
-    __shared__ WaveSlot slots[kWaves];
-    const int64_t id = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
-    if (id >= g.nchunks) return;
-    encode_chunk(bp, g, dst_stream, staging, id, slots[threadIdx.x >> 6]);
+// one chunk per wave, kW chunks per workgroup
+template <int kW>
+__global__ __launch_bounds__(64 * kW) void encode_kernel(const uint8_t* __restrict__ bp, ChunkGeom g, EncodeOut o) {
+    __shared__ WaveSlot slots[kW];
+    __shared__ GroupLds G;
+    encode_chunk<kW>(bp, g, o, (int64_t)blockIdx.x * kW, blockIdx.x, slots[threadIdx.x >> 6], G);
 }
 
-// one launch over the chunks of every field of a batch (one chunk per wave)
-__global__ __launch_bounds__(64 * kWaves) void batch_encode_kernel(const BatchField* __restrict__ fields,
-                                                                   const int64_t* __restrict__ prefix, int nf,
-                                                                   int64_t total) {
-    __shared__ WaveSlot slots[kWaves];
-    const int64_t gid = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
-    if (gid >= total) return;
-    const int f = find_field(prefix, 0, nf, gid);
+// one launch over the chunks of every field of a batch (one chunk per
+// wave); each field's chunks start at a multiple of 8 in the launch (its
+// prefix entry), so a workgroup never spans two fields, and they are
+// dispatched in id order
+template <int kW>
+__global__ __launch_bounds__(64 * kW) void batch_encode_kernel(const BatchField* __restrict__ fields,
+                                                               const int64_t* __restrict__ prefix, int nf) {
+    __shared__ WaveSlot slots[kW];
+    __shared__ GroupLds G;
+    const int64_t gid0 = (int64_t)blockIdx.x * kW;
+    const int f = find_field(prefix, 0, nf, gid0);
     const BatchField& F = fields[f];
-    encode_chunk(F.bp, F.cg, F.out, F.staging, gid - prefix[f], slots[threadIdx.x >> 6]);
+    const EncodeOut o{F.out, F.cum, F.outsize, F.encws + 1, reinterpret_cast<int32_t*>(F.encws)};
+    const int64_t id0 = gid0 - prefix[f];
+    encode_chunk<kW>(F.bp, F.cg, o, id0, id0 / kW, slots[threadIdx.x >> 6], G);
 }
 
-// Payload offsets by one workgroup scan of the size table, then the stream
-// header, the stream length and the cumulative size table by (level, plane)
-// (zbpc.cu:176-181, 231-262).
+// clears an encoder workspace: the error word, then the n status words
+__global__ void encode_clear_kernel(uint64_t* __restrict__ ws, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= n) ws[i] = 0ull;
+}
+
+// the same for every field of a batch (one thread per launch slot of a
+// chunk, plus one per field for its error word)
+__global__ void batch_encode_clear_kernel(const BatchField* __restrict__ fields, const int64_t* __restrict__ prefix,
+                                          int nf, int64_t total) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total + nf) return;
+    const int64_t gid = i < total ? i : 0;
+    const int f = i < total ? find_field(prefix, 0, nf, gid) : (int)(i - total);
+    const BatchField& F = fields[f];
+    const int64_t k = i < total ? gid - prefix[f] : -1;  // status word k (-1: the error word)
+    if (k < F.cg.nchunks) F.encws[1 + k] = 0ull;
+}
+
+// The decoder's payload offsets by one workgroup scan of the size table
+// (zbpc.cu:265-365 sums the sizes before each chunk).
 // One workgroup of 1024 lanes: lane t takes the run of `per` chunks from
 // t * per.  Up to kScanLds sizes are first staged through LDS with
 // coalesced loads, so the runs read LDS rather than scattered 2-byte words.
@@ -349,109 +571,6 @@ __device__ __forceinline__ const uint16_t* scan_runs(const uint16_t* __restrict_
     *total = tot;
     __syncthreads();
     return sz;
-}
-
-// inclusive offset of chunk id (after it), from the run bases
-__device__ __forceinline__ int64_t scan_inclusive(const uint16_t* sz, int64_t n, const ScanLds& S, int64_t id) {
-    const int64_t per = (n + 1023) / 1024;
-    const int64_t t = id / per;
-    int64_t r = S.base[t];
-    for (int64_t i = t * per; i <= id; ++i) r += sz[i];
-    return r;
-}
-
-__device__ __forceinline__ void stream_scan(const uint16_t* __restrict__ sizes, const ChunkGeom& g,
-                                            int64_t* __restrict__ offsets, uint8_t* __restrict__ dst_stream,
-                                            uint64_t* __restrict__ cum, uint64_t* __restrict__ outsize) {
-    __shared__ ScanLds S;
-    const int64_t n = g.nchunks;
-    int64_t total;
-    const uint16_t* sz = scan_runs(sizes, n, S, offsets, &total);
-    // cum[lr][plane]: the payload end of the plane's last chunk (every level
-    // has 32 planes of cb >= 1 chunks)
-    if (threadIdx.x < 128) {
-        const int lr = threadIdx.x >> 5, plane = threadIdx.x & 31;
-        const int64_t id = g.chunk_start[lr] + (int64_t)(plane + 1) * g.cb[3 - lr] - 1;
-        cum[lr * 32 + plane] = (uint64_t)scan_inclusive(sz, n, S, id);
-    }
-    if (threadIdx.x == 0) {
-        *reinterpret_cast<uint64_t*>(dst_stream) = (uint64_t)g.host_size;
-        *outsize = (uint64_t)(8 + 2 * n + total);
-    }
-}
-
-__global__ __launch_bounds__(1024) void stream_scan_kernel(const uint16_t* __restrict__ sizes, ChunkGeom g,
-                                                           int64_t* __restrict__ offsets, uint8_t* __restrict__ dst_stream,
-                                                           uint64_t* __restrict__ cum, uint64_t* __restrict__ outsize) {
-    stream_scan(sizes, g, offsets, dst_stream, cum, outsize);
-}
-
-// one workgroup per field of a batch
-__global__ __launch_bounds__(1024) void batch_scan_kernel(const BatchField* __restrict__ fields) {
-    const BatchField& F = fields[blockIdx.x];
-    stream_scan(reinterpret_cast<const uint16_t*>(F.out + 8), F.cg, F.offsets, F.out, F.cum, F.outsize);
-}
-
-// One chunk per wave64: csize bytes from staging (or from the bit-planes for
-// a raw chunk) to their offset in the stream: head bytes, funnel-shifted
-// words, tail bytes.  No source word past the chunk's own bytes is read.
-__device__ __forceinline__ void gather_chunk(const uint8_t* __restrict__ bp, const uint8_t* __restrict__ staging,
-                                             const ChunkGeom& g, const int64_t* __restrict__ offsets,
-                                             uint8_t* __restrict__ dst_stream, int64_t id) {
-    const int lane = threadIdx.x & 63;
-    const int csize = reinterpret_cast<const uint16_t*>(dst_stream + 8)[id];
-    const ChunkPos cp = chunk_pos(g, id);
-    const bool raw = csize == cp.osize;
-    const uint8_t* sb = raw ? bp + cp.base : staging + id * kCS;  // both 4-byte aligned
-    const uint32_t* sw = reinterpret_cast<const uint32_t*>(sb);
-    const int swords = (csize + 3) >> 2;  // source words that hold chunk bytes
-    uint8_t* d = dst_stream + 8 + 2 * g.nchunks + offsets[id];
-    int head = (int)((4 - ((uintptr_t)d & 3)) & 3);
-    if (head > csize) head = csize;
-    if (lane < head) d[lane] = sb[lane];
-    const int nw = (csize - head) >> 2;
-    uint32_t* dw = reinterpret_cast<uint32_t*>(d + head);
-    // batches of 8 words per lane: all 16 loads of a batch are in flight
-    // before the first store (a chunk is up to 4096 words)
-    constexpr int kB = 8;
-    for (int w0 = lane; w0 < nw; w0 += 64 * kB) {
-        uint32_t lo[kB], hi[kB];
-#pragma unroll
-        for (int k = 0; k < kB; ++k) {
-            const int w = w0 + 64 * k;
-            lo[k] = w < nw ? sw[w] : 0u;
-            hi[k] = w < nw && w + 1 < swords ? sw[w + 1] : 0u;
-        }
-#pragma unroll
-        for (int k = 0; k < kB; ++k) {
-            const int w = w0 + 64 * k;
-            if (w < nw) dw[w] = head == 0 ? lo[k] : (lo[k] >> (8 * head)) | (hi[k] << (32 - 8 * head));
-        }
-    }
-    const int tail0 = head + 4 * nw;
-    if (lane < csize - tail0) d[tail0 + lane] = sb[tail0 + lane];
-}
-
-__global__ __launch_bounds__(256) void gather_kernel(const uint8_t* __restrict__ bp, const uint8_t* __restrict__ staging,
-                                                     ChunkGeom g, const int64_t* __restrict__ offsets,
-                                                     uint8_t* __restrict__ dst_stream) {
-    const int64_t id = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (id >= g.nchunks) return;
-    gather_chunk(bp, staging, g, offsets, dst_stream, id);
-}
-
-// `run` consecutive chunks per wave
-__global__ __launch_bounds__(256) void batch_gather_kernel(const BatchField* __restrict__ fields,
-                                                           const int64_t* __restrict__ prefix, int nf, int64_t total,
-                                                           int run) {
-    const int64_t gid0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * run;
-    for (int r = 0; r < run; ++r) {
-        const int64_t gid = gid0 + r;
-        if (gid >= total) return;
-        const int f = find_field(prefix, 0, nf, gid);
-        const BatchField& F = fields[f];
-        gather_chunk(F.bp, F.staging, F.cg, F.offsets, F.out, gid - prefix[f]);
-    }
 }
 
 // compress_merge body: zero pad to 8 bytes, then flags, range, size table,
@@ -862,29 +981,28 @@ int pa::retr_decode(const RetrDev& r, hipStream_t st) {
     return PA_OK;
 }
 
-// offsets (8 B per chunk), then a 16 KiB staging slot per chunk (+16 B of
-// slack for the rounded-up copy of the last slot)
+// the error word (an 8-byte slot), then the look-back status words (8 B per chunk)
 extern "C" size_t pa_encode_workspace(const pa_layout* L) {
     if (!L) return 0;
-    return ((size_t)L->nchunks * 8 + 255) / 256 * 256 + (size_t)L->nchunks * kCS + 256;
+    return ((size_t)L->nchunks + 1) * 8;
 }
+
+// chunks per encoder workgroup (one per wave64)
+constexpr int kEncodeWaves = 4;
 
 extern "C" int pa_lossless_encode(const uint8_t* d_bp, const pa_layout* L, uint8_t* d_out, uint64_t* d_cum,
                                   uint64_t* d_outsize, void* d_ws, size_t ws_bytes, void* stream) {
     if (!L || !d_bp || !d_out || !d_cum || !d_outsize) return PA_ERR_ARG;
-    if (!d_ws || ws_bytes < pa_encode_workspace(L)) return PA_ERR_WORKSPACE;
+    if (!d_ws || ws_bytes < pa_encode_workspace(L) || ((uintptr_t)d_ws & 7) != 0) return PA_ERR_WORKSPACE;
     hipStream_t st = as_stream(stream);
     ChunkGeom g = make_chunk_geom(L);
-    int64_t* offsets = static_cast<int64_t*>(d_ws);
-    uint8_t* staging = static_cast<uint8_t*>(d_ws) + ((size_t)L->nchunks * 8 + 255) / 256 * 256;
-    const unsigned eblocks = (unsigned)((L->nchunks + kWaves - 1) / kWaves);
-    hipLaunchKernelGGL(encode_kernel, dim3(eblocks), dim3(64 * kWaves), 0, st, d_bp, g, d_out, staging);
+    uint64_t* ws = static_cast<uint64_t*>(d_ws);
+    hipLaunchKernelGGL(encode_clear_kernel, dim3((unsigned)((L->nchunks + 1 + 255) / 256)), dim3(256), 0, st, ws,
+                       L->nchunks);
     PA_LAUNCH_CHECK();
-    hipLaunchKernelGGL(stream_scan_kernel, dim3(1), dim3(1024), 0, st, reinterpret_cast<const uint16_t*>(d_out + 8),
-                       g, offsets, d_out, d_cum, d_outsize);
-    PA_LAUNCH_CHECK();
-    hipLaunchKernelGGL(gather_kernel, dim3((unsigned)((L->nchunks + 3) / 4)), dim3(256), 0, st, d_bp, staging, g,
-                       offsets, d_out);
+    EncodeOut o{d_out, d_cum, d_outsize, ws + 1, reinterpret_cast<int32_t*>(ws)};
+    const unsigned eblocks = (unsigned)((L->nchunks + kEncodeWaves - 1) / kEncodeWaves);
+    hipLaunchKernelGGL(encode_kernel<kEncodeWaves>, dim3(eblocks), dim3(64 * kEncodeWaves), 0, st, d_bp, g, o);
     PA_LAUNCH_CHECK();
     return PA_OK;
 }
@@ -1005,25 +1123,29 @@ int64_t pa::batch_merge_blocks(const pa_layout* L, int dtype) {
 }
 
 int pa::batch_encode(const BatchDev& b, hipStream_t st) {
-    if (b.n_chunk > 0) {
-        hipLaunchKernelGGL(batch_encode_kernel, dim3((unsigned)((b.n_chunk + kWaves - 1) / kWaves)), dim3(64 * kWaves),
-                           0, st, b.fields, b.p_chunk, b.nf, b.n_chunk);
-        PA_LAUNCH_CHECK();
-    }
-    hipLaunchKernelGGL(batch_scan_kernel, dim3((unsigned)b.nf), dim3(1024), 0, st, b.fields);
-    PA_LAUNCH_CHECK();
-    int e = batch_gather(b, st);
+    int e = batch_encode_chunks(b, st);
     if (e != PA_OK) return e;
     hipLaunchKernelGGL(batch_merge_kernel, dim3((unsigned)b.n_merge), dim3(256), 0, st, b.fields, b.p_merge, b.nf);
     PA_LAUNCH_CHECK();
     return PA_OK;
 }
 
-int pa::batch_gather(const BatchDev& b, hipStream_t st) {
+template <int kW>
+static void launch_batch_encode(const BatchDev& b, hipStream_t st) {
+    hipLaunchKernelGGL(batch_encode_kernel<kW>, dim3((unsigned)(b.n_chunk / kW)), dim3(64 * kW), 0, st, b.fields,
+                       b.p_chunk, b.nf);
+}
+
+int pa::batch_encode_chunks(const BatchDev& b, hipStream_t st) {
+    hipLaunchKernelGGL(batch_encode_clear_kernel, dim3((unsigned)((b.n_chunk + b.nf + 255) / 256)), dim3(256), 0, st,
+                       b.fields, b.p_chunk, b.nf, b.n_chunk);
+    PA_LAUNCH_CHECK();
     if (b.n_chunk <= 0) return PA_OK;
-    const int64_t per = 4 * (int64_t)b.gather_run;
-    hipLaunchKernelGGL(batch_gather_kernel, dim3((unsigned)((b.n_chunk + per - 1) / per)), dim3(256), 0, st, b.fields,
-                       b.p_chunk, b.nf, b.n_chunk, b.gather_run);
+    switch (b.encode_waves) {
+        case 2: launch_batch_encode<2>(b, st); break;
+        case 8: launch_batch_encode<8>(b, st); break;
+        default: launch_batch_encode<4>(b, st); break;
+    }
     PA_LAUNCH_CHECK();
     return PA_OK;
 }

@@ -157,8 +157,7 @@ struct BatchField {
     int32_t* qc;        // code_len
     void* anchors;      // n4 values
     uint8_t* bp;        // host_size
-    int64_t* offsets;   // nchunks
-    uint8_t* staging;   // nchunks x 16 KiB
+    uint64_t* encws;    // encoder workspace: error word, then nchunks look-back status words
     uint8_t* out;       // pa_compress_bound bytes
     uint64_t* cum;      // 128
     uint64_t* outsize;  // unpadded stream length

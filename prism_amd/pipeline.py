@@ -194,14 +194,18 @@ def compress(data: torch.Tensor, eb: float, rel: bool = True, progressive: bool 
     dev = data.device
     V = C.c_void_p
     # the small per-call buffers in one allocation (byte offsets): bounds
-    # double[16] at 0, min/max double[4] at 128, scores double[15] at 160,
-    # flags uint8[8] at 280, cum int64[128] at 288, sizes int64[2] at 1312,
-    # then the tuner's workspace at 1344
+    # double[16] at 0, the encoder's workspace at 128 (its error word first,
+    # so that the one host read below takes it with the bounds), then min/max
+    # double[4], scores double[15], flags uint8[8], cum int64[128], sizes
+    # int64[2] and the tuner's workspace from e = 128 + encoder workspace
     tws_bytes = max(8, lib().pa_autotune_workspace(C.byref(L)))
-    small = torch.empty((1344 + tws_bytes + 7) // 8, dtype=torch.float64, device=dev)
+    ews_bytes = lib().pa_encode_workspace(C.byref(L))
+    e = 128 + (ews_bytes + 127) // 128 * 128
+    small = torch.empty((e + 1216 + tws_bytes + 7) // 8, dtype=torch.float64, device=dev)
     sp = small.data_ptr()
-    p_bounds, p_mm, p_scores, p_flags, p_cum, p_sizes, p_tws = (V(sp), V(sp + 128), V(sp + 160), V(sp + 280),
-                                                                V(sp + 288), V(sp + 1312), V(sp + 1344))
+    p_bounds, p_ews, p_mm, p_scores, p_flags, p_cum, p_sizes, p_tws = (
+        V(sp), V(sp + 128), V(sp + e), V(sp + e + 32), V(sp + e + 152), V(sp + e + 160), V(sp + e + 1184),
+        V(sp + e + 1216))
     if rel:  # extrema, auto-tuning and the level bounds share one launch sequence
         with tm("extrema"):
             check(lib().pa_extrema_autotune(_p(data), dt, C.byref(L), p_mm, float(eb), p_bounds, p_scores, p_flags,
@@ -221,24 +225,26 @@ def compress(data: torch.Tensor, eb: float, rel: bool = True, progressive: bool 
         check(lib().pa_bitshuffle(_p(qc), C.byref(L), mode, _p(bp), st), "pa_bitshuffle")
     bound = lib().pa_compress_bound(C.byref(L), dt)
     out = _empty(bound)
-    ews = _empty(lib().pa_encode_workspace(C.byref(L)))
     with tm("encode"):
-        check(lib().pa_lossless_encode(_p(bp), C.byref(L), _p(out), p_cum, p_sizes, _p(ews), ews.numel(), st),
+        check(lib().pa_lossless_encode(_p(bp), C.byref(L), _p(out), p_cum, p_sizes, p_ews, ews_bytes, st),
               "pa_lossless_encode")
     with tm("merge"):  # the stream length goes to bounds[15] (int64 bits)
         check(lib().pa_compress_merge(_p(out), C.byref(L), dt, p_sizes, p_flags, V(sp + 8), p_cum, _p(anchors),
                                       V(sp + 120), st), "pa_compress_merge")
-    # the one host read: stream length, status, range
-    vals = _host_read(small[0:16])
+    # the one host read: stream length, status, range, the encoder's error word
+    vals = _host_read(small[0:17])
     if vals[14] != 0:
         raise ValueError("error bound must be positive (constant field in REL mode?)")
+    if vals[16:17].view(np.int64)[0] & 0xFFFFFFFF:
+        raise _lib.PrismError("pa_lossless_encode: a chunk gave up waiting for the payload offsets before it")
     total = int(vals.view(np.int64)[15])
     rng = float(vals[1]) if rel else 1.0
     c = Compressed(stream=out[:total], shape=(nx, ny, nz), dtype=data.dtype, eb_input=eb, rel=rel,
                    progressive=progressive, range=rng, flags=[])
     if return_parts:
-        return c, {"qc": qc, "anchors": anchors, "bp": bp, "cum": small[36:164].view(torch.int64),
-                   "flags": small[35:36].view(torch.uint8), "scores": small[20:35], "layout": L}
+        e8 = e // 8
+        return c, {"qc": qc, "anchors": anchors, "bp": bp, "cum": small[e8 + 20:e8 + 148].view(torch.int64),
+                   "flags": small[e8 + 19:e8 + 20].view(torch.uint8), "scores": small[e8 + 4:e8 + 19], "layout": L}
     return c
 
 
@@ -295,8 +301,8 @@ class BatchCompressor:
         p = (C.c_int32 * 3)()
         t = (C.c_float * 9)()
         check(lib().pa_batch_plan_params(self.plan, p, t), "pa_batch_plan_params")
-        return {"extrema_batches": p[0], "xcd_order": p[1], "gather_run": p[2],
-                "tuned_ms": {"extrema": list(t[0:4]), "spline_order": list(t[4:6]), "gather": list(t[6:9])}}
+        return {"extrema_batches": p[0], "xcd_order": p[1], "encode_waves": p[2],
+                "tuned_ms": {"extrema": list(t[0:4]), "spline_order": list(t[4:6]), "encode": list(t[6:9])}}
 
     def _descs(self, fields, ebs):
         if len(fields) != self.nf:

@@ -62,12 +62,12 @@ def _tensor_bytes(*ts):
 
 def _compress_need(t: torch.Tensor) -> int:
     """Device bytes a compression of t holds while it runs: output bound and
-    batch workspace (codes, bit-planes, staging and tables)."""
+    batch workspace (codes, bit-planes, encoder status words and tables)."""
     nx, ny, nz = pipeline.shape_xyz(t)
     L = pipeline.layout(nx, ny, nz)
     lib = pipeline.lib()
     bound = lib.pa_compress_bound(pipeline.C.byref(L), pipeline.DTYPES[t.dtype])
-    return int(bound + 4 * L.code_len + L.host_size + L.nchunks * (16384 + 8) + L.n4 * t.element_size() + 8192)
+    return int(bound + 4 * L.code_len + L.host_size + (L.nchunks + 1) * 8 + L.n4 * t.element_size() + 8192)
 
 
 def _result_bytes(req: Request) -> int:

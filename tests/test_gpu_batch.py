@@ -52,9 +52,9 @@ def test_mixed_batch_equals_plain(pl, graph, autotune):
     assert b.launches <= 15
     assert_same(b.run(), want, f"graph={graph} autotune={autotune}")
     p = b.params
-    assert p["extrema_batches"] in (2, 4, 8, 16) and p["xcd_order"] in (0, 1) and p["gather_run"] in (1, 2, 4)
+    assert p["extrema_batches"] in (2, 4, 8, 16) and p["xcd_order"] in (0, 1) and p["encode_waves"] in (2, 4, 8)
     if autotune:
-        assert all(t > 0 for t in p["tuned_ms"]["extrema"] + p["tuned_ms"]["gather"])
+        assert all(t > 0 for t in p["tuned_ms"]["extrema"] + p["tuned_ms"]["encode"])
         assert_same(b.run(), want, "replay after tuning")
 
 
