@@ -37,6 +37,10 @@ def parse():
     ap.add_argument("--shape", type=int, nargs=3, default=[384, 384, 256], metavar=("NX", "NY", "NZ"))
     ap.add_argument("--rel-eb", type=float, default=1e-6)
     ap.add_argument("--targets", type=float, nargs="+", default=[1e-2, 1e-3, 1e-4, 1e-5, 1e-6])
+    ap.add_argument("--grf-steps", type=int, default=5,
+                    help="timed steps of the 512^3 float32 Gaussian random field (rel 1e-4); 0 skips it")
+    ap.add_argument("--only", choices=["all", "grf"], default="all",
+                    help="grf: run only the Gaussian random field workload and print its own JSON line")
     ap.add_argument("--batch-fields", type=int, default=64,
                     help="fields of the mixed batch (batched vs plain per-field launches); 0 skips it")
     ap.add_argument("--batch-steps", type=int, default=5)
@@ -55,11 +59,9 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (prism_amd runs each field on one GPU and shards nothing)",
               file=sys.stderr)
         sys.exit(2)
-    import numpy as np
     import torch
 
-    from prism_amd import fields, pipeline
-    from prism_amd._lib import layout
+    from prism_amd import fields
 
     torch.cuda.init()
     dev = torch.device("cuda", 0)
@@ -70,22 +72,93 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.only == "grf":  # the Gaussian random field workload alone (for profiling), its own JSON line
+        print(json.dumps({"grf512": grf_section(args, dev)}))
+        return
     nx, ny, nz = args.shape
     field = fields.density(nx, ny, nz, seed=2, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
+    main = progressive_workload(field, args.rel_eb, args.targets, args.steps, args.warmup)
+    s = field.element_size()
+    stats = main["stages"]
+    dominant = max(stats, key=lambda k: stats[k]["avg_ms"] * stats[k]["launches_per_step"])
+    d = stats[dominant]
+    roofline = {"bound": "hbm", "kernel": dominant, "achieved": d["gbps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": (d["gbps"] / HBM_PEAK_GBPS) if d["gbps"] else None, "traffic": None,
+                "avg_launch_ms": d["avg_ms"], "algorithmic_bytes_per_launch": d["avg_bytes"]}
+    roofline.update(measured_traffic(dominant, "density", nx, ny, nz, s))
+    del field
+
+    grf = grf_section(args, dev) if args.grf_steps > 0 else None
+
+    batched = batched_section(args) if args.batch_fields > 0 else None
+    serving = serving_section(args) if args.serve_requests > 0 else None
+
+    cpu = None
+    if args.cpu_baseline == "on" and int(os.environ.get("RANK", "0")) == 0:
+        cpu = cpu_baseline(fields.density(nx, ny, nz, seed=2, dtype=torch.float64, device=dev), args)
+
+    line = {
+        "metric": METRIC, "value": main["value"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": main["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (Miranda-like density: 1 + 0.5 tanh(8 plane waves) + N(0,1e-4), seed 2)",
+        "config": {"workload": "progressive density field 384x384x256 f64, rel eb 1e-6, retrieve 1e-2..1e-6",
+                   "shape_xyz": [nx, ny, nz], "rel_eb": args.rel_eb, "targets": args.targets},
+        "compress_GBps": main["compress_GBps"], "decompress_GBps": main["decompress_GBps"],
+        "compression_ratio": main["compression_ratio"], "stream_bytes": main["stream_bytes"],
+        "levels": main["levels"], "stages": stats, "roofline": roofline,
+        "roofline_end_to_end": main["roofline_end_to_end"], "cpu_baseline": cpu, "grf512": grf,
+        "batched": batched, "serving": serving,
+    }
+    print(json.dumps(line))
+
+
+def grf_section(args, dev):
+    """The 512^3 float32 Gaussian random field (power ~ |k|^-5/3, random
+    phases seed 1, unit variance) at relative bound 1e-4, retrieved at 1e-1,
+    1e-2, 1e-3 and 1e-4."""
+    import torch
+
+    from prism_amd import fields
+
+    g = fields.grf(512, 512, 512, seed=1, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    out = progressive_workload(g, 1e-4, [1e-1, 1e-2, 1e-3, 1e-4], args.grf_steps, 1)
+    out["config"] = {"workload": "Gaussian random field 512^3 f32 (power ~ |k|^-5/3, seed 1, unit variance), "
+                                 "rel eb 1e-4, retrieve 1e-1..1e-4", "shape_xyz": [512, 512, 512]}
+    return out
+
+
+def progressive_workload(field, rel_eb, targets, steps, warmup):
+    """One field, compressed as a progressive stream at rel_eb and retrieved
+    progressively at each target, with the field resident in HBM.  A step
+    is one compress plus a fresh retrieval through every target; compress
+    and retrieval are timed separately with host clocks around
+    device-synchronised phases.  Reports throughput, per-stage device time
+    with algorithmic bytes, per-level quality against the original, and the
+    end-to-end roofline fractions on primary bytes (compress: n s + |Z|;
+    retrieval round k: |Z_k| + n s written + n s of the previous level read
+    from round 2 on)."""
+    import torch
+
+    from prism_amd import pipeline
+    from prism_amd._lib import layout
+
+    nz, ny, nx = field.shape
     L = layout(nx, ny, nz)
     s = field.element_size()
     field_bytes = field.numel() * s
 
     def one_step(timer=None):
-        c = pipeline.compress(field, args.rel_eb, rel=True, progressive=True, timer=timer)
+        c = pipeline.compress(field, rel_eb, rel=True, progressive=True, timer=timer)
         r = pipeline.Retriever(c)
-        for t in args.targets:
+        for t in targets:
             r.retrieve(t, timer=timer, check_stream=False)
         r.check()
         return c, r
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         one_step()
     torch.cuda.synchronize()
 
@@ -94,13 +167,13 @@ def main():
     t_cmp = t_dec = 0.0
     torch.cuda.synchronize()
     t_all0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         a = time.perf_counter()
-        c = pipeline.compress(field, args.rel_eb, rel=True, progressive=True)
+        c = pipeline.compress(field, rel_eb, rel=True, progressive=True)
         torch.cuda.synchronize()
         b = time.perf_counter()
         r = pipeline.Retriever(c)
-        for t in args.targets:
+        for t in targets:
             r.retrieve(t, check_stream=False)
         r.check()  # the decoder's error word, read once after the last round
         torch.cuda.synchronize()
@@ -108,25 +181,23 @@ def main():
         t_cmp += b - a
         t_dec += e - b
     t_all = time.perf_counter() - t_all0
-    ms_per_step = 1e3 * t_all / args.steps
-    cmp_gbps = field_bytes * args.steps / t_cmp / 1e9
-    dec_gbps = field_bytes * args.steps / t_dec / 1e9
-    value = 2 * field_bytes * args.steps / (t_cmp + t_dec) / 1e9
+    del c, r
 
     # per-stage device time (HIP events on the launch stream), separate pass
     timer = pipeline.StageTimer()
-    for _ in range(max(1, min(args.steps, 5))):
+    passes = max(1, min(steps, 5))
+    for _ in range(passes):
         one_step(timer)
     stage_ms = timer.results()
 
-    # quality and bytes per level, untimed: statistics against the FP64
-    # original with the device kernel (L-inf, PSNR, NRMSE in one pass)
-    c = pipeline.compress(field, args.rel_eb, rel=True, progressive=True)
+    # quality and bytes per level, untimed: statistics against the original
+    # with the device kernel (L-inf, PSNR, NRMSE in one pass)
+    c = pipeline.compress(field, rel_eb, rel=True, progressive=True)
     stream_bytes = c.nbytes
     r = pipeline.Retriever(c)
     levels = []
     prev = 0
-    for i, t in enumerate(args.targets):
+    for t in targets:
         torch.cuda.synchronize()
         a = time.perf_counter()
         out = r.retrieve(t)
@@ -134,23 +205,25 @@ def main():
         lat = (time.perf_counter() - a) * 1e3
         q = pipeline.quality(field, out)
         nread = int(r.bytes_read.item())
-        narrow = int(r.narrow.item())
-        levels.append({"target": t, "bound_abs": t * c.range, "linf": q["linf"], "int16_codes": bool(narrow),
-                       "linf_over_bound": q["linf"] / (t * c.range), "psnr_db": q["psnr_db"], "nrmse": q["nrmse"],
+        levels.append({"target": t, "bound_abs": t * c.range, "linf": q["linf"],
+                       "int16_codes": bool(int(r.narrow.item())),
+                       "linf_over_bound": q["linf"] / (t * c.range), "within_bound": q["linf"] <= t * c.range,
+                       "sse_finite": math.isfinite(q["sse"]), "psnr_db": q["psnr_db"], "nrmse": q["nrmse"],
                        "payload_bytes_read": nread - prev, "ms": lat,
                        "begin": r.begin.tolist(), "end": r.end.tolist()})
         prev = nread
+        del out
+    del r, c
 
-    # roofline of the dominant kernel: per-launch algorithmic bytes over the
-    # launch's own event-timed duration, averaged over the launches
-    nrounds = len(args.targets)
+    # per-launch algorithmic bytes of each stage
+    nrounds = len(targets)
     codes = 4 * (L.n - L.n4)  # int32 codes written by the predictor and read by the shuffle
 
     def planes_bytes(lv_info):
         return sum((lv_info["end"][3 - lv] - lv_info["begin"][3 - lv]) * L.dev_stride[lv] for lv in range(4))
 
     def launch_bytes(name, k):
-        if name == "extrema":
+        if name in ("extrema", "autotune"):
             return L.n * s
         if name == "predict":
             return L.n * s + codes + L.n4 * s
@@ -168,7 +241,6 @@ def main():
             return code_b * (L.n - L.n4) + L.n * s + (L.n * s if k % nrounds else L.n4 * s)
         return 0
 
-    passes = max(1, min(args.steps, 5))
     stats = {}
     for name, times in stage_ms.items():
         byts = [launch_bytes(name, k) for k in range(len(times))]
@@ -176,32 +248,28 @@ def main():
         stats[name] = {"avg_ms": avg, "launches_per_step": len(times) // passes,
                        "avg_bytes": sum(byts) / len(byts),
                        "gbps": (sum(byts) / (sum(times) * 1e-3) / 1e9) if sum(byts) else None}
-    dominant = max(stats, key=lambda k: stats[k]["avg_ms"] * stats[k]["launches_per_step"])
-    d = stats[dominant]
-    roofline = {"bound": "hbm", "kernel": dominant, "achieved": d["gbps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": (d["gbps"] / HBM_PEAK_GBPS) if d["gbps"] else None, "traffic": None,
-                "avg_launch_ms": d["avg_ms"], "algorithmic_bytes_per_launch": d["avg_bytes"]}
-    roofline.update(measured_traffic(dominant, nx, ny, nz, s))
 
-    batched = batched_section(args) if args.batch_fields > 0 else None
-    serving = serving_section(args) if args.serve_requests > 0 else None
+    # end-to-end roofline fractions on primary bytes, over the wall-clock
+    # time of the timed loop and over the summed device stage times
+    cmp_bytes = field_bytes + stream_bytes
+    ret_bytes = sum(lv["payload_bytes_read"] + field_bytes + (field_bytes if k else 0) for k, lv in enumerate(levels))
+    cmp_stages = [k for k in ("extrema", "autotune", "predict", "shuffle", "encode", "merge") if k in stats]
+    ret_stages = [k for k in ("strategy", "decode", "unshuffle", "reconstruct") if k in stats]
+    cmp_dev = sum(stats[k]["avg_ms"] * stats[k]["launches_per_step"] for k in cmp_stages) * 1e-3
+    ret_dev = sum(stats[k]["avg_ms"] * stats[k]["launches_per_step"] for k in ret_stages) * 1e-3
 
-    cpu = None
-    if args.cpu_baseline == "on" and int(os.environ.get("RANK", "0")) == 0:
-        cpu = cpu_baseline(field, args)
+    def frac(byts, sec):
+        return {"bytes": byts, "seconds": sec, "GBps": byts / sec / 1e9, "frac": byts / sec / 1e9 / HBM_PEAK_GBPS}
 
-    line = {
-        "metric": METRIC, "value": value, "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic (Miranda-like density: 1 + 0.5 tanh(8 plane waves) + N(0,1e-4), seed 2)",
-        "config": {"workload": "progressive density field 384x384x256 f64, rel eb 1e-6, retrieve 1e-2..1e-6",
-                   "shape_xyz": [nx, ny, nz], "rel_eb": args.rel_eb, "targets": args.targets},
-        "compress_GBps": cmp_gbps, "decompress_GBps": dec_gbps,
-        "compression_ratio": field_bytes / stream_bytes, "stream_bytes": stream_bytes,
-        "levels": levels, "stages": stats, "roofline": roofline, "cpu_baseline": cpu, "batched": batched,
-        "serving": serving,
-    }
-    print(json.dumps(line))
+    e2e = {"unit": "GB/s", "peak": HBM_PEAK_GBPS,
+           "compress": {"wall": frac(cmp_bytes, t_cmp / steps), "device": frac(cmp_bytes, cmp_dev)},
+           "retrieve": {"wall": frac(ret_bytes, t_dec / steps), "device": frac(ret_bytes, ret_dev),
+                        "rounds": nrounds}}
+    return {"value": 2 * field_bytes * steps / (t_cmp + t_dec) / 1e9, "ms_per_step": 1e3 * t_all / steps,
+            "compress_GBps": field_bytes * steps / t_cmp / 1e9, "decompress_GBps": field_bytes * steps / t_dec / 1e9,
+            "compress_ms": 1e3 * t_cmp / steps, "retrieve_ms": 1e3 * t_dec / steps, "steps": steps,
+            "compression_ratio": field_bytes / stream_bytes, "stream_bytes": stream_bytes, "field_bytes": field_bytes,
+            "levels": levels, "stages": stats, "roofline_end_to_end": e2e}
 
 
 def batched_section(args):
@@ -343,13 +411,13 @@ def serving_run(comp, reqs):
             "resident_bytes": srv.resident_bytes, "held_bytes_at_end": srv.held_bytes, "budget_bytes": srv.budget}
 
 
-def measured_traffic(stage, nx, ny, nz, s):
+def measured_traffic(stage, workload, nx, ny, nz, s):
     """HBM bytes per launch of a stage's kernel from the committed rocprofv3
     counter summary of this workload (FETCH_SIZE and WRITE_SIZE passes, each
     scaled by the same counter on a copy kernel of known size); empty when no
     summary covers this shape and dtype."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        f"traffic_density_{nx}x{ny}x{nz}_f{8 * s}.json")
+                        f"traffic_{workload}_{nx}x{ny}x{nz}_f{8 * s}.json")
     try:
         with open(path) as f:
             prof = json.load(f)

@@ -219,9 +219,10 @@ int pa_quality(const void* d_ori, const void* d_rec, int dtype, int64_t n, doubl
 
 /* ---- Batched compression (no reference counterpart) ----
  * Many fields of mixed shapes and dtypes compressed by one fixed sequence of
- * launches: one per stage (extrema, bound, tuner, code gaps, spline,
- * shuffle, encoder status clear, encode, trailer; at most 14 kernels) whatever the
- * number of fields, optionally captured once in a HIP graph and replayed.
+ * launches (tuner, extrema, code gaps and encoder status clear in one; the
+ * bounds and flags; spline per dtype; shuffle; encode; trailer: at most 7
+ * kernels) whatever the number of fields, optionally captured once in a
+ * HIP graph and replayed.
  * Each field's stream is byte-identical to the one the plain per-field
  * calls (pa_extrema .. pa_compress_merge) produce for the same input. */
 typedef struct pa_batch_field {

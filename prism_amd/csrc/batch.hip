@@ -97,18 +97,17 @@ int plan_order(const pa_batch_field* fields, int nf, std::vector<int>& order) {
 
 int enqueue(const pa_batch_plan* p, hipStream_t st) {
     int e;
-    if ((e = pa::batch_extrema(p->dev, st)) != PA_OK) return e;
-    if ((e = pa::batch_predict(p->dev, st)) != PA_OK) return e;
+    if ((e = pa::batch_front(p->dev, st)) != PA_OK) return e;
+    if ((e = pa::batch_spline(p->dev, st)) != PA_OK) return e;
     if ((e = pa::batch_shuffle(p->dev, st)) != PA_OK) return e;
     return pa::batch_encode(p->dev, st);
 }
 
 int count_launches(const pa::BatchDev& b) {
-    return 2 + (b.n_ext > 0)                                                      // extrema
-           + 2 + (b.n_tune32 > 0) + (b.n_tune > b.n_tune32) + (b.n_gap > 0)       // bound, tuner, pick, gaps
-           + (b.n_tile32 > 0) + (b.n_tile > b.n_tile32)                           // spline
-           + (b.n_shuf > 0)                                                       // shuffle
-           + 2 + (b.n_chunk > 0);                                                 // clear, encode, trailer
+    return 2                                        // front (tuner, extrema, gaps, status clear), finish
+           + (b.n_tile32 > 0) + (b.n_tile > b.n_tile32)  // spline, per dtype
+           + (b.n_shuf > 0)                         // shuffle
+           + (b.n_chunk > 0) + 1;                   // encode, trailer
 }
 
 // descriptors (with device pointers) of every field, in plan order; fails
@@ -208,14 +207,18 @@ int upload(pa_batch_plan* p, hipStream_t st) {
     d.n_tune32 = P[4][p->nf32];
     d.n_tile32 = P[6][p->nf32];
     const int64_t lim = 0x7fffffff;
-    if (p->ext_n[0] > lim || d.n_tune > lim || d.n_gap > lim || d.n_tile > lim || d.n_shuf > lim ||
-        d.n_chunk + d.nf > lim || d.n_merge > lim)
+    // the front launch's grid holds every tuner, extrema (most for the
+    // smallest load batch), gap and status-clear workgroup
+    const int64_t front = 15 * d.n_tune + p->ext_n[0] + d.n_gap + (d.n_chunk + nf + 255) / 256;
+    if (front > lim || d.n_tile > lim || d.n_shuf > lim || d.n_chunk + d.nf > lim || d.n_merge > lim)
         return PA_ERR_ARG;
     if (hipMemcpyAsync(p->table, p->host.data(), (size_t)nf * sizeof(BatchField), hipMemcpyHostToDevice, st) !=
         hipSuccess)
         return PA_ERR_HIP;
     if (hipMemcpyAsync(p->table + desc, pre.data(), pre.size() * 8, hipMemcpyHostToDevice, st) != hipSuccess)
         return PA_ERR_HIP;
+    const int e = pa::batch_keys_init(d, st);  // the finish kernel resets them after every run
+    if (e != PA_OK) return e;
     p->launches = count_launches(d);
     return PA_OK;
 }
@@ -340,7 +343,7 @@ extern "C" int pa_batch_plan_autotune(pa_batch_plan* p, int reps, void* stream) 
         d.ext_batches = pa::kExtChoices[i];
         d.p_ext = p->ext_tab[i];
         d.n_ext = p->ext_n[i];
-        p->tuned_ms[i] = time(pa::batch_extrema);
+        p->tuned_ms[i] = time(pa::batch_front);
         if (p->tuned_ms[i] < p->tuned_ms[best] || i == 0) best = i;
     }
     d.ext_batches = pa::kExtChoices[best];

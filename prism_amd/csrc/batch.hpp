@@ -61,11 +61,16 @@ int retr_reconstruct(const RetrDev& r, hipStream_t st);  // predictor.hip
 void retr_level_ebs(RetrJob& j, double eb, double rel_eb);
 int64_t retr_unshuf_blocks(const pa_layout* L);
 
-int batch_extrema(const BatchDev& b, hipStream_t st);  // extrema.hip
-int batch_predict(const BatchDev& b, hipStream_t st);  // predictor.hip: eb, tuner, code gaps, spline
+// The batched compression: batch_front, batch_spline, batch_shuffle,
+// batch_encode (at most 7 launches).
+int batch_front(const BatchDev& b, hipStream_t st);    // predictor.hip: tuner, extrema, code gaps and encoder
+                                                       // status clear in one launch, then bounds and flags
+int batch_spline(const BatchDev& b, hipStream_t st);   // predictor.hip: one launch per dtype
 int batch_shuffle(const BatchDev& b, hipStream_t st);  // bitplane.hip
-int batch_encode(const BatchDev& b, hipStream_t st);   // codec.hip: status clear, encode, trailer
-int batch_spline(const BatchDev& b, hipStream_t st);   // the spline launches of batch_predict
-int batch_encode_chunks(const BatchDev& b, hipStream_t st);  // the clear and encode launches of batch_encode
+int batch_encode(const BatchDev& b, hipStream_t st);   // codec.hip: encode, trailer
+int batch_keys_init(const BatchDev& b, hipStream_t st);  // predictor.hip: extrema keys before the first run
+// the encoder alone, status words cleared first (the autotuner times it
+// apart from the front launch that clears them in a run)
+int batch_encode_chunks(const BatchDev& b, hipStream_t st);
 
 }  // namespace pa

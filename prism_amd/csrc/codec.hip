@@ -506,71 +506,70 @@ __global__ void batch_encode_clear_kernel(const BatchField* __restrict__ fields,
 }
 
 // The decoder's payload offsets by one workgroup scan of the size table
-// (zbpc.cu:265-365 sums the sizes before each chunk).
-// One workgroup of 1024 lanes: lane t takes the run of `per` chunks from
-// t * per.  Up to kScanLds sizes are first staged through LDS with
-// coalesced loads, so the runs read LDS rather than scattered 2-byte words.
+// (zbpc.cu:265-365 sums the sizes before each chunk).  One workgroup of 1024
+// lanes takes the table in windows of kScanLds sizes: a window is staged
+// through LDS with coalesced 8-byte loads, lane t sums its run of
+// consecutive sizes, a wave then workgroup scan gives the run offsets, and
+// the window's total carries into the next.
 constexpr int kScanLds = 16384;
 
 struct ScanLds {
     int64_t w[16];
-    int64_t base[1024];  // exclusive offset of each lane's run
     uint16_t sz[kScanLds];
 };
 
-// exclusive offsets of the payloads; returns the lane's run start and fills
-// S.base; the sizes come from LDS when staged
-__device__ __forceinline__ const uint16_t* scan_runs(const uint16_t* __restrict__ sizes, int64_t n, ScanLds& S,
-                                                     int64_t* __restrict__ offsets, int64_t* total) {
+__device__ __forceinline__ void scan_sizes(const uint16_t* __restrict__ sizes, int64_t n, ScanLds& S,
+                                           int64_t* __restrict__ offsets) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const bool staged = n <= kScanLds;
-    if (staged) {
-        // 8-byte words (4 sizes), all of a lane's loads in flight at once:
-        // the table was just written by other CUs, so each round trip is long
-        const bool words = ((uintptr_t)sizes & 7) == 0;
-        const int64_t nw = words ? n / 4 : 0;
-        const uint2* w = reinterpret_cast<const uint2*>(sizes);
-        constexpr int kU = 4;  // 4 x 1024 words cover kScanLds sizes
-        uint2 v[kU];
+    const bool words = ((uintptr_t)sizes & 7) == 0;
+    int64_t carry = 0;
+    for (int64_t w0 = 0; w0 < n; w0 += kScanLds) {
+        const int64_t m = n - w0 < kScanLds ? n - w0 : kScanLds;
+        {
+            // 8-byte words (4 sizes), all of a lane's loads in flight at once:
+            // the table may have just been written by other CUs, so each
+            // round trip is long
+            const int64_t nw = words ? m / 4 : 0;
+            const uint2* wv = reinterpret_cast<const uint2*>(sizes + w0);
+            constexpr int kU = kScanLds / 4 / 1024;
+            uint2 v[kU];
 #pragma unroll
-        for (int k = 0; k < kU; ++k) {
-            const int64_t i = tid + 1024 * k;
-            v[k] = i < nw ? w[i] : make_uint2(0u, 0u);
-        }
+            for (int k = 0; k < kU; ++k) {
+                const int64_t i = tid + 1024 * k;
+                v[k] = i < nw ? wv[i] : make_uint2(0u, 0u);
+            }
 #pragma unroll
-        for (int k = 0; k < kU; ++k) {
-            const int64_t i = tid + 1024 * k;
-            if (i < nw) reinterpret_cast<uint2*>(S.sz)[i] = v[k];
+            for (int k = 0; k < kU; ++k) {
+                const int64_t i = tid + 1024 * k;
+                if (i < nw) reinterpret_cast<uint2*>(S.sz)[i] = v[k];
+            }
+            for (int64_t i = 4 * nw + tid; i < m; i += 1024) S.sz[i] = sizes[w0 + i];
         }
-        for (int64_t i = 4 * nw + tid; i < n; i += 1024) S.sz[i] = sizes[i];
         __syncthreads();
-    }
-    const uint16_t* sz = staged ? S.sz : sizes;
-    const int64_t per = (n + 1023) / 1024;
-    const int64_t lo = tid * per, hi = lo + per < n ? lo + per : n;
-    int64_t sum = 0;
-    for (int64_t i = lo; i < hi; ++i) sum += sz[i];
-    int64_t x = sum;
+        const int64_t per = (m + 1023) / 1024;
+        const int64_t lo = tid * per, hi = lo + per < m ? lo + per : m;
+        int64_t sum = 0;
+        for (int64_t i = lo; i < hi; ++i) sum += S.sz[i];
+        int64_t x = sum;
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int64_t y = __shfl_up(x, d, 64);
-        if (lane >= d) x += y;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int64_t y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
+        }
+        if (lane == 63) S.w[wave] = x;
+        __syncthreads();
+        int64_t run = carry + x - sum, tot = 0;
+        for (int w = 0; w < 16; ++w) {
+            run += w < wave ? S.w[w] : 0;
+            tot += S.w[w];
+        }
+        for (int64_t i = lo; i < hi; ++i) {
+            offsets[w0 + i] = run;
+            run += S.sz[i];
+        }
+        carry += tot;
+        __syncthreads();  // the next window reuses S
     }
-    if (lane == 63) S.w[wave] = x;
-    __syncthreads();
-    int64_t run = x - sum, tot = 0;
-    for (int w = 0; w < 16; ++w) {
-        run += w < wave ? S.w[w] : 0;
-        tot += S.w[w];
-    }
-    S.base[tid] = run;
-    for (int64_t i = lo; i < hi; ++i) {
-        offsets[i] = run;
-        run += sz[i];
-    }
-    *total = tot;
-    __syncthreads();
-    return sz;
 }
 
 // compress_merge body: zero pad to 8 bytes, then flags, range, size table,
@@ -619,8 +618,7 @@ __global__ void batch_merge_kernel(const BatchField* __restrict__ fields, const 
 // exclusive offsets of the payloads from the size table; one workgroup
 __device__ __forceinline__ void size_scan(const uint16_t* __restrict__ sizes, int64_t n, int64_t* __restrict__ offsets) {
     __shared__ ScanLds S;
-    int64_t total;
-    scan_runs(sizes, n, S, offsets, &total);
+    scan_sizes(sizes, n, S, offsets);
 }
 
 // also clears the decoder's error words (the 64-byte slot before the offsets)
@@ -1122,24 +1120,14 @@ int64_t pa::batch_merge_blocks(const pa_layout* L, int dtype) {
     return blocks < 1 ? 1 : (blocks > 2048 ? 2048 : blocks);
 }
 
-int pa::batch_encode(const BatchDev& b, hipStream_t st) {
-    int e = batch_encode_chunks(b, st);
-    if (e != PA_OK) return e;
-    hipLaunchKernelGGL(batch_merge_kernel, dim3((unsigned)b.n_merge), dim3(256), 0, st, b.fields, b.p_merge, b.nf);
-    PA_LAUNCH_CHECK();
-    return PA_OK;
-}
-
 template <int kW>
 static void launch_batch_encode(const BatchDev& b, hipStream_t st) {
     hipLaunchKernelGGL(batch_encode_kernel<kW>, dim3((unsigned)(b.n_chunk / kW)), dim3(64 * kW), 0, st, b.fields,
                        b.p_chunk, b.nf);
 }
 
-int pa::batch_encode_chunks(const BatchDev& b, hipStream_t st) {
-    hipLaunchKernelGGL(batch_encode_clear_kernel, dim3((unsigned)((b.n_chunk + b.nf + 255) / 256)), dim3(256), 0, st,
-                       b.fields, b.p_chunk, b.nf, b.n_chunk);
-    PA_LAUNCH_CHECK();
+// the encoder launch (its status words cleared by the front launch of the run)
+static int batch_encode_only(const BatchDev& b, hipStream_t st) {
     if (b.n_chunk <= 0) return PA_OK;
     switch (b.encode_waves) {
         case 2: launch_batch_encode<2>(b, st); break;
@@ -1148,4 +1136,19 @@ int pa::batch_encode_chunks(const BatchDev& b, hipStream_t st) {
     }
     PA_LAUNCH_CHECK();
     return PA_OK;
+}
+
+int pa::batch_encode(const BatchDev& b, hipStream_t st) {
+    int e = batch_encode_only(b, st);
+    if (e != PA_OK) return e;
+    hipLaunchKernelGGL(batch_merge_kernel, dim3((unsigned)b.n_merge), dim3(256), 0, st, b.fields, b.p_merge, b.nf);
+    PA_LAUNCH_CHECK();
+    return PA_OK;
+}
+
+int pa::batch_encode_chunks(const BatchDev& b, hipStream_t st) {
+    hipLaunchKernelGGL(batch_encode_clear_kernel, dim3((unsigned)((b.n_chunk + b.nf + 255) / 256)), dim3(256), 0, st,
+                       b.fields, b.p_chunk, b.nf, b.n_chunk);
+    PA_LAUNCH_CHECK();
+    return batch_encode_only(b, st);
 }

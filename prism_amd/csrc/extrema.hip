@@ -89,33 +89,6 @@ __global__ void extrema_finish_kernel(const unsigned long long* keys, double* ou
     out[1] = key_value(keys[1]);
 }
 
-// ----------------------------------------------------------------- batch --
-__global__ void batch_extrema_init_kernel(BatchField* fields, int nf) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nf) return;
-    unsigned long long* keys = reinterpret_cast<unsigned long long*>(fields[f].minmax + 2);
-    keys[0] = ~0ull;
-    keys[1] = 0ull;
-}
-
-__global__ __launch_bounds__(256) void batch_extrema_kernel(BatchField* fields, const int64_t* prefix, int nf,
-                                                            int64_t span) {
-    const int f = find_field(prefix, 0, nf, blockIdx.x);
-    const BatchField& F = fields[f];
-    unsigned long long* keys = reinterpret_cast<unsigned long long*>(F.minmax + 2);
-    const int64_t wg = blockIdx.x - prefix[f];
-    if (F.dtype == PA_F32) extrema_span<float>((const float*)F.data, F.L.n, wg, span, keys);
-    else extrema_span<double>((const double*)F.data, F.L.n, wg, span, keys);
-}
-
-__global__ void batch_extrema_finish_kernel(BatchField* fields, int nf) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nf) return;
-    const unsigned long long* keys = reinterpret_cast<const unsigned long long*>(fields[f].minmax + 2);
-    fields[f].minmax[0] = key_value(keys[0]);
-    fields[f].minmax[1] = key_value(keys[1]);
-}
-
 }  // namespace
 }  // namespace pa
 
@@ -154,16 +127,3 @@ int64_t pa::batch_ext_blocks(const pa_layout* L, int dtype, int ext_batches) {
     return nv / (ext_batches * kExtBatch) + 1;  // the last workgroup also takes the scalar tail
 }
 
-int pa::batch_extrema(const BatchDev& b, hipStream_t st) {
-    const int fb = (b.nf + 255) / 256;
-    hipLaunchKernelGGL(batch_extrema_init_kernel, dim3(fb), dim3(256), 0, st, b.fields, b.nf);
-    PA_LAUNCH_CHECK();
-    if (b.n_ext > 0) {
-        hipLaunchKernelGGL(batch_extrema_kernel, dim3((unsigned)b.n_ext), dim3(256), 0, st, b.fields, b.p_ext, b.nf,
-                           (int64_t)b.ext_batches * kExtBatch);
-        PA_LAUNCH_CHECK();
-    }
-    hipLaunchKernelGGL(batch_extrema_finish_kernel, dim3(fb), dim3(256), 0, st, b.fields, b.nf);
-    PA_LAUNCH_CHECK();
-    return PA_OK;
-}

@@ -1769,16 +1769,6 @@ __device__ int64_t fill_bounds(const double* minmax, double eb_in, int rel, int 
     return status;
 }
 
-__global__ void batch_eb_kernel(BatchField* fields, int nf) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nf) return;
-    BatchField& F = fields[f];
-    const int64_t status = fill_bounds(F.minmax, F.eb, F.rel, F.dtype, F.alpha, F.ebinfo);
-    F.result[1] = (uint64_t)status;
-    F.result[2] = (uint64_t)__double_as_longlong(F.ebinfo[1]);
-    F.result[3] = (uint64_t)__double_as_longlong(F.ebinfo[0]);
-}
-
 __global__ void level_bounds_kernel(const double* minmax, double eb, int dtype, double alpha, double* bounds) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     bounds[14] = (double)fill_bounds(minmax, eb, minmax != nullptr, dtype, alpha, bounds);
@@ -1786,36 +1776,98 @@ __global__ void level_bounds_kernel(const double* minmax, double eb, int dtype, 
 }
 
 
-// tuner over the sample blocks of fields [lo, hi) (one dtype); grid.y = candidate
-template <typename T>
-__global__ __launch_bounds__(kThreads) void batch_tune_kernel(const BatchField* __restrict__ fields,
-                                                              const int64_t* __restrict__ prefix, int lo, int hi,
-                                                              int64_t base) {
-    __shared__ T tile[kTileVals];
+// The batch's work before the bounds are known, in one launch: the tuner's
+// (sample block, candidate) pairs of every field (each candidate's blocks
+// one after the other, as the grid of tune_kernel), the extrema spans, the
+// zeroing of the code arrays' gaps and the clearing of the encoder's
+// look-back status words (one lane per launch slot of a chunk, then one per
+// field for its error word).  None reads what another writes.  The tile is
+// sized for a float64 tuner block; a float32 block uses half of it.
+__global__ __launch_bounds__(kThreads) void batch_front_kernel(const BatchField* __restrict__ fields,
+                                                               const int64_t* __restrict__ p_tune,
+                                                               const int64_t* __restrict__ p_ext,
+                                                               const int64_t* __restrict__ p_gap,
+                                                               const int64_t* __restrict__ p_chunk, int nf,
+                                                               int64_t n_tune, int64_t n_ext, int64_t n_gap,
+                                                               int64_t n_chunk, int64_t span) {
+    __shared__ double tile[kTileVals];
     __shared__ double red[kThreads];
-    const int64_t gb = base + blockIdx.x;
-    const int f = find_field(prefix, lo, hi, gb);
+    int64_t b = blockIdx.x;
+    if (b < 15 * n_tune) {
+        const int cand = (int)(b / n_tune);
+        const int64_t u = b - cand * n_tune;
+        const int f = find_field(p_tune, 0, nf, u);
+        const BatchField& F = fields[f];
+        const int bid = (int)(u - p_tune[f]);
+        if (F.dtype == PA_F32)
+            tune_block<float>(static_cast<const float*>(F.data), F.g, F.tsx0, F.tsy0, F.tsz0, F.tnsx, F.tnsy, bid,
+                              cand, F.tune_part, reinterpret_cast<float*>(tile), red);
+        else
+            tune_block<double>(static_cast<const double*>(F.data), F.g, F.tsx0, F.tsy0, F.tsz0, F.tnsx, F.tnsy, bid,
+                               cand, F.tune_part, tile, red);
+        return;
+    }
+    b -= 15 * n_tune;
+    if (b < n_ext) {
+        const int f = find_field(p_ext, 0, nf, b);
+        const BatchField& F = fields[f];
+        unsigned long long* keys = reinterpret_cast<unsigned long long*>(F.minmax + 2);
+        const int64_t wg = b - p_ext[f];
+        if (F.dtype == PA_F32) extrema_span<float>(static_cast<const float*>(F.data), F.L.n, wg, span, keys);
+        else extrema_span<double>(static_cast<const double*>(F.data), F.L.n, wg, span, keys);
+        return;
+    }
+    b -= n_ext;
+    if (b < n_gap) {
+        const int f = find_field(p_gap, 0, nf, b);
+        const BatchField& F = fields[f];
+        zero_gaps(F.qc, F.gap_lo, F.gap_hi, b - p_gap[f], p_gap[f + 1] - p_gap[f]);
+        return;
+    }
+    b -= n_gap;
+    const int64_t i = b * kThreads + threadIdx.x;
+    if (i >= n_chunk + nf) return;
+    const int64_t gid = i < n_chunk ? i : 0;
+    const int f = i < n_chunk ? find_field(p_chunk, 0, nf, gid) : (int)(i - n_chunk);
     const BatchField& F = fields[f];
-    tune_block<T>(static_cast<const T*>(F.data), F.g, F.tsx0, F.tsy0, F.tsz0, F.tnsx, F.tnsy,
-                  (int)(gb - prefix[f]), blockIdx.y, F.tune_part, tile, red);
+    const int64_t k = i < n_chunk ? gid - p_chunk[f] : -1;  // status word k (-1: the error word)
+    if (k < F.cg.nchunks) F.encws[1 + k] = 0ull;
 }
 
-__global__ void batch_pick_kernel(BatchField* fields, int nf) {
+// Per field, after the front launch: min and max from the extrema keys
+// (then reset for the next run), the absolute bound and its level bounds,
+// the tuner's pick (all flags false where the field has no tuner samples)
+// and the field's result words.
+__global__ __launch_bounds__(kThreads) void batch_finish_kernel(BatchField* fields) {
+    __shared__ double s[kPickLds];
+    __shared__ double e[15];
+    BatchField& F = fields[blockIdx.x];
+    if (F.tune_ok) {
+        pick_parallel(F.tune_part, F.tnblocks, F.scores, F.flags, s, e);
+    } else {
+        if (threadIdx.x < 8) F.flags[threadIdx.x] = 0;
+        if (threadIdx.x < 15) F.scores[threadIdx.x] = 0.0;
+    }
+    if (threadIdx.x != 0) return;
+    unsigned long long* keys = reinterpret_cast<unsigned long long*>(F.minmax + 2);
+    F.minmax[0] = key_value(keys[0]);
+    F.minmax[1] = key_value(keys[1]);
+    keys[0] = ~0ull;
+    keys[1] = 0ull;
+    const int64_t status = fill_bounds(F.minmax, F.eb, F.rel, F.dtype, F.alpha, F.ebinfo);
+    F.result[1] = (uint64_t)status;
+    F.result[2] = (uint64_t)__double_as_longlong(F.ebinfo[1]);
+    F.result[3] = (uint64_t)__double_as_longlong(F.ebinfo[0]);
+}
+
+// the extrema keys of every field at their start values (once per plan; the
+// finish kernel resets them after each run)
+__global__ void batch_keys_init_kernel(BatchField* fields, int nf) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nf) return;
-    BatchField& F = fields[f];
-    if (F.tune_ok) {
-        pick_flags(F.tune_part, F.tnblocks, F.scores, F.flags);
-    } else {
-        for (int i = 0; i < 8; ++i) F.flags[i] = 0;
-        for (int i = 0; i < 15; ++i) F.scores[i] = 0.0;
-    }
-}
-
-__global__ void batch_gaps_kernel(const BatchField* __restrict__ fields, const int64_t* __restrict__ prefix, int nf) {
-    const int f = find_field(prefix, 0, nf, blockIdx.x);
-    const BatchField& F = fields[f];
-    zero_gaps(F.qc, F.gap_lo, F.gap_hi, blockIdx.x - prefix[f], prefix[f + 1] - prefix[f]);
+    unsigned long long* keys = reinterpret_cast<unsigned long long*>(fields[f].minmax + 2);
+    keys[0] = ~0ull;
+    keys[1] = 0ull;
 }
 
 // Compression tiles of fields [lo, hi) (one dtype), with the XCD-aware
@@ -2161,27 +2213,22 @@ void pa::batch_predict_setup(BatchField& f) {
     f.tnblocks = f.tune_ok ? n[0] * n[1] * n[2] : 0;
 }
 
-int pa::batch_predict(const BatchDev& b, hipStream_t st) {
-    const unsigned fb = (unsigned)((b.nf + 63) / 64);
-    hipLaunchKernelGGL(batch_eb_kernel, dim3(fb), dim3(64), 0, st, b.fields, b.nf);
+int pa::batch_front(const BatchDev& b, hipStream_t st) {
+    const int64_t clr = (b.n_chunk + b.nf + kThreads - 1) / kThreads;
+    const int64_t blocks = 15 * b.n_tune + b.n_ext + b.n_gap + clr;
+    hipLaunchKernelGGL(batch_front_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, st, b.fields, b.p_tune, b.p_ext,
+                       b.p_gap, b.p_chunk, b.nf, b.n_tune, b.n_ext, b.n_gap, b.n_chunk,
+                       (int64_t)b.ext_batches * kExtBatch);
     PA_LAUNCH_CHECK();
-    if (b.n_tune32 > 0) {
-        hipLaunchKernelGGL(batch_tune_kernel<float>, dim3((unsigned)b.n_tune32, 15), dim3(kThreads), 0, st, b.fields,
-                           b.p_tune, 0, b.nf32, (int64_t)0);
-        PA_LAUNCH_CHECK();
-    }
-    if (b.n_tune - b.n_tune32 > 0) {
-        hipLaunchKernelGGL(batch_tune_kernel<double>, dim3((unsigned)(b.n_tune - b.n_tune32), 15), dim3(kThreads), 0,
-                           st, b.fields, b.p_tune, b.nf32, b.nf, b.n_tune32);
-        PA_LAUNCH_CHECK();
-    }
-    hipLaunchKernelGGL(batch_pick_kernel, dim3(fb), dim3(64), 0, st, b.fields, b.nf);
+    hipLaunchKernelGGL(batch_finish_kernel, dim3((unsigned)b.nf), dim3(kThreads), 0, st, b.fields);
     PA_LAUNCH_CHECK();
-    if (b.n_gap > 0) {
-        hipLaunchKernelGGL(batch_gaps_kernel, dim3((unsigned)b.n_gap), dim3(256), 0, st, b.fields, b.p_gap, b.nf);
-        PA_LAUNCH_CHECK();
-    }
-    return batch_spline(b, st);
+    return PA_OK;
+}
+
+int pa::batch_keys_init(const BatchDev& b, hipStream_t st) {
+    hipLaunchKernelGGL(batch_keys_init_kernel, dim3((unsigned)((b.nf + 255) / 256)), dim3(256), 0, st, b.fields, b.nf);
+    PA_LAUNCH_CHECK();
+    return PA_OK;
 }
 
 int pa::batch_spline(const BatchDev& b, hipStream_t st) {

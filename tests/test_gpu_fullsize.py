@@ -5,10 +5,15 @@ random field at rel 1e-4.
 
 - bit-planes survive the codec: decode(stream) equals the shuffled planes;
 - codes survive the shuffle: inverse_shuffle(planes) equals the codes;
-- every progressive level is within its bound (float64: exactly; float32:
-  plus 4 units of float32 epsilon times the field's magnitude per round,
-  the rounding of the method's own float32 sums);
-- a plain (SM) stream decompresses within its bound.
+- every progressive level is within its bound.  Float64: exactly.
+  Float32: the method's own float32 arithmetic (the prediction and
+  pred + code * 2eb in the quantizer, then one delta sum per round) rounds
+  past the bound by a fraction of a unit in the last place; measured on the
+  Gaussian random field: 1.00026 x the bound for a plain decompression and
+  1.00066 x after the fourth progressive round.  The test prints each
+  level's L-inf / bound and allows (k + 1) units of float32 epsilon times
+  the field's magnitude after round k (1.0 in the plain case);
+- a plain (SM) stream decompresses within its bound (same allowance).
 """
 import ctypes as C
 import sys
@@ -71,12 +76,19 @@ def test_full_size_properties(mods, case):
     r = pipeline.Retriever(c)
     for k, t in enumerate(targets):
         out = r.retrieve(t)
-        err = pipeline.quality(f, out)["linf"]
-        assert err <= t * c.range + 4 * (k + 1) * eps * mag, (case, t, err, t * c.range)
+        q = pipeline.quality(f, out)
+        err, bound = q["linf"], t * c.range
+        over = "" if err <= bound or eps == 0 else f"  (over by {(err - bound) / (eps * mag):.2f} eps x max|f|)"
+        print(f"LEVEL {case} target {t:g}: linf / bound = {err / bound:.8f}{over}", flush=True)
+        assert q["sse"] == q["sse"], "NaN in the reconstruction"
+        assert err <= bound + (k + 1) * eps * mag, (case, t, err, bound)
     del r, out, parts
     announce("full size plain stream", case)
     c2, parts2 = pipeline.compress(f, rel, rel=True, progressive=False, return_parts=True)
     check_round_trips(_lib, pipeline, c2, parts2)
     out = pipeline.decompress(c2)
-    err = pipeline.quality(f, out)["linf"]
-    assert err <= rel * c2.range + 4 * eps * mag, (case, err, rel * c2.range)
+    q = pipeline.quality(f, out)
+    err, bound = q["linf"], rel * c2.range
+    print(f"LEVEL {case} plain: linf / bound = {err / bound:.8f}", flush=True)
+    assert q["sse"] == q["sse"], "NaN in the reconstruction"
+    assert err <= bound + eps * mag, (case, err, bound)
