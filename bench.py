@@ -407,6 +407,12 @@ def serving_run(comp, reqs):
             "wall_s": wall, "throughput_req_s": len(done) / wall, "GBps": nbytes / wall / 1e9,
             "algorithmic_bytes": nbytes, "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "mean_ms": float(lat.mean()),
             "max_ms": float(lat.max()), "by_kind": by, "batches": srv.batches,
+            # host time of the loop per completed request (admission, job
+            # tables and launches; completion and hand-over), and the share
+            # of the wall time the device spent on batches
+            "host_us_per_request": 1e6 * (srv.host_launch_s + srv.host_poll_s) / max(len(done), 1),
+            "host_launch_us_per_request": 1e6 * srv.host_launch_s / max(len(done), 1),
+            "device_busy_fraction": srv.device_s / wall,
             "mean_batch": len(done) / max(srv.batches, 1), "resident_fields": len(srv.fields),
             "resident_bytes": srv.resident_bytes, "held_bytes_at_end": srv.held_bytes, "budget_bytes": srv.budget}
 

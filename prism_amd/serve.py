@@ -120,6 +120,12 @@ class Server:
         self.completed: list[Request] = []
         self.finished = 0  # requests finished so far, whichever way they were handed over
         self.batches = 0
+        # host seconds spent admitting and launching batches and completing
+        # them, and device seconds of the completed batches (start to end
+        # event of each batch on the stream)
+        self.host_launch_s = 0.0
+        self.host_poll_s = 0.0
+        self.device_s = 0.0
 
     def _finish(self, req: Request) -> None:
         self.finished += 1
@@ -198,6 +204,7 @@ class Server:
         launch slot is free.  Returns the number of requests launched."""
         self.poll()
         launched = 0
+        t0 = time.perf_counter()
         while self.queue and len(self.inflight) < self.max_inflight:
             batch, need = self._admit()
             if not batch:
@@ -207,6 +214,8 @@ class Server:
                 break
             self._launch(batch, need)
             launched += len(batch)
+        if launched:
+            self.host_launch_s += time.perf_counter() - t0
         return launched
 
     def _admit(self):
@@ -233,6 +242,8 @@ class Server:
 
     def _launch(self, batch, need):
         now = self.clock()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev0.record()
         rets = [r for r in batch if r.kind == "retrieve"]
         comps = [r for r in batch if r.kind == "compress"]
         keep = []
@@ -255,19 +266,21 @@ class Server:
                                           progressive=True, graph=False)
             bc.enqueue()
             keep.append(bc)
-        ev = torch.cuda.Event()
+        ev = torch.cuda.Event(enable_timing=True)
         ev.record()
         for r in batch:
             r.state, r.admitted = "running", now
         self.transient_bytes += need
-        self.inflight.append((ev, batch, need, keep, comps, bc, rets, rb))
+        self.inflight.append((ev, batch, need, keep, comps, bc, rets, rb, ev0))
         self.batches += 1
 
     def poll(self) -> int:
         """Marks the requests of completed batches done (in stream order)."""
         n = 0
         while self.inflight and self.inflight[0][0].query():
-            ev, batch, need, keep, comps, bc, rets, rb = self.inflight.popleft()
+            t0 = time.perf_counter()
+            ev, batch, need, keep, comps, bc, rets, rb, ev0 = self.inflight.popleft()
+            self.device_s += ev0.elapsed_time(ev) * 1e-3
             now = self.clock()
             if bc is not None:
                 res = bc.results(strict=False)
@@ -291,6 +304,7 @@ class Server:
                 r.done = now
                 self._finish(r)
             n += len(batch)
+            self.host_poll_s += time.perf_counter() - t0
         return n
 
     def drain(self) -> None:
