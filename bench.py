@@ -206,7 +206,8 @@ def progressive_workload(field, rel_eb, targets, steps, warmup):
         q = pipeline.quality(field, out)
         nread = int(r.bytes_read.item())
         levels.append({"target": t, "bound_abs": t * c.range, "linf": q["linf"],
-                       "int16_codes": bool(int(r.narrow.item())),
+                       # how the round's codes reached the reconstruction
+                       "code_source": ("int32", "int16", "planes")[int(r.narrow.item())],
                        "linf_over_bound": q["linf"] / (t * c.range), "within_bound": q["linf"] <= t * c.range,
                        "sse_finite": math.isfinite(q["sse"]), "psnr_db": q["psnr_db"], "nrmse": q["nrmse"],
                        "payload_bytes_read": nread - prev, "ms": lat,
@@ -234,11 +235,16 @@ def progressive_workload(field, rel_eb, targets, steps, warmup):
         rnd = levels[k % nrounds]
         if name == "decode":
             return rnd["payload_bytes_read"] + planes_bytes(rnd)
-        code_b = 2 if rnd["int16_codes"] else 4  # the round's code hand-off width
+        # the round's code hand-off (int32 or int16 codes), or none: the
+        # reconstruction then reads the window's planes itself and the
+        # inverse shuffle only its window
+        src = rnd["code_source"]
+        code_b = {"int32": 4, "int16": 2, "planes": 0}[src]
         if name == "unshuffle":
-            return planes_bytes(rnd) + code_b * L.code_len
+            return 0 if src == "planes" else planes_bytes(rnd) + code_b * L.code_len
         if name == "reconstruct":
-            return code_b * (L.n - L.n4) + L.n * s + (L.n * s if k % nrounds else L.n4 * s)
+            codes_in = planes_bytes(rnd) if src == "planes" else code_b * (L.n - L.n4)
+            return codes_in + L.n * s + (L.n * s if k % nrounds else L.n4 * s)
         return 0
 
     stats = {}

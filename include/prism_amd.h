@@ -147,6 +147,23 @@ int pa_spline_delta_reconstruct_narrow(const void* d_qc, const int32_t* d_narrow
                                        double eb, double rel_eb, const uint8_t* d_flags8,
                                        const int32_t* d_maxlevel, const void* d_old, void* d_out, void* stream);
 
+/* Progressive round without a code hand-off (no reference counterpart: the
+ * inverse shuffle, bitplane.cu:645-667, folded into the reconstruction,
+ * InterpolationPredictor.cu:2209-2239).  The deferred inverse shuffle acts
+ * as the _narrow one, except that for a 3-D field (nz > 1) whose window
+ * holds at most 8 planes on every level it writes no codes and sets *d_src
+ * to 2; pa_spline_reconstruct_planes then reads each tile's codes from the
+ * decoded planes d_bp (L->host_size bytes) under the window d_begin/d_end,
+ * and otherwise from d_qc (int32 or int16 by *d_src, as the _narrow
+ * reconstructions).  Exactly one of d_anchors (a field's first round) and
+ * d_old (the delta form) is given.  Results equal the int32 path's. */
+int pa_inverse_bitshuffle_window_deferred(const uint8_t* d_bp, const pa_layout* L, const int32_t* d_begin,
+                                          const int32_t* d_end, void* d_qc, int32_t* d_src, void* stream);
+int pa_spline_reconstruct_planes(const void* d_qc, const int32_t* d_src, const uint8_t* d_bp,
+                                 const int32_t* d_begin, const int32_t* d_end, const void* d_anchors,
+                                 const void* d_old, int dtype, const pa_layout* L, double eb, double rel_eb,
+                                 const uint8_t* d_flags8, const int32_t* d_maxlevel, void* d_out, void* stream);
+
 /* lossless_encode (zbpc.cu:476-520, kernel zbpc_encode 155-263): writes
  * header, size table and payloads to d_out (capacity >= pa_compress_bound),
  * the 4x32 cumulative size table to d_cum and the unpadded stream length to

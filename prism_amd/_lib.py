@@ -99,6 +99,8 @@ SIGNATURES = {
     "pa_inverse_bitshuffle_window_narrow": (I32, [P, LP, P, P, P, P, P]),
     "pa_spline_reconstruct_narrow": (I32, [P, P, P, I32, LP, D, D, P, P, P, P]),
     "pa_spline_delta_reconstruct_narrow": (I32, [P, P, I32, LP, D, D, P, P, P, P, P]),
+    "pa_inverse_bitshuffle_window_deferred": (I32, [P, LP, P, P, P, P, P]),
+    "pa_spline_reconstruct_planes": (I32, [P, P, P, P, P, P, P, I32, LP, D, D, P, P, P, P]),
     "pa_encode_workspace": (SZ, [LP]),
     "pa_lossless_encode": (I32, [P, LP, P, P, P, P, SZ, P]),
     "pa_compress_merge": (I32, [P, LP, I32, P, P, P, P, P, P, P]),

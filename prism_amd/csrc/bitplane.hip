@@ -234,16 +234,27 @@ __device__ __forceinline__ void unshuffle_block(const uint8_t* __restrict__ bp, 
 }
 
 // NB window inverse that writes int16 codes when the window allows it and
-// records the choice in *narrow for the reconstruction
+// records the choice in *narrow for the reconstruction (0 int32, 1 int16).
+// With `defer`, a window that the reconstruction reads from the bit-planes
+// itself (window_from_planes) writes no codes and records 2.
 __global__ __launch_bounds__(kThreads) void unshuffle_narrow_kernel(const uint8_t* __restrict__ bp, Levels v,
                                                                     const int32_t* __restrict__ begin,
                                                                     const int32_t* __restrict__ end,
                                                                     int32_t* __restrict__ qc,
-                                                                    int32_t* __restrict__ narrow_out) {
+                                                                    int32_t* __restrict__ narrow_out, int defer) {
     __shared__ int32_t stage[kStageWords];
+    if (defer && window_from_planes(begin, end)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) *narrow_out = 2;
+        return;
+    }
     const bool narrow = window_narrow(begin, end);
     if (blockIdx.x == 0 && threadIdx.x == 0) *narrow_out = narrow;
-    unshuffle_block<PA_MODE_NB, true>(bp, v, begin, end, qc, blockIdx.x, stage, narrow);
+    // a deferring launch holds one resident grid, which takes the blocks in
+    // turn (its workgroups then leave at once when the window is narrow)
+    for (int blk = blockIdx.x; blk < v.blk_start[4]; blk += gridDim.x) {
+        unshuffle_block<PA_MODE_NB, true>(bp, v, begin, end, qc, blk, stage, narrow);
+        __syncthreads();  // the next block reuses the stage
+    }
 }
 
 template <int MODE, bool WINDOW>
@@ -261,8 +272,12 @@ __global__ __launch_bounds__(kThreads) void retr_unshuffle_kernel(const RetrJob*
     __shared__ int32_t stage[kStageWords];
     const int j = find_field(prefix, 0, nj, blockIdx.x);
     const RetrJob& J = jobs[j];
-    const bool narrow = window_narrow(J.state, J.state + 4);
     const int blk = (int)(blockIdx.x - prefix[j]);
+    if (J.g.nz > 1 && window_from_planes(J.state, J.state + 4)) {  // the reconstruction reads the planes
+        if (blk == 0 && threadIdx.x == 0) J.state[9] = 2;
+        return;
+    }
+    const bool narrow = window_narrow(J.state, J.state + 4);
     if (blk == 0 && threadIdx.x == 0) J.state[9] = narrow;
     unshuffle_block<PA_MODE_NB, true>(J.bp, J.lv, J.state, J.state + 4, J.qc, blk, stage, narrow);
 }
@@ -341,7 +356,23 @@ extern "C" int pa_inverse_bitshuffle_window_narrow(const uint8_t* d_bp, const pa
     if (!L || !d_qc || !d_bp || !d_begin || !d_end || !d_narrow) return PA_ERR_ARG;
     Levels v = make_levels(L);
     hipLaunchKernelGGL(unshuffle_narrow_kernel, dim3(v.blk_start[4]), dim3(kThreads), 0, as_stream(stream), d_bp, v,
-                       d_begin, d_end, static_cast<int32_t*>(d_qc), d_narrow);
+                       d_begin, d_end, static_cast<int32_t*>(d_qc), d_narrow, 0);
+    PA_LAUNCH_CHECK();
+    return PA_OK;
+}
+
+extern "C" int pa_inverse_bitshuffle_window_deferred(const uint8_t* d_bp, const pa_layout* L,
+                                                     const int32_t* d_begin, const int32_t* d_end, void* d_qc,
+                                                     int32_t* d_src, void* stream) {
+    if (!L || !d_qc || !d_bp || !d_begin || !d_end || !d_src) return PA_ERR_ARG;
+    Levels v = make_levels(L);
+    // only 3-D fields (nz > 1) reconstruct tile by tile from the planes; the
+    // grid is what stays resident (four 33 KB workgroups per CU)
+    const int defer = L->nz > 1 ? 1 : 0;
+    const int resident = 4 * device_cus();
+    const int grid = defer && v.blk_start[4] > resident ? resident : v.blk_start[4];
+    hipLaunchKernelGGL(unshuffle_narrow_kernel, dim3(grid), dim3(kThreads), 0, as_stream(stream), d_bp, v, d_begin,
+                       d_end, static_cast<int32_t*>(d_qc), d_src, defer);
     PA_LAUNCH_CHECK();
     return PA_OK;
 }

@@ -127,6 +127,18 @@ __host__ __device__ inline Levels make_levels(const pa_layout* L) {
     return v;
 }
 
+// A progressive round whose window holds at most kPlaneWindow planes on
+// every level is reconstructed straight from the decoded bit-planes: the
+// deferring inverse shuffle then writes no codes.  Both kernels evaluate
+// this on the same device window.
+constexpr int kPlaneWindow = 8;
+__device__ __forceinline__ bool window_from_planes(const int32_t* begin, const int32_t* end) {
+    bool ok = true;
+#pragma unroll
+    for (int lr = 0; lr < 4; ++lr) ok = ok && end[lr] - begin[lr] <= kPlaneWindow;
+    return ok;
+}
+
 // ------------------------------------------------------------------ batch --
 // One field of a batch: its input, its layout, and the device buffers every
 // stage of the compression reads and writes.  The batched kernels find the
@@ -184,9 +196,9 @@ struct RetrJob {
     const uint64_t* cum;   // trailer: cumulative payload sizes [4][32]
     const uint8_t* flags;  // trailer: flags8
     const void* anchors;   // trailer: anchors
-    int32_t* state;        // begin[4], end[4], maxlevel, narrow (int16 codes this round)
+    int32_t* state;        // begin[4], end[4], maxlevel, code source (0 int32, 1 int16, 2 bit-planes)
     unsigned long long* keys;  // strategy workgroup keys (33 x 33)
-    uint8_t* bp;           // decoded bit-planes (host layout), kept between rounds
+    uint8_t* bp;           // decoded bit-planes (device layout in host_size bytes), kept between rounds
     int32_t* qc;           // codes, code_len
     uint8_t* dws;          // decode workspace: error word (64-byte slot) | offsets
     uint64_t* bytes_read;  // may be null

@@ -916,13 +916,26 @@ __device__ __forceinline__ void zero_gaps(int32_t* p, const int64_t* lo, const i
         for (int64_t i = lo[r] + blk * blockDim.x + threadIdx.x; i < hi[r]; i += nblk * blockDim.x) p[i] = 0;
 }
 
+// The decoded bit-planes of a progressive round (device layout: level l,
+// plane k at base[l] + k * stride[l]) and the round's window, for a
+// reconstruction that reads its codes from them.
+struct PlaneSrc {
+    const uint8_t* bp;
+    const int32_t* begin;  // per level lr = 3 - l, planes [begin, end)
+    const int32_t* end;
+    int64_t base[4], stride[4];
+    int64_t size;  // bytes of bp that may be read
+};
+
 template <typename T>
 struct SplineArgs {
     const T* data;
     Geom g;
     LevelEb<T> le;
     const double* le_dev;  // when set, the level bounds come from this device block instead of `le`
-    const int32_t* narrow;  // when set and non-zero on the device, qc_in holds int16 codes
+    // when set, the device word picks where the codes come from: 0 int32
+    // codes in qc_in, 1 int16 codes in qc_in, 2 the bit-planes of `ps`
+    const int32_t* narrow;
     bool q16;
     const uint8_t* flags8;
     const int32_t* d_maxlevel;
@@ -934,6 +947,8 @@ struct SplineArgs {
     T* out;
     Gaps gaps;    // compression: the gaps of qc_out, zeroed by the same launch
     int gaps_on;  // (its workgroups share them out; disjoint from the codes)
+    PlaneSrc ps;  // decompression: the round's bit-planes (ps.bp null: none)
+    bool planes;  // resolved: the codes come from ps
 };
 
 // Compression launches clear the code array's gaps alongside the tiles.
@@ -1007,6 +1022,153 @@ __device__ __forceinline__ void codes_store(T* tile, const TileCodes<T>& c) {
     if (c.aslot >= 0) tile[c.aslot] = c.anc;
 }
 
+// ------------------------------------------ codes from the bit-planes --
+// A progressive round whose window is narrow reads each tile's codes from
+// the decoded bit-planes instead of a code hand-off: the inverse shuffle
+// (bitplane.cu:645-667) folded into the reconstruction's code load
+// (InterpolationPredictor.cu:2209-2239).  Lane l takes rows l and 256 + l of
+// the tile's 404 code rows, in the row table's order.  A row's codes are
+// adjacent in its level's segment, so on each plane of the window they lie
+// in at most 3 adjacent bytes (17 codes from any bit offset): two aligned
+// 4-byte loads, a funnel shift, and the bit order reversed within each
+// byte put code m of the run at bit m.  Code m gathers bit m of each window
+// plane k at bit 31 - k, and the negabinary inverse of that word is the
+// round's code, as the inverse shuffle computes it.
+struct PlaneRow {
+    int row;     // row (j, k) = (row mod E, row / E) of the level's lattice
+    bool full;   // the row holds every i, or the odd i only
+    int nv;      // codes of the row inside the field, read
+    int sh, np;  // bit offset of code 0 in its byte; planes read
+    uint32_t w[kPlaneWindow];
+};
+
+// 8 bytes at a 4-byte aligned address in one load
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef u32x2 u32x2_a4 __attribute__((aligned(4)));
+
+template <int L, typename T, bool IN>
+__device__ __forceinline__ void plane_row_load(const Ctx<T, IN>& t, const PlaneSrc& ps, const int32_t* tab, int row,
+                                               int maxlevel, int b, int e, PlaneRow& r) {
+    constexpr int U = 1 << L, E = 16 / U + 1;
+    const int j = row % E, k = row / E;
+    r.row = row;
+    r.full = ((j | k) & 1) != 0;
+    const int n = r.full ? E : (E - 1) / 2;
+    r.nv = 0;
+    if (L <= maxlevel && t.template inf<1>(t.oy + U * j) && t.template inf<2>(t.oz + U * k)) {
+        if (IN) {
+            r.nv = n;
+        } else {  // i = m (full) or 2m + 1 (odd i) with ox + U i < nx
+            const int ilim = (t.g.nx - t.ox + U - 1) >> L;
+            r.nv = min(n, r.full ? ilim : ilim >> 1);
+        }
+    }
+    r.np = r.nv > 0 && e > b ? e - b : 0;
+    r.sh = 0;
+#pragma unroll
+    for (int q = 0; q < kPlaneWindow; ++q) r.w[q] = 0u;
+    if (r.np == 0) return;
+    // the run's first code, relative to the level's segment (32-bit, as the
+    // row table: the code array is shorter than 2^31)
+    const uint32_t start = (uint32_t)tab[row_off<L>() + row] + (uint32_t)(r.full ? t.ox >> L : t.ox >> (L + 1));
+    const uint32_t rel = start - (uint32_t)t.g.prefix[L];
+    r.sh = (int)(rel & 7u);
+    const uint32_t byte = rel >> 3;
+    const int64_t stride = ps.stride[L];
+    const uint8_t* p0 = ps.bp + ps.base[L] + (byte & ~3u) + (int64_t)b * stride;
+    // the low word lies inside plane k's row of bytes (stride is a multiple
+    // of 8 and byte < stride); the high one may pass its end, never the buffer's
+    const uint8_t* last = ps.bp + ps.size - 8;
+#pragma unroll
+    for (int q = 0; q < kPlaneWindow; ++q) {
+        if (q < r.np) {
+            const uint8_t* p = p0 + q * stride;
+            uint32_t lo, hi = 0u;
+            if (p <= last) {
+                const u32x2 v = *reinterpret_cast<const u32x2_a4*>(p);
+                lo = v.x;
+                hi = v.y;
+            } else {
+                lo = *reinterpret_cast<const uint32_t*>(p);
+            }
+            r.w[q] = __builtin_amdgcn_alignbyte(hi, lo, byte & 3u);
+        }
+    }
+}
+
+template <int L, typename T>
+__device__ __forceinline__ void plane_row_store(T* s, const PlaneRow& r, int b) {
+    constexpr int U = 1 << L, E = 16 / U + 1;
+    const int j = r.row % E, k = r.row / E;
+    // the digits of codes 4c .. 4c + 3 in bytes 0..3 of dg[c], digit q (plane
+    // b + q) at bit 7 - q of its byte: x * 0x204081 puts bit m of a 4-bit x
+    // at bit 8m (its four shifted copies do not overlap, so nothing carries)
+    constexpr int G = (E + 3) / 4;
+    uint32_t dg[G];
+#pragma unroll
+    for (int c = 0; c < G; ++c) dg[c] = 0u;
+#pragma unroll
+    for (int q = 0; q < kPlaneWindow; ++q) {
+        if (q < r.np) {
+            const uint32_t w = __builtin_bitreverse32(__builtin_bswap32(r.w[q])) >> r.sh;
+#pragma unroll
+            for (int c = 0; c < G; ++c) dg[c] |= ((((w >> (4 * c)) & 0xFu) * 0x204081u) & 0x01010101u) << (7 - q);
+        }
+    }
+    // digit q belongs at bit 31 - b - q: the code's byte to the top, then
+    // right by b (b < 32 wherever a plane was read; else the byte is 0)
+    const int sb = b & 31;
+    const int n = r.full ? E : (E - 1) / 2;
+    const int first = sidx(r.full ? 0 : U, U * j, U * k), step = r.full ? U : 2 * U;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        if (m < n) {
+            const uint32_t top = __builtin_amdgcn_perm(0u, dg[m >> 2], 0x000C0C0Cu | ((uint32_t)(m & 3) << 24));
+            s[first + step * m] = code_slot(m < r.nv ? nb_inv(top >> sb) : 0, T());
+        }
+    }
+}
+
+// Every code slot and corner anchor of the tile from the bit-planes.  The
+// first rows of all lanes are level 0's; the second rows are the rest of
+// level 0, then levels 1-3 (each level's rows with its lattice sizes known
+// at compile time).
+template <typename T, bool IN>
+__device__ __forceinline__ void plane_codes(const Ctx<T, IN>& t, const PlaneSrc& ps, const int32_t* rowtab,
+                                            int maxlevel, const T* anchors_in, int bx, int by, int bz) {
+    const Geom& g = t.g;
+    int wb[4], we[4];  // per level lr = 3 - L
+#pragma unroll
+    for (int lr = 0; lr < 4; ++lr) {
+        wb[lr] = ps.begin[lr];
+        we[lr] = ps.end[lr];
+    }
+    const int lane = lane_id();
+    constexpr int k1 = kRows0 + kRows1, k2 = k1 + kRows2;
+    const int rg = lane + kThreads;
+    PlaneRow r0, r1;
+    plane_row_load<0>(t, ps, rowtab, lane, maxlevel, wb[3], we[3], r0);
+    if (rg < kRows0) plane_row_load<0>(t, ps, rowtab, rg, maxlevel, wb[3], we[3], r1);
+    else if (rg < k1) plane_row_load<1>(t, ps, rowtab, rg - kRows0, maxlevel, wb[2], we[2], r1);
+    else if (rg < k2) plane_row_load<2>(t, ps, rowtab, rg - k1, maxlevel, wb[1], we[1], r1);
+    else if (rg < kRowTab) plane_row_load<3>(t, ps, rowtab, rg - k2, maxlevel, wb[0], we[0], r1);
+    T anc = (T)0;
+    int aslot = -1;
+    if (threadIdx.x < 8) {
+        const int cx = threadIdx.x & 1, cy = (threadIdx.x >> 1) & 1, cz = threadIdx.x >> 2;
+        const int ax = bx + cx, ay = by + cy, az = bz + cz;
+        aslot = sidx(16 * cx, 16 * cy, 16 * cz);
+        if (anchors_in && ax < g.gx && ay < g.gy && az < g.gz)
+            anc = anchors_in[ax + (int64_t)ay * g.gx + (int64_t)az * g.gx * g.gy];
+    }
+    plane_row_store<0>(t.s, r0, wb[3]);
+    if (rg < kRows0) plane_row_store<0>(t.s, r1, wb[3]);
+    else if (rg < k1) plane_row_store<1>(t.s, r1, wb[2]);
+    else if (rg < k2) plane_row_store<2>(t.s, r1, wb[1]);
+    else if (rg < kRowTab) plane_row_store<3>(t.s, r1, wb[0]);
+    if (aslot >= 0) t.s[aslot] = anc;
+}
+
 // The previous reconstruction of the tile (delta form); lane: (x, y) =
 // (l mod 16, l / 16), iteration: z.
 constexpr int kOutIters = 4096 / kThreads;
@@ -1057,11 +1219,15 @@ __device__ __forceinline__ void tile_levels(Ctx<T, IN>& t, const LevelEb<T>& le,
 
 // One tile: compress (load, predict, quantise, store codes) or decompress
 // (fill the code slots, reconstruct, add old, store).
-template <typename T, int M, bool IN, bool Q16 = false>
+// Where a decompressing tile finds its codes (resolved per launch, one
+// kernel body each): int32 or int16 codes of a hand-off array, or the
+// round's decoded bit-planes.
+enum CodeSrc { CODES_32, CODES_16, CODES_PLANES };
+
+template <typename T, int M, bool IN, int CS = CODES_32>
 __device__ __forceinline__ void tile_run(const SplineArgs<T>& a, T* tile, int32_t* rowtab, int bx, int by, int bz) {
     const Geom& g = a.g;
-    // q16 is resolved per launch: one code width per kernel body
-    Ctx<T, IN> t = tile_ctx<T, IN>(a, tile, bx, by, bz, Q16);
+    Ctx<T, IN> t = tile_ctx<T, IN>(a, tile, bx, by, bz, CS == CODES_16);
     uint8_t fl[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) fl[i] = a.flags8[i];
@@ -1085,10 +1251,15 @@ __device__ __forceinline__ void tile_run(const SplineArgs<T>& a, T* tile, int32_
     } else {
         fill_rows(t, rowtab);
         lds_barrier();
-        TileCodes<T> c;
-        codes_load(t, rowtab, maxlevel, a.anchors_in, bx, by, bz, c);
-        __builtin_amdgcn_s_setprio(0);
-        codes_store(tile, c);
+        if constexpr (CS == CODES_PLANES) {
+            plane_codes(t, a.ps, rowtab, maxlevel, a.anchors_in, bx, by, bz);
+            __builtin_amdgcn_s_setprio(0);
+        } else {
+            TileCodes<T> c;
+            codes_load(t, rowtab, maxlevel, a.anchors_in, bx, by, bz, c);
+            __builtin_amdgcn_s_setprio(0);
+            codes_store(tile, c);
+        }
         __syncthreads();
     }
 
@@ -1272,7 +1443,11 @@ template <typename T>
 __device__ __forceinline__ SplineArgs<T> resolve(const SplineArgs<T>& a) {
     SplineArgs<T> r = a;
     if (a.le_dev) r.le = load_level_ebs<T>(a.le_dev);
-    if (a.narrow) r.q16 = *a.narrow != 0;
+    if (a.narrow) {
+        const int32_t src = *a.narrow;
+        r.q16 = src == 1;
+        r.planes = src == 2 && a.ps.bp != nullptr;
+    }
     return r;
 }
 
@@ -1498,9 +1673,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     const bool interior = bx * 16 + 16 < g.nx && by * 16 + 16 < g.ny && bz * 16 + 16 < g.nz;
     const SplineArgs<T> r = resolve(a);
     if constexpr (M == DECOMPRESS) {
+        if (r.planes) {
+            if (interior) tile_run<T, M, true, CODES_PLANES>(r, tile, rowtab, bx, by, bz);
+            else tile_run<T, M, false, CODES_PLANES>(r, tile, rowtab, bx, by, bz);
+            return;
+        }
         if (r.q16) {
-            if (interior) tile_run<T, M, true, true>(r, tile, rowtab, bx, by, bz);
-            else tile_run<T, M, false, true>(r, tile, rowtab, bx, by, bz);
+            if (interior) tile_run<T, M, true, CODES_16>(r, tile, rowtab, bx, by, bz);
+            else tile_run<T, M, false, CODES_16>(r, tile, rowtab, bx, by, bz);
             return;
         }
     }
@@ -1920,9 +2100,22 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         le.eb_r[i] = sizeof(T) == 4 ? (T)J.le32[i] : (T)J.le64[i];
         le.ebx2[i] = sizeof(T) == 4 ? (T)J.le32[4 + i] : (T)J.le64[4 + i];
     }
-    const SplineArgs<T> a{nullptr, J.g, le, nullptr, nullptr, J.state[9] != 0, J.flags, J.state + 8, nullptr, J.qc, nullptr,
-                          J.first ? static_cast<const T*>(J.anchors) : nullptr,
-                          J.first ? nullptr : static_cast<const T*>(J.old), static_cast<T*>(J.out)};
+    // the round's code source: 0 int32 or 1 int16 codes in qc, 2 the planes
+    const int32_t src = J.state[9];
+    SplineArgs<T> a{nullptr, J.g, le, nullptr, nullptr, src == 1, J.flags, J.state + 8, nullptr, J.qc, nullptr,
+                    J.first ? static_cast<const T*>(J.anchors) : nullptr,
+                    J.first ? nullptr : static_cast<const T*>(J.old), static_cast<T*>(J.out)};
+    if (src == 2) {
+        a.ps.bp = J.bp;
+        a.ps.begin = J.state;
+        a.ps.end = J.state + 4;
+        for (int lv = 0; lv < 4; ++lv) {
+            a.ps.base[lv] = J.lv.base[lv];
+            a.ps.stride[lv] = J.lv.stride[lv];
+        }
+        a.ps.size = J.lv.host_size;
+        a.planes = true;
+    }
     const Geom& g = a.g;
     const int64_t tix = gt - prefix[j];
     if (line_field(g)) {
@@ -1935,9 +2128,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     }
     const int bx = (int)(tix % g.gx), by = (int)((tix / g.gx) % g.gy), bz = (int)(tix / ((int64_t)g.gx * g.gy));
     const bool interior = bx * 16 + 16 < g.nx && by * 16 + 16 < g.ny && bz * 16 + 16 < g.nz;
+    if (a.planes) {
+        if (interior) tile_run<T, DECOMPRESS, true, CODES_PLANES>(a, tile, rowtab, bx, by, bz);
+        else tile_run<T, DECOMPRESS, false, CODES_PLANES>(a, tile, rowtab, bx, by, bz);
+        return;
+    }
     if (a.q16) {
-        if (interior) tile_run<T, DECOMPRESS, true, true>(a, tile, rowtab, bx, by, bz);
-        else tile_run<T, DECOMPRESS, false, true>(a, tile, rowtab, bx, by, bz);
+        if (interior) tile_run<T, DECOMPRESS, true, CODES_16>(a, tile, rowtab, bx, by, bz);
+        else tile_run<T, DECOMPRESS, false, CODES_16>(a, tile, rowtab, bx, by, bz);
         return;
     }
     if (interior) tile_run<T, DECOMPRESS, true>(a, tile, rowtab, bx, by, bz);
@@ -1969,7 +2167,7 @@ template <typename T, int M>
 int launch_spline(const pa_layout* L, const T* data, double eb, double rel_eb, const uint8_t* flags8,
                   const int32_t* d_maxlevel, int32_t* qc_out, const int32_t* qc_in, T* anchors,
                   const T* anchors_in, const T* old, T* out, hipStream_t st, const double* le_dev = nullptr,
-                  const int32_t* narrow = nullptr, const Gaps* gaps = nullptr) {
+                  const int32_t* narrow = nullptr, const Gaps* gaps = nullptr, const PlaneSrc* ps = nullptr) {
     Geom g = make_geom(L);
     const int64_t ntiles = (int64_t)g.gx * g.gy * g.gz;
     if (ntiles > 0x7fffffff) return PA_ERR_ARG;
@@ -1980,6 +2178,7 @@ int launch_spline(const pa_layout* L, const T* data, double eb, double rel_eb, c
         args.gaps = *gaps;
         args.gaps_on = 1;
     }
+    if (ps) args.ps = *ps;
     if (line_field(g))
         hipLaunchKernelGGL((line_kernel<T, M>), dim3((unsigned)((ntiles + kLineTiles - 1) / kLineTiles)),
                            dim3(kThreads), 0, st, args);
@@ -2171,6 +2370,34 @@ extern "C" int pa_spline_delta_reconstruct_narrow(const void* d_qc, const int32_
                                                 nullptr, (const float*)d_old, (float*)d_out, st, nullptr, d_narrow);
     return launch_spline<double, DECOMPRESS>(L, nullptr, eb, rel_eb, d_flags8, d_maxlevel, nullptr, q, nullptr,
                                              nullptr, (const double*)d_old, (double*)d_out, st, nullptr, d_narrow);
+}
+
+extern "C" int pa_spline_reconstruct_planes(const void* d_qc, const int32_t* d_src, const uint8_t* d_bp,
+                                            const int32_t* d_begin, const int32_t* d_end, const void* d_anchors,
+                                            const void* d_old, int dtype, const pa_layout* L, double eb,
+                                            double rel_eb, const uint8_t* d_flags8, const int32_t* d_maxlevel,
+                                            void* d_out, void* stream) {
+    if (!L || !d_qc || !d_src || !d_bp || !d_begin || !d_end || !d_flags8 || !d_out || !valid_dtype(dtype) ||
+        !(eb > 0) || (d_anchors == nullptr) == (d_old == nullptr))
+        return PA_ERR_ARG;
+    PlaneSrc ps{};
+    ps.bp = d_bp;
+    ps.begin = d_begin;
+    ps.end = d_end;
+    for (int lv = 0; lv < 4; ++lv) {
+        ps.base[lv] = L->dev_base[lv];
+        ps.stride[lv] = L->dev_stride[lv];
+    }
+    ps.size = L->host_size;
+    hipStream_t st = as_stream(stream);
+    const int32_t* q = static_cast<const int32_t*>(d_qc);
+    if (dtype == PA_F32)
+        return launch_spline<float, DECOMPRESS>(L, nullptr, eb, rel_eb, d_flags8, d_maxlevel, nullptr, q, nullptr,
+                                                (const float*)d_anchors, (const float*)d_old, (float*)d_out, st,
+                                                nullptr, d_src, nullptr, &ps);
+    return launch_spline<double, DECOMPRESS>(L, nullptr, eb, rel_eb, d_flags8, d_maxlevel, nullptr, q, nullptr,
+                                             (const double*)d_anchors, (const double*)d_old, (double*)d_out, st,
+                                             nullptr, d_src, nullptr, &ps);
 }
 
 extern "C" int pa_spline_delta_reconstruct(const int32_t* d_qc, int dtype, const pa_layout* L, double eb,

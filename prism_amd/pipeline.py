@@ -515,7 +515,7 @@ class Retriever:
         return self._small[8:9]
 
     @property
-    def narrow(self):  # codes of this round handed over as int16
+    def narrow(self):  # this round's codes: 0 int32 hand-off, 1 int16 hand-off, 2 read from the bit-planes
         return self._small[9:10]
 
     @property
@@ -568,8 +568,9 @@ class Retriever:
                                                a["bytes_read"], int(self.rounds > 0), a["dws"], dws.numel(), st),
                   "pa_lossless_decode_window")
         with tm("unshuffle"):
-            check(lb.pa_inverse_bitshuffle_window_narrow(a["bp"], a["L"], a["begin"], a["end"], a["qc"], a["narrow"],
-                                                         st), "pa_inverse_bitshuffle_window_narrow")
+            # no codes written when the reconstruction reads the planes itself
+            check(lb.pa_inverse_bitshuffle_window_deferred(a["bp"], a["L"], a["begin"], a["end"], a["qc"],
+                                                           a["narrow"], st), "pa_inverse_bitshuffle_window_deferred")
         nx, ny, nz = self.c.shape
         new = torch.empty((nz, ny, nx), dtype=self.c.dtype, device=self._small.device)
         with tm("reconstruct"):
@@ -602,14 +603,11 @@ class Retriever:
 
     def _reconstruct(self, new, st):
         a = self._a
-        if self.out is None:
-            check(lib().pa_spline_reconstruct_narrow(a["qc"], a["narrow"], a["anchors"], self.dt, a["L"], self.eb_abs,
-                                                     self.rel_eb, a["flags"], a["maxlevel"], _p(new), st),
-                  "pa_spline_reconstruct_narrow")
-        else:
-            check(lib().pa_spline_delta_reconstruct_narrow(a["qc"], a["narrow"], self.dt, a["L"], self.eb_abs,
-                                                           self.rel_eb, a["flags"], a["maxlevel"], _p(self.out),
-                                                           _p(new), st), "pa_spline_delta_reconstruct_narrow")
+        first = self.out is None
+        check(lib().pa_spline_reconstruct_planes(a["qc"], a["narrow"], a["bp"], a["begin"], a["end"],
+                                                 a["anchors"] if first else None, None if first else _p(self.out),
+                                                 self.dt, a["L"], self.eb_abs, self.rel_eb, a["flags"], a["maxlevel"],
+                                                 _p(new), st), "pa_spline_reconstruct_planes")
 
 
 class RetrievalBatch:

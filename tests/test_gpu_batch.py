@@ -139,6 +139,7 @@ def test_batched_retrieval_equals_plain(pl):
     batched = [pl.Retriever(c) for c in comp]
     rng = np.random.default_rng(11)
     targets = [1e-2, 1e-3, 1e-4, 1e-5, 1e-6]
+    sources = set()  # (3-D field, code source of the round)
     for rnd in range(5):
         items, want = [], []
         for i in range(len(comp)):
@@ -151,5 +152,11 @@ def test_batched_retrieval_equals_plain(pl):
         torch.cuda.synchronize()
         for (i, w), g in zip(want, got.outs):
             assert torch.equal(w.view(torch.uint8), g.view(torch.uint8)), (rnd, i)
-            assert torch.equal(plain[i].state, batched[i].state), (rnd, i)
+            # begin / end / maxlevel (word 9 names the code hand-off each path took)
+            assert torch.equal(plain[i].state[0:9], batched[i].state[0:9]), (rnd, i)
             assert int(plain[i].bytes_read.item()) == int(batched[i].bytes_read.item()), (rnd, i)
+            sources.add((comp[i].shape[2] > 1, int(batched[i].state[9].item())))
+    # the batched rounds took every code source: int32 and int16 hand-offs
+    # and, on 3-D fields only, the bit-planes
+    assert {(True, 0), (True, 2)} <= sources and (False, 2) not in sources, sources
+    assert any(s == 1 for _, s in sources), sources

@@ -135,7 +135,8 @@ def test_serving_mix_alone_equals_in_mix(mods):
             want = digest(alone.retrieve(target, check_stream=False))
             assert torch.equal(want, got[rid]), (k, rid)
         alone.check()
-        assert torch.equal(alone.state, srv.fields[k].retriever.state), k
+        # begin / end / maxlevel (word 9 names the code hand-off each path took)
+        assert torch.equal(alone.state[0:9], srv.fields[k].retriever.state[0:9]), k
     announce("serving: every compression alone")
     for r in reqs:
         if r.kind == "compress":

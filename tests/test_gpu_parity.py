@@ -292,6 +292,51 @@ def test_codec_parity(pa, oracle, shape, kind):
             assert np.array_equal(got[lo:hi], bp_np[lo:hi])
 
 
+def test_codec_parity_many_chunks(pa, oracle):
+    """A stream of more than 16,384 chunks (512 x 512 x 300 codes: 19,200
+    chunks, 10,000+ encoder workgroups, so the look-back runs over several
+    rounds of workgroups and the decoder's size scan over several windows):
+    stream, size table and cumulative table byte-equal to the oracle's,
+    full and windowed decode equal to the planes."""
+    _lib, pipe = pa
+    nx, ny, nz = 512, 512, 300
+    announce("codec many chunks", (nz, ny, nx))
+    L = _lib.layout(nx, ny, nz)
+    Lo = oracle.layout(nx, ny, nz)
+    assert L.nchunks > 16384
+    rng = np.random.default_rng(7)
+    # each 1 KiB block empty, sparse, half, dense or full (uint8 draws: fast
+    # at 314 MB); a byte survives where its draw is below the block's level
+    nblk = (L.host_size + 1023) // 1024
+    level = np.repeat(rng.choice(np.array([0, 1, 128, 230, 255], np.uint8), nblk), 1024)[:L.host_size]
+    keep = rng.integers(0, 255, L.host_size, dtype=np.uint8) < level
+    bp_np = np.maximum(rng.integers(0, 256, L.host_size, dtype=np.uint8), 1) * keep.astype(np.uint8)
+    s_o, cum_o = oracle.lossless_encode(bp_np, Lo)
+    out, s_g, cum_g = _gpu_encode(_lib, L, bp_np)
+    assert s_g.size + (-s_g.size) % 8 == s_o.size
+    assert np.array_equal(s_g, s_o[:s_g.size])
+    assert np.array_equal(cum_g, cum_o)
+    bp = torch.zeros(L.host_size, dtype=torch.uint8, device="cuda")
+    ws = torch.empty(_lib.lib().pa_decode_workspace(C.byref(L)), dtype=torch.uint8, device="cuda")
+    _lib.check(_lib.lib().pa_lossless_decode(P(out), out.numel(), C.byref(L), P(bp), P(ws), ws.numel(), S()))
+    assert np.array_equal(host(bp), bp_np)
+    assert int(host(ws[:4].view(torch.int32))[0]) == 0
+    b, e = [0, 3, 17, 30], [12, 32, 24, 31]
+    be = dev(np.array(b + e, np.int32))
+    bp2 = torch.zeros(L.host_size, dtype=torch.uint8, device="cuda")
+    nread = torch.zeros(1, dtype=torch.int64, device="cuda")
+    _lib.check(_lib.lib().pa_lossless_decode_window(P(out), out.numel(), C.byref(L), P(be[0:4]), P(be[4:8]), P(bp2),
+                                                     P(nread), 0, P(ws), ws.numel(), S()))
+    got = host(bp2)
+    ref, nb = oracle.lossless_decode_window(s_o, Lo, b, e, corrected=True)
+    assert int(host(nread)[0]) == nb
+    for lr in range(4):
+        lv = 3 - lr
+        lo = L.dev_base[lv] + b[lr] * L.dev_stride[lv]
+        hi = L.dev_base[lv] + e[lr] * L.dev_stride[lv]
+        assert np.array_equal(got[lo:hi], bp_np[lo:hi])
+
+
 def test_strategy_parity(pa, oracle):
     _lib, pipe = pa
     rng = np.random.default_rng(5)
@@ -386,45 +431,71 @@ def test_quality_statistics(pa, dtype):
             assert abs(q["nrmse"] - np.sqrt(mse) / rg) <= 1e-12 * np.sqrt(mse) / rg
 
 
-def test_narrow_handoff_equals_int32(pa, oracle):
-    """Progressive rounds through the int16 hand-off (every level reading
-    planes >= 17) and through the int32 stage calls give the same bits; the
-    first round (planes from 0) stays int32."""
+HANDOFF_CASES = [((40, 50, 60), np.float64, (1e-2, 1e-3, 1e-4, 1e-5, 1e-6)),
+                 ((33, 17, 18), np.float64, (1e-1, 1e-3, 1e-5, 1e-6)),
+                 ((65, 47, 34), np.float32, (1e-1, 1e-2, 1e-3, 1e-4)),
+                 ((1, 80, 96), np.float64, (1e-2, 1e-4, 1e-6))]  # (nz, ny, nx)
+
+
+@pytest.mark.parametrize("shape,dtype,targets", HANDOFF_CASES,
+                         ids=["x".join(map(str, c[0])) + "-" + np.dtype(c[1]).name for c in HANDOFF_CASES])
+def test_code_handoffs_equal_int32(pa, oracle, shape, dtype, targets):
+    """The three ways a progressive round's codes reach the reconstruction
+    give the same bits as the int32 stage calls: the Retriever's (the
+    reconstruction reads the decoded bit-planes when every level's window
+    holds at most 8 planes; else an int16 or int32 hand-off) and the int16
+    hand-off entries (every level reading planes >= 17).  The first round
+    (planes from 0) hands over int32 codes; 2-D fields never read planes."""
     _lib, pipe = pa
-    shape = (40, 50, 60)
-    data = smooth(shape, np.float64, seed=3)
-    c = pipe.compress(dev(data), 1e-6, rel=True, progressive=True)
-    r = pipe.Retriever(c)  # narrow path
+    data = smooth(shape, dtype, seed=3)
+    c = pipe.compress(dev(data), 1e-6 if dtype == np.float64 else 1e-4, rel=True, progressive=True)
+    r = pipe.Retriever(c)
     L = r.L
+    dt = _lib.PA_F64 if dtype == np.float64 else _lib.PA_F32
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
     flags_seen = []
     st32 = torch.zeros(16, dtype=torch.int32, device="cuda")
     bp32 = torch.empty(L.host_size, dtype=torch.uint8, device="cuda")
     qc32 = torch.empty(L.code_len, dtype=torch.int32, device="cuda")
+    qc16 = torch.empty(L.code_len, dtype=torch.int32, device="cuda")
+    src16 = torch.zeros(1, dtype=torch.int32, device="cuda")
     dws = torch.empty(_lib.lib().pa_decode_workspace(C.byref(L)), dtype=torch.uint8, device="cuda")
     sws = torch.empty(_lib.lib().pa_strategy_workspace(), dtype=torch.uint8, device="cuda")
-    old = None
-    for k, t in enumerate((1e-2, 1e-3, 1e-4, 1e-5, 1e-6)):
-        announce("narrow hand-off round", k, t)
+    old = old16 = None
+    lib = _lib.lib()
+    for k, t in enumerate(targets):
+        announce("code hand-off round", shape, k, t)
         got = r.retrieve(t)
         flags_seen.append(int(r.narrow.item()))
-        lib = _lib.lib()
         _lib.check(lib.pa_find_strategy(P(r.cum), P(st32[0:4]), P(st32[4:8]), P(st32[8:9]), r.eb_abs, t * r.range,
                                         P(sws), sws.numel(), S()))
         _lib.check(lib.pa_lossless_decode_window(P(c.stream), c.nbytes, C.byref(L), P(st32[0:4]), P(st32[4:8]),
                                                  P(bp32), None, int(k > 0), P(dws), dws.numel(), S()))
         _lib.check(lib.pa_inverse_bitshuffle_window(P(bp32), C.byref(L), _lib.PA_MODE_NB, P(st32[0:4]), P(st32[4:8]),
                                                     P(qc32), S()))
-        new = torch.empty(shape, dtype=torch.float64, device="cuda")
+        _lib.check(lib.pa_inverse_bitshuffle_window_narrow(P(bp32), C.byref(L), P(st32[0:4]), P(st32[4:8]), P(qc16),
+                                                           P(src16), S()))
+        new = torch.empty(shape, dtype=tdt, device="cuda")
+        new16 = torch.empty_like(new)
         if old is None:
-            _lib.check(lib.pa_spline_reconstruct(P(qc32), P(r.anchors), _lib.PA_F64, C.byref(L), r.eb_abs, r.rel_eb,
+            _lib.check(lib.pa_spline_reconstruct(P(qc32), P(r.anchors), dt, C.byref(L), r.eb_abs, r.rel_eb,
                                                  P(r.flags), P(st32[8:9]), P(new), S()))
+            _lib.check(lib.pa_spline_reconstruct_narrow(P(qc16), P(src16), P(r.anchors), dt, C.byref(L), r.eb_abs,
+                                                        r.rel_eb, P(r.flags), P(st32[8:9]), P(new16), S()))
         else:
-            _lib.check(lib.pa_spline_delta_reconstruct(P(qc32), _lib.PA_F64, C.byref(L), r.eb_abs, r.rel_eb,
+            _lib.check(lib.pa_spline_delta_reconstruct(P(qc32), dt, C.byref(L), r.eb_abs, r.rel_eb,
                                                        P(r.flags), P(st32[8:9]), P(old), P(new), S()))
-        old = new
+            _lib.check(lib.pa_spline_delta_reconstruct_narrow(P(qc16), P(src16), dt, C.byref(L), r.eb_abs, r.rel_eb,
+                                                              P(r.flags), P(st32[8:9]), P(old16), P(new16), S()))
+        old, old16 = new, new16
         assert torch.equal(got.view(torch.uint8), new.view(torch.uint8)), k
+        assert torch.equal(new16.view(torch.uint8), new.view(torch.uint8)), k
         assert torch.equal(r.state[0:9], st32[0:9])
-    assert flags_seen[0] == 0 and 1 in flags_seen[1:], flags_seen
+    assert flags_seen[0] == 0, flags_seen
+    if shape[0] > 1:  # nz > 1
+        assert 2 in flags_seen[1:], flags_seen
+    else:
+        assert 2 not in flags_seen, flags_seen
 
 
 # ---- the closed-form pins of the tuner's sample geometry and of the

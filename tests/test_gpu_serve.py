@@ -57,7 +57,8 @@ def test_results_alone_equal_in_mix(mods):
             want = alone.retrieve(target)
             got = byrid[rid].result
             assert torch.equal(want.view(torch.uint8), got.view(torch.uint8)), (k, rid)
-        assert torch.equal(alone.state, srv.fields[k].retriever.state)
+        # begin / end / maxlevel (word 9 names the code hand-off each path took)
+        assert torch.equal(alone.state[0:9], srv.fields[k].retriever.state[0:9])
     # compressions: alone through the plain per-field path
     for r in reqs:
         if r.kind == "compress":
