@@ -498,6 +498,63 @@ def test_code_handoffs_equal_int32(pa, oracle, shape, dtype, targets):
         assert 2 not in flags_seen, flags_seen
 
 
+# (begin[4], end[4]) per level lr = 3 - l: the 8 top planes, the 8 last,
+# mixed widths with an empty level, every level empty, odd offsets
+PLANE_WINDOWS = [([0, 0, 0, 0], [8, 8, 8, 8]), ([24, 25, 26, 27], [32, 32, 32, 32]), ([31, 5, 9, 0], [32, 5, 17, 8]),
+                 ([3, 3, 3, 3], [3, 3, 3, 3]), ([12, 0, 30, 17], [20, 3, 32, 25])]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", [(33, 47, 29), (16, 16, 16), (35, 18, 20), (17, 17, 17), (40, 64, 64), (1, 40, 70)])
+def test_plane_reads_equal_int32_handoff(pa, shape, dtype):
+    """The reconstruction that reads its codes from the bit-planes equals the
+    int32 hand-off (windowed inverse shuffle, then the reconstruction) on
+    random plane bytes, under chosen windows of at most 8 planes per level
+    (the first and the last planes, an empty level, all levels empty), for
+    maxlevel 0 and 3, in the anchored and the delta form.  A 2-D field is
+    never deferred: its reconstruction reads the int32 or int16 hand-off."""
+    _lib, pipe = pa
+    nz, ny, nx = shape
+    L = _lib.layout(nx, ny, nz)
+    dt = _lib.PA_F32 if dtype == np.float32 else _lib.PA_F64
+    lib = _lib.lib()
+    rng = np.random.default_rng(nx * 7 + ny)
+    bp = dev(rng.integers(0, 256, L.host_size, dtype=np.uint8))
+    an = dev(rng.normal(0, 1, L.n4).astype(dtype))
+    old = dev(rng.normal(0, 1, shape).astype(dtype))
+    fl = dev(np.array([1, 0, 1, 0, 0, 1, 0, 1], np.uint8))
+    eb, rel = 1e-3, 1e-4
+    qc32 = torch.zeros(L.code_len, dtype=torch.int32, device="cuda")
+    qcd = torch.zeros(L.code_len, dtype=torch.int32, device="cuda")
+    src = torch.zeros(1, dtype=torch.int32, device="cuda")
+    for b, e in PLANE_WINDOWS:
+        for maxlevel in (0, 3):
+            announce("plane reads", shape, dtype.__name__, b, e, maxlevel)
+            st = dev(np.array(b + e + [maxlevel], np.int32))
+            B, E, ML = st[0:4], st[4:8], st[8:9]
+            _lib.check(lib.pa_inverse_bitshuffle_window(P(bp), C.byref(L), _lib.PA_MODE_NB, P(B), P(E), P(qc32), S()))
+            _lib.check(lib.pa_inverse_bitshuffle_window_deferred(P(bp), C.byref(L), P(B), P(E), P(qcd), P(src), S()))
+            assert int(host(src)[0]) == (2 if nz > 1 else int(min(b[lr] >= 17 or b[lr] >= e[lr]
+                                                                   for lr in range(4))))
+            for form in ("anchored", "delta"):
+                want = torch.empty(shape, dtype=old.dtype, device="cuda")
+                got = torch.empty_like(want)
+                if form == "anchored":
+                    _lib.check(lib.pa_spline_reconstruct(P(qc32), P(an), dt, C.byref(L), eb, rel, P(fl), P(ML),
+                                                         P(want), S()))
+                    _lib.check(lib.pa_spline_reconstruct_planes(P(qcd), P(src), P(bp), P(B), P(E), P(an), None, dt,
+                                                                C.byref(L), eb, rel, P(fl), P(ML), P(got), S()))
+                else:
+                    _lib.check(lib.pa_spline_delta_reconstruct(P(qc32), dt, C.byref(L), eb, rel, P(fl), P(ML),
+                                                               P(old), P(want), S()))
+                    _lib.check(lib.pa_spline_reconstruct_planes(P(qcd), P(src), P(bp), P(B), P(E), None, P(old), dt,
+                                                                C.byref(L), eb, rel, P(fl), P(ML), P(got), S()))
+                assert torch.equal(got.view(torch.uint8), want.view(torch.uint8)), (b, e, maxlevel, form)
+    # exactly one of anchors and old
+    assert lib.pa_spline_reconstruct_planes(P(qcd), P(src), P(bp), P(B), P(E), P(an), P(old), dt, C.byref(L), eb,
+                                            rel, P(fl), P(ML), P(old), S()) == _lib.PA_ERR_ARG
+
+
 # ---- the closed-form pins of the tuner's sample geometry and of the
 # partial-tile rule selection, run through the HIP kernels ----
 
