@@ -58,9 +58,10 @@ def S():
 
 # (nz, ny, nx); 1-D shapes span one and several 256-tile workgroups of the
 # line-field kernel, 2-D shapes one and several 4-tile workgroups of the
-# plane kernel, with partial last tiles
+# plane kernel, with partial last tiles; (6, 1, 70) is a 3-D field one row
+# high (the tile kernel with a single y)
 SHAPES = [(33, 47, 29), (16, 16, 16), (1, 1, 1000), (1, 40, 70), (35, 18, 20), (64, 64, 64), (17, 17, 17),
-          (1, 1, 5), (1, 1, 4097), (1, 1, 12345), (1, 2, 300), (1, 17, 33), (1, 130, 75)]
+          (1, 1, 5), (1, 1, 4097), (1, 1, 12345), (1, 2, 300), (1, 17, 33), (1, 130, 75), (6, 1, 70)]
 FLAGSETS = [[0] * 8, [1] * 4 + [0] * 4, [0] * 4 + [1] * 4, [1, 0, 1, 0, 0, 1, 0, 1]]
 
 
@@ -505,7 +506,8 @@ PLANE_WINDOWS = [([0, 0, 0, 0], [8, 8, 8, 8]), ([24, 25, 26, 27], [32, 32, 32, 3
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("shape", [(33, 47, 29), (16, 16, 16), (35, 18, 20), (17, 17, 17), (40, 64, 64), (1, 40, 70)])
+@pytest.mark.parametrize("shape", [(33, 47, 29), (16, 16, 16), (35, 18, 20), (17, 17, 17), (40, 64, 64), (6, 1, 70),
+                                   (1, 40, 70)])
 def test_plane_reads_equal_int32_handoff(pa, shape, dtype):
     """The reconstruction that reads its codes from the bit-planes equals the
     int32 hand-off (windowed inverse shuffle, then the reconstruction) on
