@@ -101,7 +101,7 @@ class Compressor {
     uint8_t* d_flags{nullptr};      // use_md[4], reverse[4]
     double* d_minmax{nullptr};      // [min, max, range, scratch]
     uint64_t* d_sizes{nullptr};     // [lossless length, total length, bytes read]
-    int32_t* d_maxlevel{nullptr};
+    int32_t* d_maxlevel{nullptr};   // [maxlevel, code source of the round (0 int32, 1 int16, 2 the planes)]
     // one workspace allocation cut into the tuner's, the encoder's, the
     // decoder's (kept across retrieval rounds) and the strategy search's
     uint8_t* ws{nullptr};

@@ -219,7 +219,7 @@ void Compressor<T, E>::init(prism_context* config) {
     check_hip(hipMalloc((void**)&d_flags, 8), "hipMalloc");
     check_hip(hipMalloc((void**)&d_minmax, 4 * sizeof(double)), "hipMalloc");
     check_hip(hipMalloc((void**)&d_sizes, 4 * sizeof(uint64_t)), "hipMalloc");
-    check_hip(hipMalloc((void**)&d_maxlevel, sizeof(int32_t)), "hipMalloc");
+    check_hip(hipMalloc((void**)&d_maxlevel, 2 * sizeof(int32_t)), "hipMalloc");
     ws_len[0] = pa_autotune_workspace(&L);
     ws_len[1] = pa_encode_workspace(&L);
     ws_len[2] = pa_decode_workspace(&L);
@@ -394,16 +394,18 @@ void Compressor<T, E>::decompress_progressive_pipeline(context* config, StatBuff
              "pa_lossless_decode_window");
     offsets_ready = true;
     check_hip(hipEventRecord(ev[2], S(stream)), "hipEventRecord");
-    check_pa(pa_inverse_bitshuffle_window((const uint8_t*)bp->d, &L, PA_MODE_NB, begin, end, (int32_t*)qc->d, stream),
-             "pa_inverse_bitshuffle_window");
+    // the inverse shuffle writes int32 or int16 codes, or none when the
+    // reconstruction reads this round's planes itself (d_maxlevel[1] says which)
+    int32_t* d_src = d_maxlevel + 1;
+    check_pa(pa_inverse_bitshuffle_window_deferred((const uint8_t*)bp->d, &L, begin, end, qc->d, d_src, stream),
+             "pa_inverse_bitshuffle_window_deferred");
     check_hip(hipEventRecord(ev[3], S(stream)), "hipEventRecord");
     const int dt = sizeof(T) == 4 ? PA_F32 : PA_F64;
-    if (first || output_old == nullptr)
-        check_pa(pa_spline_reconstruct((const int32_t*)qc->d, ap->d, dt, &L, config->eb, config->rel_eb, d_flags,
-                                       d_maxlevel, output_new->d, stream),
-                 "pa_spline_reconstruct");
-    else
-        decompress_progressive_predict(config, output_old, output_new, stream);
+    const bool anchored = first || output_old == nullptr;
+    check_pa(pa_spline_reconstruct_planes(qc->d, d_src, (const uint8_t*)bp->d, begin, end, anchored ? ap->d : nullptr,
+                                          anchored ? nullptr : output_old->d, dt, &L, config->eb, config->rel_eb,
+                                          d_flags, d_maxlevel, output_new->d, stream),
+             "pa_spline_reconstruct_planes");
     check_hip(hipEventRecord(ev[4], S(stream)), "hipEventRecord");
     int32_t err = 0;
     check_hip(hipMemcpyAsync(&err, ws + ws_off[2], sizeof(err), hipMemcpyDeviceToHost, S(stream)), "D2H");
