@@ -1029,11 +1029,12 @@ __device__ __forceinline__ void codes_store(T* tile, const TileCodes<T>& c) {
 // (InterpolationPredictor.cu:2209-2239).  Lane l takes rows l and 256 + l of
 // the tile's 404 code rows, in the row table's order.  A row's codes are
 // adjacent in its level's segment, so on each plane of the window they lie
-// in at most 3 adjacent bytes (17 codes from any bit offset): two aligned
-// 4-byte loads, a funnel shift, and the bit order reversed within each
-// byte put code m of the run at bit m.  Code m gathers bit m of each window
-// plane k at bit 31 - k, and the negabinary inverse of that word is the
-// round's code, as the inverse shuffle computes it.
+// in at most 3 adjacent bytes (17 codes from any bit offset): one 8-byte
+// load from the 4-byte aligned address below them, a funnel shift, and the
+// bit order reversed within each byte put code m of the run at bit m.  Code
+// m gathers bit m of each window plane k at bit 31 - k, and the negabinary
+// inverse of that word is the round's code, as the inverse shuffle
+// computes it.
 struct PlaneRow {
     int row;     // row (j, k) = (row mod E, row / E) of the level's lattice
     bool full;   // the row holds every i, or the odd i only
