@@ -273,13 +273,14 @@ __global__ __launch_bounds__(kThreads) void retr_unshuffle_kernel(const RetrJob*
     const int j = find_field(prefix, 0, nj, blockIdx.x);
     const RetrJob& J = jobs[j];
     const int blk = (int)(blockIdx.x - prefix[j]);
-    if (J.g.nz > 1 && window_from_planes(J.state, J.state + 4)) {  // the reconstruction reads the planes
+    const int32_t* live = J.state + 10;  // each level's first live plane (the planes before it are zero)
+    if (J.g.nz > 1 && window_from_planes(live, J.state + 4)) {  // the reconstruction reads the planes
         if (blk == 0 && threadIdx.x == 0) J.state[9] = 2;
         return;
     }
-    const bool narrow = window_narrow(J.state, J.state + 4);
+    const bool narrow = window_narrow(live, J.state + 4);
     if (blk == 0 && threadIdx.x == 0) J.state[9] = narrow;
-    unshuffle_block<PA_MODE_NB, true>(J.bp, J.lv, J.state, J.state + 4, J.qc, blk, stage, narrow);
+    unshuffle_block<PA_MODE_NB, true>(J.bp, J.lv, live, J.state + 4, J.qc, blk, stage, narrow);
 }
 
 }  // namespace

@@ -636,7 +636,11 @@ __device__ __forceinline__ int64_t chunk_of_byte(const ChunkGeom& g, int64_t x) 
     while (i < 3 && x >= g.hbase[2 - i]) ++i;
     const int lv = 3 - i;
     const int64_t r = x - g.hbase[lv];
-    const int64_t p = r / g.e[lv];
+    // the host plane p = r / e < 32 by bisection (no 64-bit divide)
+    int p = 0;
+#pragma unroll
+    for (int s = 16; s >= 1; s >>= 1)
+        if (r >= (int64_t)(p + s) * g.e[lv]) p += s;
     return g.chunk_start[i] + p * g.cb[lv] + (r - p * g.e[lv]) / kCS;
 }
 
@@ -681,6 +685,35 @@ __device__ __forceinline__ int64_t window_chunk(const Window& w, int64_t k) {
         k -= n;
     }
     return w.lo[3] + k;
+}
+
+// The first plane of each level's window [begin, end) that a covering chunk
+// holds a non-zero byte of (an empty chunk is its 2-byte footer alone and
+// decodes to zeros), end where there is none: the planes before it are zero
+// and need neither decoding nor reading.  A plane is tested through the
+// chunks that cover its device bytes, from their sizes and offsets.  Run by
+// one wave (lane l tests planes l and 64 + l of the 4 x 32); valid in every
+// lane.
+__device__ __forceinline__ void live_begins(const ChunkGeom& g, const int64_t* offsets, const uint16_t* sizes,
+                                            const int32_t* begin, const int32_t* end, int (&live)[4]) {
+    const int lane = threadIdx.x & 63;
+    uint64_t m[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int p = lane + 64 * h, lr = p >> 5, k = p & 31, lv = 3 - lr;
+        bool on = false;
+        if (k >= begin[lr] && k < end[lr]) {
+            const int64_t s = g.dbase[lv] + (int64_t)k * g.dstride[lv];
+            const int64_t lo = chunk_of_byte(g, s), hi = chunk_of_byte(g, s + g.dstride[lv] - 1);
+            on = offsets[hi] + sizes[hi] - offsets[lo] != 2 * (hi - lo + 1);
+        }
+        m[h] = __ballot(on);
+    }
+#pragma unroll
+    for (int lr = 0; lr < 4; ++lr) {
+        const uint32_t bits = (uint32_t)(m[lr >> 1] >> (32 * (lr & 1)));
+        live[lr] = bits ? __builtin_ctz(bits) : end[lr];
+    }
 }
 
 // payload bytes of the selected chunks, from the exclusive offsets
@@ -893,14 +926,27 @@ __device__ __forceinline__ DecodeWs decode_ws(const RetrJob& J) {
     return w;
 }
 
-// One workgroup per job, on the field's first round: the error words and the
-// payload offsets.
+// One workgroup per job: on the field's first round the error words and the
+// payload offsets; then, every round, each level's first live plane of the
+// window (state[10..13]): the decode, the inverse shuffle and the
+// reconstruction take the planes from there on.
 __global__ __launch_bounds__(1024) void retr_prep_kernel(const RetrJob* __restrict__ jobs) {
     const RetrJob& J = jobs[blockIdx.x];
-    if (J.offsets_ready) return;
     const DecodeWs w = decode_ws(J);
-    if (threadIdx.x < 16) reinterpret_cast<uint32_t*>(J.dws)[threadIdx.x] = 0u;
-    size_scan(reinterpret_cast<const uint16_t*>(J.stream + 8), J.L.nchunks, w.offsets);
+    const uint16_t* sizes = reinterpret_cast<const uint16_t*>(J.stream + 8);
+    if (!J.offsets_ready) {
+        if (threadIdx.x < 16) reinterpret_cast<uint32_t*>(J.dws)[threadIdx.x] = 0u;
+        size_scan(sizes, J.L.nchunks, w.offsets);
+        __syncthreads();  // the offsets, written across the workgroup
+    }
+    if (threadIdx.x < 64) {
+        int live[4];
+        live_begins(J.cg, w.offsets, sizes, J.state, J.state + 4, live);
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int lr = 0; lr < 4; ++lr) J.state[10 + lr] = live[lr];
+        }
+    }
 }
 
 // exclusive prefix of the per-job selected-chunk counts (one workgroup);
@@ -914,10 +960,11 @@ __global__ __launch_bounds__(1024) void retr_count_scan_kernel(const RetrJob* __
     int64_t sum = 0;
     for (int j = lo; j < hi; ++j) {
         const RetrJob& J = jobs[j];
-        const Window w = window_ranges(J.cg, J.state, J.state + 4);
-        sum += w.count;
+        // the chunks of the live planes are decoded; the whole window is counted as read
+        sum += window_ranges(J.cg, J.state + 10, J.state + 4).count;
         if (J.bytes_read)
-            *J.bytes_read += window_bytes(w, decode_ws(J).offsets, reinterpret_cast<const uint16_t*>(J.stream + 8));
+            *J.bytes_read += window_bytes(window_ranges(J.cg, J.state, J.state + 4), decode_ws(J).offsets,
+                                          reinterpret_cast<const uint16_t*>(J.stream + 8));
     }
     int64_t x = sum;
 #pragma unroll
@@ -932,7 +979,7 @@ __global__ __launch_bounds__(1024) void retr_count_scan_kernel(const RetrJob* __
     for (int j = lo; j < hi; ++j) {
         prefix[j] = run;
         const RetrJob& J = jobs[j];
-        run += window_ranges(J.cg, J.state, J.state + 4).count;
+        run += window_ranges(J.cg, J.state + 10, J.state + 4).count;
     }
     if (tid == 1023) prefix[nj] = run;
 }
@@ -947,7 +994,7 @@ __global__ __launch_bounds__(kThreads) void retr_decode_kernel(const RetrJob* __
         const int j = find_field(prefix, 0, nj, i);
         const RetrJob& J = jobs[j];
         const DecodeWs w = decode_ws(J);
-        const Window win = window_ranges(J.cg, J.state, J.state + 4);
+        const Window win = window_ranges(J.cg, J.state + 10, J.state + 4);  // the live planes
         decode_chunk(J.stream, J.cg, w.offsets, J.bp, w.err, J.stream_len, window_chunk(win, i - prefix[j]), S);
         __syncthreads();  // the next chunk reuses the LDS
     }

@@ -196,7 +196,9 @@ struct RetrJob {
     const uint64_t* cum;   // trailer: cumulative payload sizes [4][32]
     const uint8_t* flags;  // trailer: flags8
     const void* anchors;   // trailer: anchors
-    int32_t* state;        // begin[4], end[4], maxlevel, code source (0 int32, 1 int16, 2 bit-planes)
+    // begin[4], end[4], maxlevel, code source (0 int32, 1 int16, 2 bit-planes),
+    // then at [10..13] each level's first live plane of the round's window
+    int32_t* state;
     unsigned long long* keys;  // strategy workgroup keys (33 x 33)
     uint8_t* bp;           // decoded bit-planes (device layout in host_size bytes), kept between rounds
     int32_t* qc;           // codes, code_len

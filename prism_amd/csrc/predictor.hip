@@ -2108,7 +2108,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                     J.first ? nullptr : static_cast<const T*>(J.old), static_cast<T*>(J.out)};
     if (src == 2) {
         a.ps.bp = J.bp;
-        a.ps.begin = J.state;
+        a.ps.begin = J.state + 10;  // each level's first live plane
         a.ps.end = J.state + 4;
         for (int lv = 0; lv < 4; ++lv) {
             a.ps.base[lv] = J.lv.base[lv];

@@ -131,9 +131,13 @@ def test_batched_retrieval_equals_plain(pl):
     plain per-field Retriever: same reconstructions (bit for bit), same
     begin / end / maxlevel state and bytes read; rounds include looser
     targets after tighter ones, targets under the stream's bound (every
-    plane) and fields that sit a round out."""
+    plane) and fields that sit a round out.  A rough field at a tight bound
+    has codes in high planes, so its first round hands over int32 codes."""
     announce("batched retrieval rounds")
     data, ebs = mixed_fields(8, 600, max3d=72, max2d=280, max1d=1.6e5)
+    gen = torch.Generator(device="cuda").manual_seed(9)
+    data.append(torch.randn((24, 20, 30), generator=gen, dtype=torch.float64, device="cuda"))
+    ebs.append(1e-7)
     comp = [pl.compress(t, e, rel=True, progressive=True) for t, e in zip(data, ebs)]
     plain = [pl.Retriever(c) for c in comp]
     batched = [pl.Retriever(c) for c in comp]
@@ -158,5 +162,5 @@ def test_batched_retrieval_equals_plain(pl):
             sources.add((comp[i].shape[2] > 1, int(batched[i].state[9].item())))
     # the batched rounds took every code source: int32 and int16 hand-offs
     # and, on 3-D fields only, the bit-planes
-    assert {(True, 0), (True, 2)} <= sources and (False, 2) not in sources, sources
-    assert any(s == 1 for _, s in sources), sources
+    assert (True, 2) in sources and (False, 2) not in sources, sources
+    assert {s for _, s in sources} == {0, 1, 2}, sources
