@@ -302,7 +302,8 @@ typedef struct pa_retrieve_job {
     double eb;        /* absolute bound of the stream */
     double rel_eb;    /* relative bound of the stream (0 in absolute mode) */
     double target;    /* absolute target error of this round */
-    int32_t* state;   /* device int32[10]: begin[4], end[4], maxlevel, narrow flag (zeros before round 0) */
+    int32_t* state;   /* device int32[14], zeros before round 0: begin[4], end[4], maxlevel, code source
+                         (0 int32, 1 int16, 2 bit-planes), each level's first live plane of the window */
     uint8_t* bp;      /* device, layout host_size bytes, kept between rounds */
     int32_t* qc;      /* device, code_len int32, kept between rounds */
     void* dws;        /* device, pa_decode_workspace bytes, kept between rounds */
