@@ -220,7 +220,10 @@ int pa_lossless_decode_noalloc(const uint8_t* d_in, int64_t in_len, const pa_lay
                                int32_t* d_err, void* stream);
 
 /* findStrategy_h (dataloader.cu:179-208): updates d_begin/d_end in place and
- * writes d_maxlevel.  eb and target_error are absolute. */
+ * writes d_maxlevel.  eb and target_error are absolute.  One launch; the
+ * workspace holds the search's best key and arrival count: zero-fill it
+ * before its first use (every call leaves it at zero again), and do not
+ * let calls that share a workspace overlap. */
 size_t pa_strategy_workspace(void);
 int pa_find_strategy(const uint64_t* d_cum, int32_t* d_begin, int32_t* d_end,
                      int32_t* d_maxlevel, double eb, double target_error, void* d_ws,

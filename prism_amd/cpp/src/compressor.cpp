@@ -230,6 +230,8 @@ void Compressor<T, E>::init(prism_context* config) {
         total += (ws_len[i] + 255) / 256 * 256;
     }
     check_hip(hipMalloc((void**)&ws, total ? total : 256), "hipMalloc");
+    // the strategy search's best key and arrival count start at zero (its calls leave them there)
+    check_hip(hipMemset(ws + ws_off[3], 0, ws_len[3]), "hipMemset");
     for (hipEvent_t& e : ev) check_hip(hipEventCreate(&e), "hipEventCreate");
 }
 
@@ -448,6 +450,7 @@ void findStrategy_h(size_t* compressedSize_bp_d, int* begin, int* end, double eb
     check_hip(hipEventCreate(&a), "hipEventCreate");
     check_hip(hipEventCreate(&b), "hipEventCreate");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    check_hip(hipMemsetAsync(ws, 0, wsb, st), "hipMemsetAsync");  // the search's best key and count start at zero
     check_hip(hipEventRecord(a, st), "hipEventRecord");
     prism::check_pa(pa_find_strategy(reinterpret_cast<const uint64_t*>(compressedSize_bp_d), begin, end, d_maxlevel, eb,
                                      targetError, ws, wsb, stream),

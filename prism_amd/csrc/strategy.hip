@@ -7,8 +7,11 @@
 // returns the tied index with the smallest bit-reversed value, so the same
 // winner is the maximum of one 64-bit key
 //     value << 22 | (63 - rev6(b0)) << 16 | (1023 - rev10(tid)) << 6 | (63 - k)
-// (pos = tid + 1024k), which any reduction order finds: each workgroup
-// writes the maximum key of its candidates and one workgroup reduces them.
+// (pos = tid + 1024k), which any reduction order finds.  A field's search is
+// one launch of 33 workgroups (one per b0) whose best keys meet in one
+// 64-bit slot by atomic maxima; the last to arrive updates the state.  The
+// batched rounds write one key per (b0, b1) workgroup and reduce them in a
+// second launch.
 #include <rocprim/warp/warp_reduce.hpp>
 
 #include "batch.hpp"
@@ -113,10 +116,74 @@ __device__ __forceinline__ void strategy_search(const uint64_t* __restrict__ cum
     }
 }
 
-__global__ __launch_bounds__(256) void strategy_search_kernel(const uint64_t* __restrict__ cum,
-                                                              uint64_t target_cost,
-                                                              unsigned long long* __restrict__ keys) {
-    strategy_search(cum, target_cost, keys, blockIdx.x, blockIdx.y);
+// One launch: workgroup b0 (of 33) scores, with lane l of 1024, the (b2, b3)
+// = (p / 33, p mod 33) of p = l and l + 1024 (p < 1089) for every b1 (16
+// waves per workgroup: the loop's latency is hidden), and takes its best
+// key into *slot by an atomic maximum.  The arrival count
+// (acquire-release) tells the 33rd workgroup that every maximum is in: it
+// takes the slot, sets slot and count back to zero (zero before the first
+// call, zero after every call) and updates begin / end / maxlevel.
+constexpr int kRowThreads = 1024, kRowPer = (33 * 33 + kRowThreads - 1) / kRowThreads;
+__global__ __launch_bounds__(kRowThreads) void strategy_kernel(const uint64_t* __restrict__ cum,
+                                                               uint64_t target_cost, unsigned long long* slot,
+                                                               uint32_t* count, int32_t* begin, int32_t* end,
+                                                               int32_t* maxlevel) {
+    __shared__ uint64_t s_val[4][33], s_cost[4][33];
+    __shared__ unsigned long long s_best[kRowThreads / 64];
+    const int tid = threadIdx.x, b0 = blockIdx.x;
+    if (tid < 4 * 33) {
+        const int lr = tid / 33, b = tid % 33;
+        uint64_t v;
+        if (b == 0) v = cum[lr * 32 + 31];  // cumulative quirk, dataloader.cu:47
+        else if (b == 32) v = 0;
+        else v = cum[lr * 32 + 31] - cum[lr * 32 + b - 1];
+        s_val[lr][b] = v;
+        s_cost[lr][b] = plane_cost(lr, b);
+    }
+    __syncthreads();
+    uint64_t c23[kRowPer], v23[kRowPer];
+#pragma unroll
+    for (int i = 0; i < kRowPer; ++i) {
+        const int p = tid + kRowThreads * i;
+        c23[i] = 0;
+        v23[i] = 0;
+        if (p < 33 * 33) {
+            c23[i] = s_cost[2][p / 33] + s_cost[3][p % 33];
+            v23[i] = s_val[2][p / 33] + s_val[3][p % 33];
+        }
+    }
+    const uint64_t tie0 = (uint64_t)(63u - (__brev((uint32_t)b0) >> 26)) << 16;
+    unsigned long long best = 0ull;
+    for (int b1 = 0; b1 < 33; ++b1) {
+        const uint64_t c01 = s_cost[0][b0] + s_cost[1][b1];
+        if (c01 > target_cost) continue;
+        const uint64_t v01 = s_val[0][b0] + s_val[1][b1];
+#pragma unroll
+        for (int i = 0; i < kRowPer; ++i) {
+            if (tid + kRowThreads * i >= 33 * 33 || c01 + c23[i] > target_cost) continue;
+            const uint64_t v = v01 + v23[i];
+            if (v == 0) continue;
+            const int pos = b1 * 1089 + tid + kRowThreads * i;  // the reference's thread + 1024 k position
+            const uint64_t key = (v << 22) | tie0 |
+                                 ((uint64_t)(1023u - (__brev((uint32_t)(pos & 1023)) >> 22)) << 6) |
+                                 (uint64_t)(63 - (pos >> 10));
+            if (key > best) best = key;
+        }
+    }
+    best = wave_max(best);
+    if ((tid & 63) == 0) s_best[tid >> 6] = best;
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long k = s_best[0];
+        for (int w = 1; w < kRowThreads / 64; ++w) k = s_best[w] > k ? s_best[w] : k;
+        __hip_atomic_fetch_max(slot, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (__hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 32u) {
+            const unsigned long long kmax =
+                __hip_atomic_exchange(slot, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            strategy_apply(kmax, begin, end, maxlevel);
+        }
+    }
 }
 
 // maximum of the workgroup keys, then the update of begin / end / maxlevel
@@ -132,11 +199,6 @@ __device__ __forceinline__ void strategy_reduce_apply(const unsigned long long* 
         for (int w = 1; w < 4; ++w) k = s_best[w] > k ? s_best[w] : k;
         strategy_apply(k, begin, end, maxlevel);
     }
-}
-
-__global__ __launch_bounds__(256) void strategy_apply_kernel(const unsigned long long* __restrict__ keys,
-                                                             int32_t* begin, int32_t* end, int32_t* maxlevel) {
-    strategy_reduce_apply(keys, begin, end, maxlevel);
 }
 
 // loadall (dataloader.cu:21-28)
@@ -170,7 +232,8 @@ __global__ __launch_bounds__(256) void retr_apply_kernel(const RetrJob* __restri
 
 using namespace pa;
 
-extern "C" size_t pa_strategy_workspace(void) { return sizeof(unsigned long long) * kSearchBlocks; }
+// the best key's slot and the arrival count (zero before the first call)
+extern "C" size_t pa_strategy_workspace(void) { return 2 * sizeof(unsigned long long); }
 
 extern "C" int pa_find_strategy(const uint64_t* d_cum, int32_t* d_begin, int32_t* d_end, int32_t* d_maxlevel,
                                 double eb, double target_error, void* d_ws, size_t ws_bytes, void* stream) {
@@ -181,10 +244,10 @@ extern "C" int pa_find_strategy(const uint64_t* d_cum, int32_t* d_begin, int32_t
         const double q = target_error * 0.954 / eb;
         if (!(q < 1.8e19)) return PA_ERR_ARG;
         const uint64_t target_cost = (uint64_t)__builtin_floor(q) - 1;
-        unsigned long long* keys = static_cast<unsigned long long*>(d_ws);
-        hipLaunchKernelGGL(strategy_search_kernel, dim3(33, 33), dim3(256), 0, st, d_cum, target_cost, keys);
-        PA_LAUNCH_CHECK();
-        hipLaunchKernelGGL(strategy_apply_kernel, dim3(1), dim3(256), 0, st, keys, d_begin, d_end, d_maxlevel);
+        unsigned long long* slot = static_cast<unsigned long long*>(d_ws);
+        uint32_t* count = reinterpret_cast<uint32_t*>(slot + 1);
+        hipLaunchKernelGGL(strategy_kernel, dim3(33), dim3(kRowThreads), 0, st, d_cum, target_cost, slot, count, d_begin,
+                           d_end, d_maxlevel);
         PA_LAUNCH_CHECK();
     } else {
         hipLaunchKernelGGL(load_all_kernel, dim3(1), dim3(64), 0, st, d_begin, d_end, d_maxlevel);

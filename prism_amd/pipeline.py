@@ -456,7 +456,8 @@ class Retriever:
         # state (begin[4] end[4] maxlevel narrow, int32[16]) and the byte
         # counter (int64) in one zero-filled block; views on demand
         self._small = torch.zeros(18, dtype=torch.int32, device=dev)
-        self.sws = _empty(lib().pa_strategy_workspace())
+        # the strategy search's best key and arrival count (zero before first use)
+        self.sws = torch.zeros(lib().pa_strategy_workspace(), dtype=torch.uint8, device=dev)
         self._bufs = None  # bit-planes, codes and decoder workspace, allocated on first use
         self.out = None
         self.rounds = 0

@@ -341,7 +341,7 @@ def test_codec_parity_many_chunks(pa, oracle):
 def test_strategy_parity(pa, oracle):
     _lib, pipe = pa
     rng = np.random.default_rng(5)
-    ws = torch.empty(_lib.lib().pa_strategy_workspace(), dtype=torch.uint8, device="cuda")
+    ws = torch.zeros(_lib.lib().pa_strategy_workspace(), dtype=torch.uint8, device="cuda")
     for trial in range(12):
         announce("strategy", trial)
         sizes = rng.integers(2, 6000, 128) if trial % 3 else rng.choice([2, 4], 128)
@@ -461,7 +461,7 @@ def test_code_handoffs_equal_int32(pa, oracle, shape, dtype, targets):
     qc16 = torch.empty(L.code_len, dtype=torch.int32, device="cuda")
     src16 = torch.zeros(1, dtype=torch.int32, device="cuda")
     dws = torch.empty(_lib.lib().pa_decode_workspace(C.byref(L)), dtype=torch.uint8, device="cuda")
-    sws = torch.empty(_lib.lib().pa_strategy_workspace(), dtype=torch.uint8, device="cuda")
+    sws = torch.zeros(_lib.lib().pa_strategy_workspace(), dtype=torch.uint8, device="cuda")
     old = old16 = None
     lib = _lib.lib()
     for k, t in enumerate(targets):
