@@ -1130,44 +1130,59 @@ __device__ __forceinline__ void plane_row_store(T* s, const PlaneRow& r, int b) 
     }
 }
 
-// Every code slot and corner anchor of the tile from the bit-planes.  The
-// first rows of all lanes are level 0's; the second rows are the rest of
-// level 0, then levels 1-3 (each level's rows with its lattice sizes known
-// at compile time).
+// Every code slot and corner anchor of the tile from the bit-planes, in two
+// halves (the loads, then the slot writes) so that other loads can be
+// issued between them.  The first rows of all lanes are level 0's; the
+// second rows are the rest of level 0, then levels 1-3 (each level's rows
+// with its lattice sizes known at compile time).
+template <typename T>
+struct PlaneCodes {
+    PlaneRow r0, r1;
+    int wb[4];  // per level lr = 3 - L
+    T anc;
+    int aslot;
+};
+
 template <typename T, bool IN>
-__device__ __forceinline__ void plane_codes(const Ctx<T, IN>& t, const PlaneSrc& ps, const int32_t* rowtab,
-                                            int maxlevel, const T* anchors_in, int bx, int by, int bz) {
+__device__ __forceinline__ void plane_codes_load(const Ctx<T, IN>& t, const PlaneSrc& ps, const int32_t* rowtab,
+                                                 int maxlevel, const T* anchors_in, int bx, int by, int bz,
+                                                 PlaneCodes<T>& c) {
     const Geom& g = t.g;
-    int wb[4], we[4];  // per level lr = 3 - L
+    int we[4];
 #pragma unroll
     for (int lr = 0; lr < 4; ++lr) {
-        wb[lr] = ps.begin[lr];
+        c.wb[lr] = ps.begin[lr];
         we[lr] = ps.end[lr];
     }
     const int lane = lane_id();
     constexpr int k1 = kRows0 + kRows1, k2 = k1 + kRows2;
     const int rg = lane + kThreads;
-    PlaneRow r0, r1;
-    plane_row_load<0>(t, ps, rowtab, lane, maxlevel, wb[3], we[3], r0);
-    if (rg < kRows0) plane_row_load<0>(t, ps, rowtab, rg, maxlevel, wb[3], we[3], r1);
-    else if (rg < k1) plane_row_load<1>(t, ps, rowtab, rg - kRows0, maxlevel, wb[2], we[2], r1);
-    else if (rg < k2) plane_row_load<2>(t, ps, rowtab, rg - k1, maxlevel, wb[1], we[1], r1);
-    else if (rg < kRowTab) plane_row_load<3>(t, ps, rowtab, rg - k2, maxlevel, wb[0], we[0], r1);
-    T anc = (T)0;
-    int aslot = -1;
+    plane_row_load<0>(t, ps, rowtab, lane, maxlevel, c.wb[3], we[3], c.r0);
+    if (rg < kRows0) plane_row_load<0>(t, ps, rowtab, rg, maxlevel, c.wb[3], we[3], c.r1);
+    else if (rg < k1) plane_row_load<1>(t, ps, rowtab, rg - kRows0, maxlevel, c.wb[2], we[2], c.r1);
+    else if (rg < k2) plane_row_load<2>(t, ps, rowtab, rg - k1, maxlevel, c.wb[1], we[1], c.r1);
+    else if (rg < kRowTab) plane_row_load<3>(t, ps, rowtab, rg - k2, maxlevel, c.wb[0], we[0], c.r1);
+    c.anc = (T)0;
+    c.aslot = -1;
     if (threadIdx.x < 8) {
         const int cx = threadIdx.x & 1, cy = (threadIdx.x >> 1) & 1, cz = threadIdx.x >> 2;
         const int ax = bx + cx, ay = by + cy, az = bz + cz;
-        aslot = sidx(16 * cx, 16 * cy, 16 * cz);
+        c.aslot = sidx(16 * cx, 16 * cy, 16 * cz);
         if (anchors_in && ax < g.gx && ay < g.gy && az < g.gz)
-            anc = anchors_in[ax + (int64_t)ay * g.gx + (int64_t)az * g.gx * g.gy];
+            c.anc = anchors_in[ax + (int64_t)ay * g.gx + (int64_t)az * g.gx * g.gy];
     }
-    plane_row_store<0>(t.s, r0, wb[3]);
-    if (rg < kRows0) plane_row_store<0>(t.s, r1, wb[3]);
-    else if (rg < k1) plane_row_store<1>(t.s, r1, wb[2]);
-    else if (rg < k2) plane_row_store<2>(t.s, r1, wb[1]);
-    else if (rg < kRowTab) plane_row_store<3>(t.s, r1, wb[0]);
-    if (aslot >= 0) t.s[aslot] = anc;
+}
+
+template <typename T, bool IN>
+__device__ __forceinline__ void plane_codes_store(const Ctx<T, IN>& t, const PlaneCodes<T>& c) {
+    constexpr int k1 = kRows0 + kRows1, k2 = k1 + kRows2;
+    const int rg = lane_id() + kThreads;
+    plane_row_store<0>(t.s, c.r0, c.wb[3]);
+    if (rg < kRows0) plane_row_store<0>(t.s, c.r1, c.wb[3]);
+    else if (rg < k1) plane_row_store<1>(t.s, c.r1, c.wb[2]);
+    else if (rg < k2) plane_row_store<2>(t.s, c.r1, c.wb[1]);
+    else if (rg < kRowTab) plane_row_store<3>(t.s, c.r1, c.wb[0]);
+    if (c.aslot >= 0) t.s[c.aslot] = c.anc;
 }
 
 // The previous reconstruction of the tile (delta form); lane: (x, y) =
@@ -1225,7 +1240,10 @@ __device__ __forceinline__ void tile_levels(Ctx<T, IN>& t, const LevelEb<T>& le,
 // round's decoded bit-planes.
 enum CodeSrc { CODES_32, CODES_16, CODES_PLANES };
 
-template <typename T, int M, bool IN, int CS = CODES_32>
+// DELTA: a delta round (a.old given), a compile-time fact so that the
+// previous values' loads are unconditional in interior tiles and the code
+// slots wait for the code loads alone
+template <typename T, int M, bool IN, int CS = CODES_32, bool DELTA = false>
 __device__ __forceinline__ void tile_run(const SplineArgs<T>& a, T* tile, int32_t* rowtab, int bx, int by, int bz) {
     const Geom& g = a.g;
     Ctx<T, IN> t = tile_ctx<T, IN>(a, tile, bx, by, bz, CS == CODES_16);
@@ -1234,30 +1252,32 @@ __device__ __forceinline__ void tile_run(const SplineArgs<T>& a, T* tile, int32_
     for (int i = 0; i < 8; ++i) fl[i] = a.flags8[i];
     const int maxlevel = a.d_maxlevel ? *a.d_maxlevel : 4;
 
-    // decompression: the previous reconstruction (delta form) and every
-    // code of the tile are requested before any is needed
+    // decompression: every code of the tile and the previous reconstruction
+    // (delta form) are requested before any is needed, the codes first:
+    // loads return in issue order, so the code slots are filled while the
+    // previous values (needed only at the output) are still in flight
     T prev[kOutIters];
-    if constexpr (M == DECOMPRESS) {
-        // a raised wave priority while the tile's loads are issued puts them in
-        // flight ahead of other workgroups' stage arithmetic (reconstruction
-        // 1-2 us faster)
-        __builtin_amdgcn_s_setprio(3);
-        if (a.old) prev_load(t, a.old, prev);
-    }
-
     if constexpr (M == COMPRESS) {
         load_tile(t, a.data);
         __syncthreads();
         if (threadIdx.x == 0) a.anchors[bx + (int64_t)by * g.gx + (int64_t)bz * g.gx * g.gy] = tile[0];
     } else {
+        // a raised wave priority while the tile's loads are issued puts them in
+        // flight ahead of other workgroups' stage arithmetic (reconstruction
+        // 1-2 us faster)
+        __builtin_amdgcn_s_setprio(3);
         fill_rows(t, rowtab);
         lds_barrier();
         if constexpr (CS == CODES_PLANES) {
-            plane_codes(t, a.ps, rowtab, maxlevel, a.anchors_in, bx, by, bz);
+            PlaneCodes<T> c;
+            plane_codes_load(t, a.ps, rowtab, maxlevel, a.anchors_in, bx, by, bz, c);
+            if constexpr (DELTA) prev_load(t, a.old, prev);
             __builtin_amdgcn_s_setprio(0);
+            plane_codes_store(t, c);
         } else {
             TileCodes<T> c;
             codes_load(t, rowtab, maxlevel, a.anchors_in, bx, by, bz, c);
+            if constexpr (DELTA) prev_load(t, a.old, prev);
             __builtin_amdgcn_s_setprio(0);
             codes_store(tile, c);
         }
@@ -1266,7 +1286,15 @@ __device__ __forceinline__ void tile_run(const SplineArgs<T>& a, T* tile, int32_
 
     tile_levels<T, M>(t, a.le, fl, maxlevel);
 
-    if constexpr (M == DECOMPRESS) tile_out(t, a.old != nullptr, prev, a.out);
+    if constexpr (M == DECOMPRESS) tile_out(t, DELTA, prev, a.out);
+}
+
+// a decompressing tile of either form: delta (a.old given) or anchored
+template <typename T, bool IN, int CS>
+__device__ __forceinline__ void tile_run_dec(const SplineArgs<T>& a, T* tile, int32_t* rowtab, int bx, int by,
+                                             int bz) {
+    if (a.old) tile_run<T, DECOMPRESS, IN, CS, true>(a, tile, rowtab, bx, by, bz);
+    else tile_run<T, DECOMPRESS, IN, CS, false>(a, tile, rowtab, bx, by, bz);
 }
 
 // ------------------------------------------------------------ 2-D fields --
@@ -1675,18 +1703,21 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     const SplineArgs<T> r = resolve(a);
     if constexpr (M == DECOMPRESS) {
         if (r.planes) {
-            if (interior) tile_run<T, M, true, CODES_PLANES>(r, tile, rowtab, bx, by, bz);
-            else tile_run<T, M, false, CODES_PLANES>(r, tile, rowtab, bx, by, bz);
+            if (interior) tile_run_dec<T, true, CODES_PLANES>(r, tile, rowtab, bx, by, bz);
+            else tile_run_dec<T, false, CODES_PLANES>(r, tile, rowtab, bx, by, bz);
             return;
         }
         if (r.q16) {
-            if (interior) tile_run<T, M, true, CODES_16>(r, tile, rowtab, bx, by, bz);
-            else tile_run<T, M, false, CODES_16>(r, tile, rowtab, bx, by, bz);
+            if (interior) tile_run_dec<T, true, CODES_16>(r, tile, rowtab, bx, by, bz);
+            else tile_run_dec<T, false, CODES_16>(r, tile, rowtab, bx, by, bz);
             return;
         }
+        if (interior) tile_run_dec<T, true, CODES_32>(r, tile, rowtab, bx, by, bz);
+        else tile_run_dec<T, false, CODES_32>(r, tile, rowtab, bx, by, bz);
+    } else {
+        if (interior) tile_run<T, M, true>(r, tile, rowtab, bx, by, bz);
+        else tile_run<T, M, false>(r, tile, rowtab, bx, by, bz);
     }
-    if (interior) tile_run<T, M, true>(r, tile, rowtab, bx, by, bz);
-    else tile_run<T, M, false>(r, tile, rowtab, bx, by, bz);
 }
 
 
@@ -2130,17 +2161,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     const int bx = (int)(tix % g.gx), by = (int)((tix / g.gx) % g.gy), bz = (int)(tix / ((int64_t)g.gx * g.gy));
     const bool interior = bx * 16 + 16 < g.nx && by * 16 + 16 < g.ny && bz * 16 + 16 < g.nz;
     if (a.planes) {
-        if (interior) tile_run<T, DECOMPRESS, true, CODES_PLANES>(a, tile, rowtab, bx, by, bz);
-        else tile_run<T, DECOMPRESS, false, CODES_PLANES>(a, tile, rowtab, bx, by, bz);
+        if (interior) tile_run_dec<T, true, CODES_PLANES>(a, tile, rowtab, bx, by, bz);
+        else tile_run_dec<T, false, CODES_PLANES>(a, tile, rowtab, bx, by, bz);
         return;
     }
     if (a.q16) {
-        if (interior) tile_run<T, DECOMPRESS, true, CODES_16>(a, tile, rowtab, bx, by, bz);
-        else tile_run<T, DECOMPRESS, false, CODES_16>(a, tile, rowtab, bx, by, bz);
+        if (interior) tile_run_dec<T, true, CODES_16>(a, tile, rowtab, bx, by, bz);
+        else tile_run_dec<T, false, CODES_16>(a, tile, rowtab, bx, by, bz);
         return;
     }
-    if (interior) tile_run<T, DECOMPRESS, true>(a, tile, rowtab, bx, by, bz);
-    else tile_run<T, DECOMPRESS, false>(a, tile, rowtab, bx, by, bz);
+    if (interior) tile_run_dec<T, true, CODES_32>(a, tile, rowtab, bx, by, bz);
+    else tile_run_dec<T, false, CODES_32>(a, tile, rowtab, bx, by, bz);
 }
 
 // ------------------------------------------------------------- host side --
