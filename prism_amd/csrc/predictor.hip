@@ -1048,8 +1048,8 @@ typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef u32x2 u32x2_a4 __attribute__((aligned(4)));
 
 template <int L, typename T, bool IN>
-__device__ __forceinline__ void plane_row_load(const Ctx<T, IN>& t, const PlaneSrc& ps, const int32_t* tab, int row,
-                                               int maxlevel, int b, int e, PlaneRow& r) {
+__device__ __forceinline__ void plane_row_load(const Ctx<T, IN>& t, const PlaneSrc& ps, int row, int maxlevel, int b,
+                                               int e, PlaneRow& r) {
     constexpr int U = 1 << L, E = 16 / U + 1;
     const int j = row % E, k = row / E;
     r.row = row;
@@ -1069,9 +1069,11 @@ __device__ __forceinline__ void plane_row_load(const Ctx<T, IN>& t, const PlaneS
 #pragma unroll
     for (int q = 0; q < kPlaneWindow; ++q) r.w[q] = 0u;
     if (r.np == 0) return;
-    // the run's first code, relative to the level's segment (32-bit, as the
-    // row table: the code array is shorter than 2^31)
-    const uint32_t start = (uint32_t)tab[row_off<L>() + row] + (uint32_t)(r.full ? t.ox >> L : t.ox >> (L + 1));
+    // the run's first code, relative to the level's segment (32-bit: the
+    // code array is shorter than 2^31); the row base computed here, so the
+    // loads need no row table
+    const uint32_t start = (uint32_t)row_base<L>(t.g, t.oy + U * j, t.oz + U * k) +
+                           (uint32_t)(r.full ? t.ox >> L : t.ox >> (L + 1));
     const uint32_t rel = start - (uint32_t)t.g.prefix[L];
     r.sh = (int)(rel & 7u);
     const uint32_t byte = rel >> 3;
@@ -1144,9 +1146,8 @@ struct PlaneCodes {
 };
 
 template <typename T, bool IN>
-__device__ __forceinline__ void plane_codes_load(const Ctx<T, IN>& t, const PlaneSrc& ps, const int32_t* rowtab,
-                                                 int maxlevel, const T* anchors_in, int bx, int by, int bz,
-                                                 PlaneCodes<T>& c) {
+__device__ __forceinline__ void plane_codes_load(const Ctx<T, IN>& t, const PlaneSrc& ps, int maxlevel,
+                                                 const T* anchors_in, int bx, int by, int bz, PlaneCodes<T>& c) {
     const Geom& g = t.g;
     int we[4];
 #pragma unroll
@@ -1157,11 +1158,11 @@ __device__ __forceinline__ void plane_codes_load(const Ctx<T, IN>& t, const Plan
     const int lane = lane_id();
     constexpr int k1 = kRows0 + kRows1, k2 = k1 + kRows2;
     const int rg = lane + kThreads;
-    plane_row_load<0>(t, ps, rowtab, lane, maxlevel, c.wb[3], we[3], c.r0);
-    if (rg < kRows0) plane_row_load<0>(t, ps, rowtab, rg, maxlevel, c.wb[3], we[3], c.r1);
-    else if (rg < k1) plane_row_load<1>(t, ps, rowtab, rg - kRows0, maxlevel, c.wb[2], we[2], c.r1);
-    else if (rg < k2) plane_row_load<2>(t, ps, rowtab, rg - k1, maxlevel, c.wb[1], we[1], c.r1);
-    else if (rg < kRowTab) plane_row_load<3>(t, ps, rowtab, rg - k2, maxlevel, c.wb[0], we[0], c.r1);
+    plane_row_load<0>(t, ps, lane, maxlevel, c.wb[3], we[3], c.r0);
+    if (rg < kRows0) plane_row_load<0>(t, ps, rg, maxlevel, c.wb[3], we[3], c.r1);
+    else if (rg < k1) plane_row_load<1>(t, ps, rg - kRows0, maxlevel, c.wb[2], we[2], c.r1);
+    else if (rg < k2) plane_row_load<2>(t, ps, rg - k1, maxlevel, c.wb[1], we[1], c.r1);
+    else if (rg < kRowTab) plane_row_load<3>(t, ps, rg - k2, maxlevel, c.wb[0], we[0], c.r1);
     c.anc = (T)0;
     c.aslot = -1;
     if (threadIdx.x < 8) {
@@ -1266,11 +1267,13 @@ __device__ __forceinline__ void tile_run(const SplineArgs<T>& a, T* tile, int32_
         // flight ahead of other workgroups' stage arithmetic (reconstruction
         // 1-2 us faster)
         __builtin_amdgcn_s_setprio(3);
-        fill_rows(t, rowtab);
-        lds_barrier();
+        if constexpr (CS != CODES_PLANES) {  // the plane loads compute their row bases themselves
+            fill_rows(t, rowtab);
+            lds_barrier();
+        }
         if constexpr (CS == CODES_PLANES) {
             PlaneCodes<T> c;
-            plane_codes_load(t, a.ps, rowtab, maxlevel, a.anchors_in, bx, by, bz, c);
+            plane_codes_load(t, a.ps, maxlevel, a.anchors_in, bx, by, bz, c);
             if constexpr (DELTA) prev_load(t, a.old, prev);
             __builtin_amdgcn_s_setprio(0);
             plane_codes_store(t, c);
