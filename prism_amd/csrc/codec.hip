@@ -757,6 +757,17 @@ __device__ __forceinline__ void bytes8(const uint32_t* w, int b, uint32_t& lo, u
     hi = __builtin_amdgcn_alignbyte(w2, w1, o);
 }
 
+// The thread id through an opaque move: the per-lane index arithmetic
+// derived from it is redone for each chunk of a workgroup's loop instead of
+// being hoisted out and held in registers across it (which left room for 4
+// workgroups per CU instead of the 7 the LDS allows).
+__device__ __forceinline__ int opaque_tid() {
+    int v;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "v"((int)threadIdx.x));
+    __builtin_assume(v >= 0 && v < kThreads);
+    return v;
+}
+
 __device__ void decode_chunk(const uint8_t* __restrict__ in, const ChunkGeom& g, const int64_t* __restrict__ offsets,
                              uint8_t* __restrict__ bp, int32_t* __restrict__ err, int64_t in_len, int64_t id,
                              DecodeLds& S) {
@@ -765,7 +776,7 @@ __device__ void decode_chunk(const uint8_t* __restrict__ in, const ChunkGeom& g,
     uint8_t* s_b1 = S.b1;
     uint8_t* s_b2 = S.b2;
     int* s_scan = S.scan;
-    const int tid = threadIdx.x;
+    const int tid = opaque_tid();
     const ChunkPos cp = chunk_pos(g, id);
     const int osize = cp.osize;
     const uint16_t* sizes = reinterpret_cast<const uint16_t*>(in + 8);
