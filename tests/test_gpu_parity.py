@@ -356,6 +356,8 @@ def test_strategy_parity(pa, oracle):
                                                     ws.numel(), S()))
             h = host(st)
             assert list(h[0:4]) == list(b_o) and list(h[4:8]) == list(e_o) and h[8] == ml_o, (trial, target)
+            # the search's best key and arrival count are back at zero for the next call
+            assert not host(ws).any(), (trial, target)
 
 
 CONFIGS = [((33, 47, 29), np.float32, 1e-3), ((16, 16, 16), np.float64, 1e-2), ((1, 1, 5000), np.float64, 1e-4),
