@@ -796,6 +796,12 @@ __device__ void decode_chunk(const uint8_t* __restrict__ in, const ChunkGeom& g,
         for (int i = tid; i < nwords; i += kThreads) dst[i] = 0ull;
         return;
     }
+    if (csize == 2 && osize > 2) {  // an empty chunk, its footer alone: zeros, no staging
+        const int orig = (int)cbg[0] | ((int)cbg[1] << 8);
+        if (orig != osize && tid == 0) atomicOr(err, 2);
+        for (int i = tid; i < nwords; i += kThreads) dst[i] = 0ull;
+        return;
+    }
     const int shift = (int)((uintptr_t)cbg & 3);
     const uint32_t* aw = reinterpret_cast<const uint32_t*>(cbg - shift);
     const int cw = (shift + csize + 3) >> 2;
